@@ -1,0 +1,183 @@
+"""Benchmark: FP64 plain-DEM particle-steps/s at 4M spheres on one MI355X.
+
+Workload (plain_large): 4,000,000 polydisperse spheres on a jittered lattice in
+a six-wall box, gravity, dt = 1e-6 s, cutoff 2.6 mm, rebuild every 20 steps.
+--warmup steps settle the packing (untimed; 2,000 by default), then --steps
+full time steps are timed (10,000 by default).  Inputs are resident in HBM
+before the timed region.  One JSON line on stdout.
+
+Also reported:
+  roofline     the fused step kernel's compulsory HBM bytes per launch over
+               its mean duration (HIP events on the library's stream).
+  cpu_baseline the CPU oracle (OpenMP, this host's cores) on a bounded sample
+               of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(dims, steps_cadence, steps_faithful):
+    """Times the CPU oracle (the reference restated in C++/OpenMP) on the same
+    workload's initial state: rebuild every 20 steps, and every step as the
+    reference does."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle_py as O
+    from cabanadem_amd import workloads as W
+
+    wl = W.plain_large(dims=dims)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    prm = O.params(dt=wl.dt, cor_pp=wl.cor, cor_pw=wl.cor, fr_pp=wl.friction,
+                   fr_pw=wl.friction, gravity=wl.gravity, mode=0)
+    st = wl.state
+    _, t1 = O.run(st, wl.walls, prm, nsteps=steps_cadence, rc=wl.cutoff, lo=wl.lo, hi=wl.hi,
+                  rebuild_every=wl.rebuild_every, max_n=64, threads=threads)
+    st = W.plain_large(dims=dims).state
+    _, t2 = O.run(st, wl.walls, prm, nsteps=steps_faithful, rc=wl.cutoff, lo=wl.lo, hi=wl.hi,
+                  rebuild_every=1, max_n=64, threads=threads)
+    n = st.n
+    return {
+        "value": n * steps_cadence / t1,
+        "unit": "particle-steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"{n} spheres of the same workload from its initial state: {steps_cadence} "
+                   f"steps rebuilding every {wl.rebuild_every} ({t1:.1f} s); rebuilding every "
+                   f"step as the reference does: {n * steps_faithful / t2:.4g} particle-steps/s "
+                   f"over {steps_faithful} steps ({t2:.1f} s)"),
+        "value_rebuild_every_step": n * steps_faithful / t2,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--warmup", type=int, default=2000)
+    ap.add_argument("--dims", type=str, default="200,200,100")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=str, default="60,10")
+    ap.add_argument("--mode", type=int, default=0, help="0 reference arithmetic, 1 corrected")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(the design for sharding is described in DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    from cabanadem_amd import abi
+    from cabanadem_amd import workloads as W
+
+    dims = tuple(int(v) for v in args.dims.split(","))
+    t0 = time.perf_counter()
+    wl = W.plain_large(dims=dims)
+    n = wl.state.n
+    sysm = wl.system(mode=args.mode)
+    log(f"[bench] {n} spheres generated and uploaded in {time.perf_counter() - t0:.1f} s")
+
+    every = 500
+    sysm.set_diag_every(every)
+    if args.warmup > 0:
+        t0 = time.perf_counter()
+        sysm.step(args.warmup)
+        log(f"[bench] warmup {args.warmup} steps in {time.perf_counter() - t0:.2f} s")
+    nrows0 = sysm.diag_history().shape[0]
+
+    sysm.set_timing(True)
+    sysm.sync()
+    t0 = time.perf_counter()
+    sysm.step(args.steps)
+    sysm.sync()
+    elapsed = time.perf_counter() - t0
+    ms, launches = sysm.timing()
+    sysm.set_timing(False)
+    log(f"[bench] timed {args.steps} steps in {elapsed:.3f} s")
+
+    diag = sysm.diag_history()[nrows0:]
+    nbar = float(diag[:, 6].mean()) if len(diag) else float("nan")
+    cbar = float(diag[:, 7].mean()) if len(diag) else float("nan")
+    wbar = float(diag[:, 8].mean()) if len(diag) else float("nan")
+    cnt = sysm.counters()
+
+    value = n * args.steps / elapsed
+    step_ms = elapsed / args.steps * 1e3
+    k_ms = ms[0] / max(launches[0], 1)
+    r_ms = ms[1] / max(launches[1], 1)
+    b_kernel = W.bytes_fused_step(nbar, cbar, wbar)
+    achieved = b_kernel * n / (k_ms * 1e-3) / 1e9
+    b_ref = W.bytes_reference_model(nbar, cbar, wbar, wl.rebuild_every)
+
+    out = {
+        "metric": "FP64 DEM particle-steps/s at 4M spheres",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": step_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (jittered-lattice packing from a seeded counter-based generator)",
+        "config": {
+            "workload": "plain DEM, large: %d polydisperse spheres (%dx%dx%d lattice), "
+                        "six-wall box, gravity, settle %d + timed %d steps" % (n, *dims, args.warmup, args.steps),
+            "spheres": n, "dt": wl.dt, "cutoff": wl.cutoff, "rebuild_every": wl.rebuild_every,
+            "physics_mode": "reference" if args.mode == 0 else "corrected",
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "fused step kernel (forces + stage3 + next stage1/stage2)",
+            "bytes_per_particle_step": b_kernel,
+            "kernel_ms": k_ms,
+            "launches": launches[0],
+        },
+        "step_model": {
+            "bytes_per_particle_step_reference_model": b_ref,
+            "frac_of_hbm_reference_model": b_ref * value / (HBM_PEAK_GBS * 1e9),
+            "rebuild_ms": r_ms,
+            "rebuilds": launches[1],
+        },
+        "contacts": {"mean_neighbors": nbar, "mean_contacts": cbar, "mean_wall_contacts": wbar,
+                     "max_neighbors": cnt.max_neighbors_seen,
+                     "contact_pairs_per_s": cbar * value},
+    }
+    sysm.close()
+    del sysm
+
+    if not args.no_cpu_baseline:
+        cs, cf = (int(v) for v in args.cpu_steps.split(","))
+        log(f"[bench] CPU baseline: {cs} + {cf} oracle steps on {n} spheres")
+        t0 = time.perf_counter()
+        out["cpu_baseline"] = cpu_baseline(dims, cs, cf)
+        log(f"[bench] CPU baseline done in {time.perf_counter() - t0:.1f} s")
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

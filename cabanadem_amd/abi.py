@@ -1,0 +1,323 @@
+"""ctypes binding of include/cabanadem.h (libcabanadem_hip.so).
+
+This is plumbing for the Python tests and the benchmark: every compute call
+goes through the C ABI into the HIP kernels.  There is no Python or CPU
+fallback; if the library is missing, importing this module raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "lib" / "libcabanadem_hip.so"
+
+DEM_MAX_WALLS = 8
+DEM_OK = 0
+ERRORS = {
+    -1: "DEM_ERR_ARG",
+    -2: "DEM_ERR_HIP",
+    -3: "DEM_ERR_NEIGH_OVERFLOW",
+    -4: "DEM_ERR_CONTACT_OVERFLOW",
+    -5: "DEM_ERR_NONFINITE",
+    -6: "DEM_ERR_SKIN",
+    -7: "DEM_ERR_BOND_OVERFLOW",
+    -8: "DEM_ERR_STATE",
+}
+MODE_REFERENCE = 0
+MODE_CORRECTED = 1
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class DemConfig(C.Structure):
+    _fields_ = [
+        ("n", C.c_int), ("n_walls", C.c_int), ("max_neighbors", C.c_int), ("max_contacts", C.c_int),
+        ("dt", C.c_double), ("cor_pp", C.c_double), ("cor_pw", C.c_double),
+        ("friction_pp", C.c_double), ("friction_pw", C.c_double),
+        ("gravity", C.c_double * 3), ("cutoff", C.c_double), ("skin", C.c_double),
+        ("lo", C.c_double * 3), ("hi", C.c_double * 3),
+        ("rebuild_every", C.c_int), ("physics_mode", C.c_int), ("block_size", C.c_int),
+        ("reserved", C.c_int),
+    ]
+
+
+class DemParticlesView(C.Structure):
+    _fields_ = [
+        ("n", C.c_int), ("max_cnts", C.c_int), ("max_walls", C.c_int), ("reserved", C.c_int),
+        ("x", _dp), ("u", _dp), ("omega", _dp), ("force", _dp), ("torque", _dp),
+        ("m", _dp), ("rad", _dp), ("E", _dp), ("nu", _dp), ("G", _dp), ("I", _dp),
+        ("sw_x", _dp), ("sw_y", _dp), ("sw_z", _dp),
+        ("ss_x", _dp), ("ss_y", _dp), ("ss_z", _dp),
+        ("ss_idx", _ip), ("ss_cnt", _ip),
+    ]
+
+
+class DemWallsView(C.Structure):
+    _fields_ = [
+        ("n", C.c_int), ("reserved", C.c_int),
+        ("x", _dp), ("normal", _dp), ("E", _dp), ("nu", _dp), ("G", _dp),
+    ]
+
+
+class DemCounters(C.Structure):
+    _fields_ = [
+        ("steps", C.c_int64), ("rebuilds", C.c_int64),
+        ("max_neighbors_seen", C.c_int), ("neighbor_capacity", C.c_int),
+        ("mean_neighbors", C.c_double), ("mean_contacts", C.c_double),
+        ("mean_wall_contacts", C.c_double),
+    ]
+
+
+SYMBOLS = {
+    "dem_create": (C.c_int, [C.POINTER(DemConfig), C.POINTER(C.c_void_p)]),
+    "dem_destroy": (None, [C.c_void_p]),
+    "dem_last_error": (C.c_char_p, [C.c_void_p]),
+    "dem_abi_version": (C.c_int, []),
+    "dem_device_count": (C.c_int, []),
+    "dem_upload_particles": (C.c_int, [C.c_void_p, C.POINTER(DemParticlesView)]),
+    "dem_download_particles": (C.c_int, [C.c_void_p, C.POINTER(DemParticlesView)]),
+    "dem_upload_walls": (C.c_int, [C.c_void_p, C.POINTER(DemWallsView)]),
+    "dem_build_neighbors": (C.c_int, [C.c_void_p]),
+    "dem_get_neighbors": (C.c_int, [C.c_void_p, _ip, _ip, C.c_int]),
+    "dem_zero_forces": (C.c_int, [C.c_void_p]),
+    "dem_update_tangential": (C.c_int, [C.c_void_p]),
+    "dem_force_pp": (C.c_int, [C.c_void_p]),
+    "dem_force_pw": (C.c_int, [C.c_void_p]),
+    "dem_compute_forces": (C.c_int, [C.c_void_p]),
+    "dem_stage1": (C.c_int, [C.c_void_p]),
+    "dem_stage2": (C.c_int, [C.c_void_p]),
+    "dem_stage3": (C.c_int, [C.c_void_p]),
+    "dem_step": (C.c_int, [C.c_void_p, C.c_int]),
+    "dem_set_diag_every": (C.c_int, [C.c_void_p, C.c_int]),
+    "dem_get_diag_history": (C.c_int, [C.c_void_p, _dp, C.c_int, _ip]),
+    "dem_diagnostics": (C.c_int, [C.c_void_p, _dp]),
+    "dem_get_counters": (C.c_int, [C.c_void_p, C.POINTER(DemCounters)]),
+    "dem_synchronize": (C.c_int, [C.c_void_p]),
+    "dem_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "dem_get_timing": (C.c_int, [C.c_void_p, _dp, C.POINTER(C.c_int64)]),
+    "dem_generate_lattice": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, _dp, C.c_double,
+                                       C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                       C.c_double, C.c_uint64, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                       _dp]),
+}
+
+
+def load_library(path: Path | str | None = None) -> C.CDLL:
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(
+            f"{p} is missing: build it with `python cabanadem_amd/build.py lib` "
+            "(there is no CPU fallback for the DEM kernels)")
+    lib = C.CDLL(str(p))
+    for name, (res, args) in SYMBOLS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+_LIB: C.CDLL | None = None
+
+
+def lib() -> C.CDLL:
+    global _LIB
+    if _LIB is None:
+        _LIB = load_library()
+    return _LIB
+
+
+class DemError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def _ptr(a, ct=C.c_double):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def generate_lattice(nx, ny, nz, spacing, lo, jitter, rmin, rmax, vsigma, rho, E, nu, seed):
+    """Jittered lattice packing from the library's host generator (no GPU needed)."""
+    n = nx * ny * nz
+    out = {k: np.zeros(3 * n if k in ("x", "u") else n) for k in
+           ("x", "u", "rad", "m", "I", "E", "nu", "G")}
+    lo3 = np.asarray(lo, dtype=np.float64)
+    rc = lib().dem_generate_lattice(nx, ny, nz, spacing, _ptr(lo3), jitter, rmin, rmax, vsigma,
+                                    rho, E, nu, seed, _ptr(out["x"]), _ptr(out["u"]),
+                                    _ptr(out["rad"]), _ptr(out["m"]), _ptr(out["I"]),
+                                    _ptr(out["E"]), _ptr(out["nu"]), _ptr(out["G"]))
+    if rc != 0:
+        raise DemError(rc, "dem_generate_lattice failed")
+    out["x"] = out["x"].reshape(n, 3)
+    out["u"] = out["u"].reshape(n, 3)
+    return out
+
+
+class ParticleState:
+    """Host particle state in the reference layout (xyz interleaved, row-major histories)."""
+
+    def __init__(self, n: int, max_cnts: int = 16, max_walls: int = 6):
+        self.n, self.max_cnts, self.max_walls = n, max_cnts, max_walls
+        z3 = lambda: np.zeros((n, 3))
+        self.x, self.u, self.omega, self.force, self.torque = z3(), z3(), z3(), z3(), z3()
+        self.m, self.rad, self.E, self.nu, self.G, self.I = (np.ones(n) for _ in range(6))
+        self.sw_x, self.sw_y, self.sw_z = (np.zeros((n, max_walls)) for _ in range(3))
+        self.ss_x, self.ss_y, self.ss_z = (np.zeros((n, max_cnts)) for _ in range(3))
+        self.ss_idx = -np.ones((n, max_cnts), dtype=np.int32)
+        self.ss_cnt = np.zeros(n, dtype=np.int32)
+
+    FIELDS = ("x", "u", "omega", "force", "torque", "m", "rad", "E", "nu", "G", "I",
+              "sw_x", "sw_y", "sw_z", "ss_x", "ss_y", "ss_z", "ss_idx", "ss_cnt")
+
+    def copy(self) -> "ParticleState":
+        c = ParticleState(self.n, self.max_cnts, self.max_walls)
+        for f in self.FIELDS:
+            setattr(c, f, getattr(self, f).copy())
+        return c
+
+    def view(self, fields=None) -> DemParticlesView:
+        for f in self.FIELDS:
+            a = getattr(self, f)
+            want = np.int32 if f in ("ss_idx", "ss_cnt") else np.float64
+            if a.dtype != want or not a.flags.c_contiguous:
+                setattr(self, f, np.ascontiguousarray(a, dtype=want))
+        v = DemParticlesView()
+        v.n, v.max_cnts, v.max_walls = self.n, self.max_cnts, self.max_walls
+        for f in self.FIELDS:
+            if fields is not None and f not in fields:
+                continue
+            ct = C.c_int if f in ("ss_idx", "ss_cnt") else C.c_double
+            setattr(v, f, _ptr(getattr(self, f), ct))
+        return v
+
+
+class Walls:
+    def __init__(self, x, normal, E, nu, G):
+        self.x = np.ascontiguousarray(np.asarray(x, dtype=np.float64).reshape(-1, 3))
+        self.normal = np.ascontiguousarray(np.asarray(normal, dtype=np.float64).reshape(-1, 3))
+        n = self.x.shape[0]
+        f = lambda v: np.ascontiguousarray(np.broadcast_to(np.asarray(v, dtype=np.float64), (n,)))
+        self.E, self.nu, self.G = f(E), f(nu), f(G)
+
+    @property
+    def n(self):
+        return self.x.shape[0]
+
+    def view(self) -> DemWallsView:
+        v = DemWallsView()
+        v.n = self.n
+        v.x, v.normal, v.E, v.nu, v.G = (_ptr(a) for a in (self.x, self.normal, self.E, self.nu, self.G))
+        return v
+
+
+class System:
+    """One DEM context on the GPU (a thin owner of a dem_ctx*)."""
+
+    def __init__(self, n, *, dt, cutoff, lo, hi, n_walls=0, max_neighbors=0, max_contacts=16,
+                 cor_pp=1.0, cor_pw=1.0, friction_pp=0.0, friction_pw=0.0, gravity=(0., 0., 0.),
+                 skin=0.0, rebuild_every=1, physics_mode=MODE_REFERENCE):
+        self._lib = lib()
+        cfg = DemConfig()
+        cfg.n, cfg.n_walls, cfg.max_neighbors, cfg.max_contacts = n, n_walls, max_neighbors, max_contacts
+        cfg.dt, cfg.cor_pp, cfg.cor_pw = dt, cor_pp, cor_pw
+        cfg.friction_pp, cfg.friction_pw = friction_pp, friction_pw
+        cfg.gravity[:] = list(gravity)
+        cfg.cutoff, cfg.skin = cutoff, skin
+        cfg.lo[:] = list(lo)
+        cfg.hi[:] = list(hi)
+        cfg.rebuild_every, cfg.physics_mode, cfg.block_size = rebuild_every, physics_mode, 0
+        self.cfg = cfg
+        self.n = n
+        h = C.c_void_p()
+        rc = self._lib.dem_create(C.byref(cfg), C.byref(h))
+        if rc != 0:
+            raise DemError(rc, self._lib.dem_last_error(None).decode())
+        self._h = h
+
+    def _check(self, rc):
+        if rc != 0:
+            raise DemError(rc, self._lib.dem_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.dem_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, st: ParticleState, fields=None):
+        v = st.view(fields)
+        self._check(self._lib.dem_upload_particles(self._h, C.byref(v)))
+
+    def download(self, st: ParticleState | None = None, fields=None) -> ParticleState:
+        if st is None:
+            st = ParticleState(self.n, self.cfg.max_contacts, DEM_MAX_WALLS)
+        v = st.view(fields)
+        self._check(self._lib.dem_download_particles(self._h, C.byref(v)))
+        return st
+
+    def upload_walls(self, w: Walls):
+        self._check(self._lib.dem_upload_walls(self._h, C.byref(w.view())))
+
+    def build_neighbors(self):
+        self._check(self._lib.dem_build_neighbors(self._h))
+
+    def neighbors(self, max_n=64):
+        counts = np.zeros(self.n, dtype=np.int32)
+        ids = np.zeros((self.n, max_n), dtype=np.int32)
+        self._check(self._lib.dem_get_neighbors(self._h, _ptr(counts, C.c_int), _ptr(ids, C.c_int), max_n))
+        return counts, ids
+
+    def zero_forces(self): self._check(self._lib.dem_zero_forces(self._h))
+    def update_tangential(self): self._check(self._lib.dem_update_tangential(self._h))
+    def force_pp(self): self._check(self._lib.dem_force_pp(self._h))
+    def force_pw(self): self._check(self._lib.dem_force_pw(self._h))
+    def compute_forces(self): self._check(self._lib.dem_compute_forces(self._h))
+    def stage1(self): self._check(self._lib.dem_stage1(self._h))
+    def stage2(self): self._check(self._lib.dem_stage2(self._h))
+    def stage3(self): self._check(self._lib.dem_stage3(self._h))
+    def step(self, nsteps=1): self._check(self._lib.dem_step(self._h, nsteps))
+    def sync(self): self._check(self._lib.dem_synchronize(self._h))
+
+    def set_diag_every(self, every):
+        self._check(self._lib.dem_set_diag_every(self._h, every))
+
+    def diag_history(self) -> np.ndarray:
+        nrows = C.c_int(0)
+        self._check(self._lib.dem_get_diag_history(self._h, None, 0, C.byref(nrows)))
+        out = np.zeros((max(nrows.value, 1), 9))
+        self._check(self._lib.dem_get_diag_history(self._h, _ptr(out), nrows.value, C.byref(nrows)))
+        return out[: nrows.value]
+
+    def diagnostics(self) -> np.ndarray:
+        out = np.zeros(5)
+        self._check(self._lib.dem_diagnostics(self._h, _ptr(out)))
+        return out
+
+    def counters(self) -> DemCounters:
+        c = DemCounters()
+        self._check(self._lib.dem_get_counters(self._h, C.byref(c)))
+        return c
+
+    def set_timing(self, on: bool):
+        self._check(self._lib.dem_set_timing(self._h, 1 if on else 0))
+
+    def timing(self):
+        ms = np.zeros(2)
+        n = (C.c_int64 * 2)()
+        self._check(self._lib.dem_get_timing(self._h, _ptr(ms), n))
+        return ms, [n[0], n[1]]
+
+
+def device_count() -> int:
+    return int(lib().dem_device_count())

@@ -1,0 +1,68 @@
+"""Build the HIP library (libcabanadem_hip.so) and the CPU oracle in-tree.
+
+The library is compiled for gfx950 with hipcc.  FMA contraction is off
+(-ffp-contract=off) so the neighbour distance test and the overlap test use
+the reference's exact expressions; packed FP32 instructions are disabled.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+LIB = PKG / "lib" / "libcabanadem_hip.so"
+ORACLE_DIR = ROOT / "oracle"
+
+HIP_SOURCES = ["dem_api.hip", "dem_build.hip", "dem_force.hip", "dem_gen.cpp"]
+
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-shared",
+    "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall",
+    "-Wno-unused-result",
+    "-Wno-unused-value",
+]
+
+
+def _run(cmd, cwd=None):
+    print("+", " ".join(str(c) for c in cmd), flush=True)
+    subprocess.run([str(c) for c in cmd], cwd=cwd, check=True)
+
+
+def build_library(extra_flags=None) -> Path:
+    LIB.parent.mkdir(parents=True, exist_ok=True)
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    tmp = LIB.with_suffix(".so.tmp")
+    cmd = [hipcc] + HIPCC_FLAGS + list(extra_flags or []) + [CSRC / s for s in HIP_SOURCES] + ["-o", tmp]
+    _run(cmd, cwd=CSRC)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle() -> Path:
+    _run(["make", "-C", ORACLE_DIR, "-B"])
+    return ORACLE_DIR / "_build" / "libdem_oracle.so"
+
+
+def build_all():
+    build_oracle()
+    build_library()
+
+
+if __name__ == "__main__":
+    what = sys.argv[1] if len(sys.argv) > 1 else "all"
+    if what == "lib":
+        build_library()
+    elif what == "oracle":
+        build_oracle()
+    else:
+        build_all()

@@ -1,0 +1,873 @@
+// dem_api.hip -- context, memory and the C ABI of include/cabanadem.h.
+//
+// The context owns every device array of one system (layout in
+// dem_internal.h) and a stream of default priority.  Host arrays cross the
+// ABI in the reference layout and original particle order; the context
+// keeps the particles in cell order on the device and translates through
+// the id array (sorted slot -> original id) on upload and download.
+#include "dem_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+using namespace demhip;
+
+namespace {
+
+std::mutex g_err_mu;
+std::string g_err;
+
+void set_global_error(const std::string& s) {
+  std::lock_guard<std::mutex> lk(g_err_mu);
+  g_err = s;
+}
+
+template <class T>
+T* dalloc(size_t count, hipError_t* err) {
+  void* p = nullptr;
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc(&p, count * sizeof(T));
+  if (e != hipSuccess && *err == hipSuccess) *err = e;
+  return static_cast<T*>(p);
+}
+
+// src/CabanaDEM_Force.hpp:206-210 and :215 -- the damping prefactor
+// (-2 sqrt(5/6)) * beta, evaluated on the host with the same operations.
+double eta_prefactor(double cor) {
+  double t1 = std::log(cor);
+  double t2 = t1 * t1 + M_PI * M_PI;
+  double beta = t1 / std::sqrt(t2);
+  return -2. * std::sqrt(5. / 6.) * beta;
+}
+
+} // namespace
+
+struct dem_ctx {
+  dem_config cfg{};
+  std::string err;
+  hipStream_t stream = nullptr;
+  int n = 0, ld = 0, nw = 0, max_n = 0, max_c = 0;
+  Grid grid{};
+  long ncell = 0;
+  double rc2 = 0.;
+
+  // double-buffered groups: P (xr, uw), S (mat, mi, I, id, ft, wall), C (contacts)
+  double4* xr[2] = {nullptr, nullptr};
+  double* uw[2] = {nullptr, nullptr};
+  double4* mat[2] = {nullptr, nullptr};
+  double2* mi[2] = {nullptr, nullptr};
+  double* Imom[2] = {nullptr, nullptr};
+  int* id[2] = {nullptr, nullptr};
+  double* ft[2] = {nullptr, nullptr};
+  int* wmask[2] = {nullptr, nullptr};
+  double* sw[2] = {nullptr, nullptr};
+  int* ccnt[2] = {nullptr, nullptr};
+  int* cj[2] = {nullptr, nullptr};
+  double* ch[2] = {nullptr, nullptr};
+  int cP = 0, cS = 0, cC = 0;
+
+  int* nn = nullptr;
+  int* nbr = nullptr;
+  double* xb = nullptr;
+  int* key = nullptr;
+  int* rank = nullptr;
+  int* key_sorted = nullptr;
+  int* cell_count = nullptr;
+  int* cell_start = nullptr;
+  int* perm = nullptr;
+  int* inv = nullptr;
+  int* scan_tmp = nullptr;
+  DevFlags* flags = nullptr;
+  double* diag_part = nullptr;
+  double* diag_rows = nullptr;
+  int diag_rows_cap = 0;
+
+  WallConst walls[DEM_MAX_WALLS]{};
+  bool list_valid = false;
+  bool built_once = false;
+  int64_t step = 0;
+  int64_t rebuilds = 0;
+  int diag_every = 0;
+  std::vector<double> diag_hist; // rows of 9
+  double last_sample[3] = {0., 0., 0.};
+  int max_nn_seen = 0;
+
+  // host copies of the static properties, original order
+  std::vector<double> h_m, h_rad, h_E, h_nu, h_G, h_I;
+
+  // timing
+  bool timing = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  std::vector<std::pair<int, size_t>> ev_marks; // (slot, index of start event)
+  double t_ms[2] = {0., 0.};
+  int64_t t_launches[2] = {0, 0};
+
+  ~dem_ctx() {
+    for (int b = 0; b < 2; ++b) {
+      hipFree(xr[b]); hipFree(uw[b]); hipFree(mat[b]); hipFree(mi[b]); hipFree(Imom[b]);
+      hipFree(id[b]); hipFree(ft[b]); hipFree(wmask[b]); hipFree(sw[b]);
+      hipFree(ccnt[b]); hipFree(cj[b]); hipFree(ch[b]);
+    }
+    hipFree(nn); hipFree(nbr); hipFree(xb); hipFree(key); hipFree(rank); hipFree(key_sorted);
+    hipFree(cell_count); hipFree(cell_start); hipFree(perm); hipFree(inv); hipFree(scan_tmp);
+    hipFree(flags); hipFree(diag_part); hipFree(diag_rows);
+    for (auto e : ev_pool) hipEventDestroy(e);
+    if (stream) hipStreamDestroy(stream);
+  }
+
+  int fail(int code, const std::string& msg) {
+    err = msg;
+    set_global_error(msg);
+    return code;
+  }
+  int hip_fail(hipError_t e, const char* where) {
+    return fail(DEM_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
+  }
+
+  StepArgs step_args() const {
+    StepArgs a{};
+    a.n = n; a.ld = ld; a.n_walls = nw; a.max_n = max_n; a.max_c = max_c;
+    a.physics_mode = cfg.physics_mode;
+    a.dt = cfg.dt;
+    a.hdt = 0.5 * cfg.dt; // src/CabanaDEM_Integrator.hpp:25
+    for (int d = 0; d < 3; ++d) a.g[d] = cfg.gravity[d];
+    a.gravity_on = (cfg.gravity[0] != 0. || cfg.gravity[1] != 0. || cfg.gravity[2] != 0.) ? 1 : 0;
+    a.c_eta_pp = eta_prefactor(cfg.cor_pp);
+    a.c_eta_pw = eta_prefactor(cfg.cor_pw);
+    a.fr_pp = cfg.friction_pp;
+    a.fr_pw = cfg.friction_pw;
+    for (int w = 0; w < DEM_MAX_WALLS; ++w) a.walls[w] = walls[w];
+    a.xr_in = xr[cP]; a.xr_out = xr[cP ^ 1];
+    a.uw_in = uw[cP]; a.uw_out = uw[cP ^ 1];
+    a.mat = mat[cS]; a.mi = mi[cS]; a.Imom = Imom[cS]; a.ft = ft[cS];
+    a.wmask = wmask[cS]; a.sw = sw[cS];
+    a.nn = nn; a.nbr = nbr;
+    a.ccnt_in = ccnt[cC]; a.cj_in = cj[cC]; a.ch_in = ch[cC];
+    a.ccnt_out = ccnt[cC ^ 1]; a.cj_out = cj[cC ^ 1]; a.ch_out = ch[cC ^ 1];
+    a.flags = flags;
+    a.diag_part = diag_part;
+    return a;
+  }
+
+  BuildArgs build_args(bool check_skin) const {
+    BuildArgs b{};
+    b.n = n; b.ld = ld; b.max_n = max_n; b.max_c = max_c;
+    b.grid = grid; b.ncell = ncell; b.rc2 = rc2;
+    double sh = 0.5 * cfg.skin;
+    b.skin_half_sq = sh * sh;
+    b.check_skin = (check_skin && cfg.skin > 0.) ? 1 : 0;
+    b.xr_in = xr[cP]; b.xr_out = xr[cP ^ 1];
+    b.uw_in = uw[cP]; b.uw_out = uw[cP ^ 1];
+    b.mat_in = mat[cS]; b.mat_out = mat[cS ^ 1];
+    b.mi_in = mi[cS]; b.mi_out = mi[cS ^ 1];
+    b.I_in = Imom[cS]; b.I_out = Imom[cS ^ 1];
+    b.id_in = id[cS]; b.id_out = id[cS ^ 1];
+    b.ft_in = ft[cS]; b.ft_out = ft[cS ^ 1];
+    b.wmask_in = wmask[cS]; b.wmask_out = wmask[cS ^ 1];
+    b.sw_in = sw[cS]; b.sw_out = sw[cS ^ 1];
+    b.ccnt_in = ccnt[cC]; b.ccnt_out = ccnt[cC ^ 1];
+    b.cj_in = cj[cC]; b.cj_out = cj[cC ^ 1];
+    b.ch_in = ch[cC]; b.ch_out = ch[cC ^ 1];
+    b.xb = xb; b.key = key; b.rank = rank; b.key_sorted = key_sorted;
+    b.cell_count = cell_count; b.cell_start = cell_start;
+    b.perm = perm; b.inv = inv; b.scan_tmp = scan_tmp;
+    b.nn = nn; b.nbr = nbr; b.flags = flags;
+    return b;
+  }
+
+  hipEvent_t next_event() {
+    if (ev_used == ev_pool.size()) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return nullptr;
+      ev_pool.push_back(e);
+    }
+    return ev_pool[ev_used++];
+  }
+  void mark_begin(int slot) {
+    if (!timing) return;
+    hipEvent_t e = next_event();
+    if (!e) return;
+    hipEventRecord(e, stream);
+    ev_marks.push_back({slot, ev_used - 1});
+  }
+  void mark_end() {
+    if (!timing || ev_marks.empty()) return;
+    hipEvent_t e = next_event();
+    if (!e) return;
+    hipEventRecord(e, stream);
+  }
+  void collect_timing() {
+    if (!timing) { ev_used = 0; ev_marks.clear(); return; }
+    for (auto& m : ev_marks) {
+      if (m.second + 1 >= ev_used) continue;
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, ev_pool[m.second], ev_pool[m.second + 1]) == hipSuccess) {
+        t_ms[m.first] += ms;
+        t_launches[m.first] += 1;
+      }
+    }
+    ev_used = 0;
+    ev_marks.clear();
+  }
+
+  int check_flags() {
+    DevFlags f;
+    hipError_t e = hipMemcpyAsync(&f, flags, sizeof(f), hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return hip_fail(e, "flag read-back");
+    max_nn_seen = std::max(max_nn_seen, f.max_nn);
+    if (f.nonfinite) return fail(DEM_ERR_NONFINITE, "non-finite particle position met by the cell binning");
+    if (f.cell_overflow)
+      return fail(DEM_ERR_ARG, "more than 4096 particles in one cell: the binning box (lo/hi) "
+                               "does not cover the particles");
+    if (f.nbr_overflow)
+      return fail(DEM_ERR_NEIGH_OVERFLOW, "neighbour row of " + std::to_string(f.nbr_overflow) +
+                                              " exceeds max_neighbors = " + std::to_string(max_n));
+    if (f.contact_overflow)
+      return fail(DEM_ERR_CONTACT_OVERFLOW, "particle with " + std::to_string(f.contact_overflow) +
+                                                " overlapping neighbours exceeds max_contacts = " +
+                                                std::to_string(max_c));
+    if (f.skin) return fail(DEM_ERR_SKIN, "a particle moved more than skin/2 between neighbour rebuilds");
+    return DEM_OK;
+  }
+  // Reports and clears the sticky device error flags.
+  int check_and_clear_flags() {
+    int r = check_flags();
+    if (r != DEM_OK && r != DEM_ERR_HIP) {
+      DevFlags z{};
+      hipMemcpyAsync(flags, &z, sizeof(z), hipMemcpyHostToDevice, stream);
+      hipStreamSynchronize(stream);
+    }
+    return r;
+  }
+
+  int alloc_nbr(int cap) {
+    hipFree(nbr);
+    nbr = nullptr;
+    hipError_t e = hipSuccess;
+    nbr = dalloc<int>((size_t)cap * ld, &e);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc neighbour rows");
+    max_n = cap;
+    return DEM_OK;
+  }
+
+  // Rebuild: re-sort into cell order, gather, search.  Grows the rows on
+  // overflow when max_neighbors was left to auto sizing.
+  int rebuild(bool sync_check, bool check_skin) {
+    mark_begin(1);
+    BuildArgs b = build_args(check_skin && built_once);
+    hipError_t e = launch_rebuild(b, stream);
+    if (e != hipSuccess) return hip_fail(e, "rebuild launch");
+    cP ^= 1; cS ^= 1; cC ^= 1;
+    b = build_args(false);
+    b.xr_out = xr[cP];
+    if ((e = launch_neighbors(b, stream)) != hipSuccess) return hip_fail(e, "neighbour launch");
+    mark_end();
+    built_once = true;
+    list_valid = true;
+    ++rebuilds;
+    if (!sync_check) return DEM_OK;
+    DevFlags f;
+    e = hipMemcpyAsync(&f, flags, sizeof(f), hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return hip_fail(e, "flag read-back");
+    if (f.nbr_overflow && cfg.max_neighbors <= 0) {
+      int cap = f.nbr_overflow + f.nbr_overflow / 2 + 8;
+      int rc = alloc_nbr(cap);
+      if (rc) return rc;
+      DevFlags z{};
+      z.max_nn = f.max_nn;
+      hipMemcpyAsync(flags, &z, sizeof(z), hipMemcpyHostToDevice, stream);
+      b.max_n = max_n;
+      b.nbr = nbr;
+      if ((e = launch_neighbors(b, stream)) != hipSuccess) return hip_fail(e, "neighbour launch");
+    }
+    return check_and_clear_flags();
+  }
+};
+
+namespace {
+
+int validate(const dem_config* c, std::string* msg) {
+  if (!c) { *msg = "null config"; return DEM_ERR_ARG; }
+  if (c->n <= 0) { *msg = "n must be positive"; return DEM_ERR_ARG; }
+  if (c->n_walls < 0 || c->n_walls > DEM_MAX_WALLS) { *msg = "n_walls must be in [0, 8]"; return DEM_ERR_ARG; }
+  if (c->max_contacts <= 0 || c->max_contacts > kMaxContacts) { *msg = "max_contacts must be in [1, 32]"; return DEM_ERR_ARG; }
+  if (!(c->dt > 0.)) { *msg = "dt must be positive"; return DEM_ERR_ARG; }
+  if (!(c->cutoff > 0.)) { *msg = "cutoff must be positive"; return DEM_ERR_ARG; }
+  if (!(c->cor_pp > 0. && c->cor_pp <= 1.) || !(c->cor_pw > 0. && c->cor_pw <= 1.)) {
+    *msg = "restitution coefficients must be in (0, 1] (log(cor) at src/CabanaDEM_Force.hpp:206)";
+    return DEM_ERR_ARG;
+  }
+  for (int d = 0; d < 3; ++d)
+    if (!(c->hi[d] > c->lo[d])) { *msg = "binning box must have hi > lo"; return DEM_ERR_ARG; }
+  if (c->physics_mode != DEM_MODE_REFERENCE && c->physics_mode != DEM_MODE_CORRECTED) {
+    *msg = "physics_mode must be 0 or 1"; return DEM_ERR_ARG;
+  }
+  if (c->block_size != 0 && c->block_size != kBlock) { *msg = "block_size must be 0 or 256"; return DEM_ERR_ARG; }
+  return DEM_OK;
+}
+
+// Cells of edge >= cutoff (1e-9 margin against rounding in (x-lo)/h),
+// capped at 64M cells by coarsening (coarser cells stay exact).
+Grid make_grid(const dem_config& c, long* ncell) {
+  Grid g{};
+  long cap = 64l << 20;
+  int nc[3];
+  for (int d = 0; d < 3; ++d) {
+    double ext = c.hi[d] - c.lo[d];
+    double cs = c.cutoff * (1.0 + 1e-9);
+    double f = std::floor(ext / cs);
+    if (f < 1.) f = 1.;
+    if (f > 1e6) f = 1e6;
+    nc[d] = (int)f;
+  }
+  while ((long)nc[0] * nc[1] * nc[2] > cap) {
+    int d = (nc[0] >= nc[1] && nc[0] >= nc[2]) ? 0 : (nc[1] >= nc[2] ? 1 : 2);
+    nc[d] = std::max(1, nc[d] / 2);
+  }
+  for (int d = 0; d < 3; ++d) {
+    double ext = c.hi[d] - c.lo[d];
+    g.lo[d] = c.lo[d];
+    g.nc[d] = nc[d];
+    g.h[d] = ext / nc[d];
+    double cs = c.cutoff * (1.0 + 1e-9);
+    if (g.h[d] < cs) g.h[d] = cs;
+    g.inv_h[d] = 1.0 / g.h[d];
+  }
+  *ncell = (long)nc[0] * nc[1] * nc[2];
+  return g;
+}
+
+int check_ctx(dem_ctx* c) {
+  if (!c) { set_global_error("null context"); return DEM_ERR_ARG; }
+  return DEM_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int dem_abi_version(void) { return DEM_ABI_VERSION; }
+
+int dem_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+const char* dem_last_error(const dem_ctx* ctx) {
+  if (ctx) return ctx->err.c_str();
+  std::lock_guard<std::mutex> lk(g_err_mu);
+  static thread_local std::string copy;
+  copy = g_err;
+  return copy.c_str();
+}
+
+int dem_create(const dem_config* cfg, dem_ctx** out) {
+  if (!out) { set_global_error("null output pointer"); return DEM_ERR_ARG; }
+  *out = nullptr;
+  std::string msg;
+  int rc = validate(cfg, &msg);
+  if (rc) { set_global_error(msg); return rc; }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    set_global_error("no HIP device available");
+    return DEM_ERR_HIP;
+  }
+  dem_ctx* c = new dem_ctx();
+  c->cfg = *cfg;
+  if (c->cfg.rebuild_every <= 0) c->cfg.rebuild_every = 1;
+  c->n = cfg->n;
+  c->ld = (cfg->n + 63) / 64 * 64;
+  c->nw = cfg->n_walls;
+  c->max_c = cfg->max_contacts;
+  c->rc2 = cfg->cutoff * cfg->cutoff; // Cabana: rsqr = delta * delta
+  c->grid = make_grid(*cfg, &c->ncell);
+  hipError_t e = hipStreamCreate(&c->stream);
+  if (e != hipSuccess) { int r = c->hip_fail(e, "hipStreamCreate"); delete c; return r; }
+  const size_t n = (size_t)c->n, ld = (size_t)c->ld;
+  e = hipSuccess;
+  for (int b = 0; b < 2; ++b) {
+    c->xr[b] = dalloc<double4>(n, &e);
+    c->uw[b] = dalloc<double>(6 * n, &e);
+    c->mat[b] = dalloc<double4>(n, &e);
+    c->mi[b] = dalloc<double2>(n, &e);
+    c->Imom[b] = dalloc<double>(n, &e);
+    c->id[b] = dalloc<int>(n, &e);
+    c->ft[b] = dalloc<double>(6 * n, &e);
+    c->wmask[b] = dalloc<int>(n, &e);
+    c->sw[b] = dalloc<double>((size_t)DEM_MAX_WALLS * 3 * ld, &e);
+    c->ccnt[b] = dalloc<int>(n, &e);
+    c->cj[b] = dalloc<int>((size_t)c->max_c * ld, &e);
+    c->ch[b] = dalloc<double>((size_t)c->max_c * 3 * ld, &e);
+  }
+  c->nn = dalloc<int>(n, &e);
+  c->xb = dalloc<double>(3 * n, &e);
+  c->key = dalloc<int>(n, &e);
+  c->rank = dalloc<int>(n, &e);
+  c->key_sorted = dalloc<int>(n, &e);
+  c->cell_count = dalloc<int>((size_t)c->ncell, &e);
+  c->cell_start = dalloc<int>((size_t)c->ncell + 1, &e);
+  c->perm = dalloc<int>(n, &e);
+  c->inv = dalloc<int>(n, &e);
+  c->scan_tmp = dalloc<int>((size_t)(c->ncell / 4096 + 2), &e);
+  c->flags = dalloc<DevFlags>(1, &e);
+  c->diag_part = dalloc<double>((size_t)force_blocks(c->n) * 8, &e);
+  if (e != hipSuccess) { int r = c->hip_fail(e, "hipMalloc"); delete c; return r; }
+  int cap = cfg->max_neighbors > 0 ? cfg->max_neighbors : 48;
+  if ((rc = c->alloc_nbr(cap))) { delete c; return rc; }
+  // zero state: identity order, no contacts, no wall history
+  std::vector<int> ids(n);
+  for (size_t i = 0; i < n; ++i) ids[i] = (int)i;
+  e = hipMemcpy(c->id[0], ids.data(), n * sizeof(int), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(c->ccnt[0], 0, n * sizeof(int));
+  if (e == hipSuccess) e = hipMemset(c->wmask[0], 0, n * sizeof(int));
+  if (e == hipSuccess) e = hipMemset(c->ft[0], 0, 6 * n * sizeof(double));
+  if (e == hipSuccess) e = hipMemset(c->uw[0], 0, 6 * n * sizeof(double));
+  if (e == hipSuccess) e = hipMemset(c->xr[0], 0, n * sizeof(double4));
+  if (e == hipSuccess) e = hipMemset(c->nn, 0, n * sizeof(int));
+  if (e == hipSuccess) e = hipMemset(c->flags, 0, sizeof(DevFlags));
+  if (e != hipSuccess) { int r = c->hip_fail(e, "initial fill"); delete c; return r; }
+  c->h_m.assign(n, 1.); c->h_rad.assign(n, 1.); c->h_E.assign(n, 1.);
+  c->h_nu.assign(n, 0.); c->h_G.assign(n, 1.); c->h_I.assign(n, 1.);
+  *out = c;
+  return DEM_OK;
+}
+
+void dem_destroy(dem_ctx* ctx) {
+  if (!ctx) return;
+  hipStreamSynchronize(ctx->stream);
+  delete ctx;
+}
+
+int dem_upload_walls(dem_ctx* c, const dem_walls_view* w) {
+  if (int r = check_ctx(c)) return r;
+  if (!w || w->n != c->nw) return c->fail(DEM_ERR_ARG, "wall count differs from dem_config.n_walls");
+  for (int j = 0; j < c->nw; ++j) {
+    WallConst& W = c->walls[j];
+    for (int d = 0; d < 3; ++d) {
+      W.x[d] = w->x ? w->x[3 * j + d] : 0.;
+      W.n[d] = w->normal ? w->normal[3 * j + d] : 0.;
+    }
+    // src/CabanaDEM_Force.hpp:562 and :613 per-wall factors
+    W.a1 = (1. - (w->nu[j] * w->nu[j])) / w->E[j];
+    W.a2 = (2. - w->nu[j]) / w->G[j];
+  }
+  return DEM_OK;
+}
+
+int dem_upload_particles(dem_ctx* c, const dem_particles_view* v) {
+  if (int r = check_ctx(c)) return r;
+  if (!v || v->n != c->n) return c->fail(DEM_ERR_ARG, "particle count differs from dem_config.n");
+  const int n = c->n;
+  const size_t ld = c->ld;
+  hipError_t e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return c->hip_fail(e, "sync");
+  // current device state, sorted order
+  std::vector<int> id(n);
+  std::vector<double4> xr(n);
+  std::vector<double> uw(6 * (size_t)n), ft(6 * (size_t)n);
+  e = hipMemcpy(id.data(), c->id[c->cS], n * sizeof(int), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(xr.data(), c->xr[c->cP], n * sizeof(double4), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(uw.data(), c->uw[c->cP], 6 * n * sizeof(double), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(ft.data(), c->ft[c->cS], 6 * n * sizeof(double), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return c->hip_fail(e, "upload read-back");
+  std::vector<int> pos(n);
+  for (int p = 0; p < n; ++p) pos[id[p]] = p;
+  // static properties (original order) -> derived factors
+  auto setv = [&](std::vector<double>& dst, const double* src) {
+    if (src) std::copy(src, src + n, dst.begin());
+  };
+  setv(c->h_m, v->m); setv(c->h_rad, v->rad); setv(c->h_E, v->E);
+  setv(c->h_nu, v->nu); setv(c->h_G, v->G); setv(c->h_I, v->I);
+  std::vector<double4> mat(n);
+  std::vector<double2> mi(n);
+  std::vector<double> Im(n);
+  for (int p = 0; p < n; ++p) {
+    const int I = id[p];
+    const double nu = c->h_nu[I];
+    // the per-particle factors of src/CabanaDEM_Force.hpp:196-220
+    mat[p].x = (1. - (nu * nu)) / c->h_E[I];
+    mat[p].y = (2. - nu) / c->h_G[I];
+    mat[p].z = 1. / c->h_rad[I];
+    mat[p].w = 1. / c->h_m[I];              // src/CabanaDEM_Integrator.hpp:42
+    mi[p].x = c->h_m[I];
+    mi[p].y = 1. / c->h_I[I];               // src/CabanaDEM_Integrator.hpp:48
+    Im[p] = c->h_I[I];
+    xr[p].w = c->h_rad[I];
+    for (int d = 0; d < 3; ++d) {
+      if (v->x) (&xr[p].x)[d] = v->x[3 * I + d];
+      if (v->u) uw[6 * p + d] = v->u[3 * I + d];
+      if (v->omega) uw[6 * p + 3 + d] = v->omega[3 * I + d];
+      if (v->force) ft[6 * p + d] = v->force[3 * I + d];
+      if (v->torque) ft[6 * p + 3 + d] = v->torque[3 * I + d];
+    }
+  }
+  e = hipMemcpy(c->xr[c->cP], xr.data(), n * sizeof(double4), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(c->uw[c->cP], uw.data(), 6 * n * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(c->ft[c->cS], ft.data(), 6 * n * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(c->mat[c->cS], mat.data(), n * sizeof(double4), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(c->mi[c->cS], mi.data(), n * sizeof(double2), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(c->Imom[c->cS], Im.data(), n * sizeof(double), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return c->hip_fail(e, "upload");
+  // wall history: a set bit marks a (possibly) non-zero history
+  if (v->sw_x && v->sw_y && v->sw_z) {
+    const int MW = v->max_walls;
+    std::vector<int> wm(n, 0);
+    std::vector<double> sw((size_t)DEM_MAX_WALLS * 3 * ld, 0.);
+    for (int p = 0; p < n; ++p) {
+      const int I = id[p];
+      for (int w = 0; w < std::min(MW, DEM_MAX_WALLS); ++w) {
+        double h[3] = {v->sw_x[(size_t)I * MW + w], v->sw_y[(size_t)I * MW + w], v->sw_z[(size_t)I * MW + w]};
+        if (h[0] != 0. || h[1] != 0. || h[2] != 0.) {
+          wm[p] |= 1 << w;
+          for (int q = 0; q < 3; ++q) sw[(size_t)(w * 3 + q) * ld + p] = h[q];
+        }
+      }
+    }
+    e = hipMemcpy(c->wmask[c->cS], wm.data(), n * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(c->sw[c->cS], sw.data(), sw.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return c->hip_fail(e, "upload wall history");
+  }
+  // tracked pp contacts: reference rows (ids, history) -> compact rows in
+  // sorted index space, ascending j; for a repeated id the last slot wins,
+  // as the reference's search does (src/CabanaDEM_Force.hpp:168-174).
+  if (v->ss_idx && v->ss_cnt) {
+    const int M = v->max_cnts;
+    std::vector<int> cnt(n, 0), cjh((size_t)c->max_c * ld, 0);
+    std::vector<double> chh((size_t)c->max_c * 3 * ld, 0.);
+    for (int p = 0; p < n; ++p) {
+      const int I = id[p];
+      std::vector<std::pair<int, int>> ent; // (sorted j, slot k)
+      for (int k = 0; k < v->ss_cnt[I] && k < M; ++k) {
+        int J = v->ss_idx[(size_t)I * M + k];
+        if (J < 0 || J >= n) continue;
+        int jp = pos[J];
+        bool dup = false;
+        for (auto& q : ent) if (q.first == jp) { q.second = k; dup = true; }
+        if (!dup) ent.push_back({jp, k});
+      }
+      std::sort(ent.begin(), ent.end());
+      if ((int)ent.size() > c->max_c)
+        return c->fail(DEM_ERR_CONTACT_OVERFLOW, "uploaded tracked contacts exceed max_contacts");
+      cnt[p] = (int)ent.size();
+      for (size_t m = 0; m < ent.size(); ++m) {
+        cjh[m * ld + p] = ent[m].first;
+        const size_t src = (size_t)I * M + ent[m].second;
+        const double h[3] = {v->ss_x ? v->ss_x[src] : 0., v->ss_y ? v->ss_y[src] : 0.,
+                             v->ss_z ? v->ss_z[src] : 0.};
+        for (int q = 0; q < 3; ++q) chh[(m * 3 + q) * ld + p] = h[q];
+      }
+    }
+    e = hipMemcpy(c->ccnt[c->cC], cnt.data(), n * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(c->cj[c->cC], cjh.data(), cjh.size() * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(c->ch[c->cC], chh.data(), chh.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return c->hip_fail(e, "upload contacts");
+  }
+  if (v->x) c->list_valid = false;
+  return DEM_OK;
+}
+
+int dem_download_particles(dem_ctx* c, dem_particles_view* v) {
+  if (int r = check_ctx(c)) return r;
+  if (!v || v->n != c->n) return c->fail(DEM_ERR_ARG, "particle count differs from dem_config.n");
+  const int n = c->n;
+  const size_t ld = c->ld;
+  hipError_t e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return c->hip_fail(e, "sync");
+  std::vector<int> id(n);
+  std::vector<double4> xr(n);
+  std::vector<double> uw(6 * (size_t)n), ft(6 * (size_t)n);
+  e = hipMemcpy(id.data(), c->id[c->cS], n * sizeof(int), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(xr.data(), c->xr[c->cP], n * sizeof(double4), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(uw.data(), c->uw[c->cP], 6 * n * sizeof(double), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(ft.data(), c->ft[c->cS], 6 * n * sizeof(double), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return c->hip_fail(e, "download");
+  for (int p = 0; p < n; ++p) {
+    const int I = id[p];
+    for (int d = 0; d < 3; ++d) {
+      if (v->x) v->x[3 * I + d] = (&xr[p].x)[d];
+      if (v->u) v->u[3 * I + d] = uw[6 * p + d];
+      if (v->omega) v->omega[3 * I + d] = uw[6 * p + 3 + d];
+      if (v->force) v->force[3 * I + d] = ft[6 * p + d];
+      if (v->torque) v->torque[3 * I + d] = ft[6 * p + 3 + d];
+    }
+  }
+  auto getv = [&](double* dst, const std::vector<double>& src) {
+    if (dst) std::copy(src.begin(), src.end(), dst);
+  };
+  getv(v->m, c->h_m); getv(v->rad, c->h_rad); getv(v->E, c->h_E);
+  getv(v->nu, c->h_nu); getv(v->G, c->h_G); getv(v->I, c->h_I);
+  if (v->sw_x || v->sw_y || v->sw_z) {
+    const int MW = v->max_walls;
+    std::vector<int> wm(n);
+    std::vector<double> sw((size_t)DEM_MAX_WALLS * 3 * ld);
+    e = hipMemcpy(wm.data(), c->wmask[c->cS], n * sizeof(int), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(sw.data(), c->sw[c->cS], sw.size() * sizeof(double), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return c->hip_fail(e, "download wall history");
+    for (int p = 0; p < n; ++p) {
+      const int I = id[p];
+      for (int w = 0; w < MW; ++w) {
+        double h[3] = {0., 0., 0.};
+        if (w < DEM_MAX_WALLS && (wm[p] & (1 << w)))
+          for (int q = 0; q < 3; ++q) h[q] = sw[(size_t)(w * 3 + q) * ld + p];
+        if (v->sw_x) v->sw_x[(size_t)I * MW + w] = h[0];
+        if (v->sw_y) v->sw_y[(size_t)I * MW + w] = h[1];
+        if (v->sw_z) v->sw_z[(size_t)I * MW + w] = h[2];
+      }
+    }
+  }
+  if (v->ss_idx || v->ss_cnt || v->ss_x) {
+    const int M = v->max_cnts;
+    std::vector<int> cnt(n), cjh((size_t)c->max_c * ld);
+    std::vector<double> chh((size_t)c->max_c * 3 * ld);
+    e = hipMemcpy(cnt.data(), c->ccnt[c->cC], n * sizeof(int), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(cjh.data(), c->cj[c->cC], cjh.size() * sizeof(int), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(chh.data(), c->ch[c->cC], chh.size() * sizeof(double), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return c->hip_fail(e, "download contacts");
+    for (int p = 0; p < n; ++p) {
+      const int I = id[p];
+      int k = 0;
+      for (; k < cnt[p] && k < M; ++k) {
+        const size_t dst = (size_t)I * M + k;
+        if (v->ss_idx) v->ss_idx[dst] = id[cjh[(size_t)k * ld + p]];
+        if (v->ss_x) v->ss_x[dst] = chh[(size_t)(k * 3 + 0) * ld + p];
+        if (v->ss_y) v->ss_y[dst] = chh[(size_t)(k * 3 + 1) * ld + p];
+        if (v->ss_z) v->ss_z[dst] = chh[(size_t)(k * 3 + 2) * ld + p];
+      }
+      for (; k < M; ++k) {
+        const size_t dst = (size_t)I * M + k;
+        if (v->ss_idx) v->ss_idx[dst] = -1;
+        if (v->ss_x) v->ss_x[dst] = 0.;
+        if (v->ss_y) v->ss_y[dst] = 0.;
+        if (v->ss_z) v->ss_z[dst] = 0.;
+      }
+      if (v->ss_cnt) v->ss_cnt[I] = std::min(cnt[p], M);
+    }
+  }
+  return DEM_OK;
+}
+
+int dem_build_neighbors(dem_ctx* c) {
+  if (int r = check_ctx(c)) return r;
+  return c->rebuild(true, c->list_valid);
+}
+
+int dem_get_neighbors(dem_ctx* c, int* counts, int* ids, int max_n) {
+  if (int r = check_ctx(c)) return r;
+  if (!c->list_valid) return c->fail(DEM_ERR_STATE, "no neighbour list has been built");
+  if (!counts || !ids || max_n <= 0) return c->fail(DEM_ERR_ARG, "bad output arrays");
+  const int n = c->n;
+  const size_t ld = c->ld;
+  hipError_t e = hipStreamSynchronize(c->stream);
+  std::vector<int> id(n), nn(n), nbr((size_t)c->max_n * ld);
+  if (e == hipSuccess) e = hipMemcpy(id.data(), c->id[c->cS], n * sizeof(int), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(nn.data(), c->nn, n * sizeof(int), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(nbr.data(), c->nbr, nbr.size() * sizeof(int), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return c->hip_fail(e, "download neighbours");
+  int over = 0;
+  std::vector<int> row;
+  for (int p = 0; p < n; ++p) {
+    row.clear();
+    for (int k = 0; k < nn[p]; ++k) row.push_back(id[nbr[(size_t)k * ld + p]]);
+    std::sort(row.begin(), row.end());
+    const int I = id[p];
+    counts[I] = (int)row.size();
+    if ((int)row.size() > max_n) over = 1;
+    for (int k = 0; k < (int)row.size() && k < max_n; ++k) ids[(size_t)I * max_n + k] = row[k];
+  }
+  if (over) return c->fail(DEM_ERR_ARG, "max_n too small for the neighbour rows");
+  return DEM_OK;
+}
+
+int dem_zero_forces(dem_ctx* c) {
+  if (int r = check_ctx(c)) return r;
+  hipError_t e = launch_zero_forces(c->step_args(), c->stream);
+  return e == hipSuccess ? DEM_OK : c->hip_fail(e, "zero forces");
+}
+
+int dem_update_tangential(dem_ctx* c) {
+  if (int r = check_ctx(c)) return r;
+  hipError_t e = launch_update_tangential(c->step_args(), c->stream);
+  if (e != hipSuccess) return c->hip_fail(e, "update tangential");
+  c->cC ^= 1;
+  return DEM_OK;
+}
+
+static int force_call(dem_ctx* c, ForceKind kind, bool needs_list) {
+  if (int r = check_ctx(c)) return r;
+  if (needs_list && !c->list_valid) return c->fail(DEM_ERR_STATE, "neighbour list not built for the current positions");
+  hipError_t e = launch_force(kind, c->step_args(), false, c->stream);
+  if (e != hipSuccess) return c->hip_fail(e, "force launch");
+  if (kind != FK_PW_ACCUM) c->cC ^= 1;
+  return c->check_and_clear_flags();
+}
+
+int dem_force_pp(dem_ctx* c) { return force_call(c, FK_PP_ACCUM, true); }
+int dem_force_pw(dem_ctx* c) { return force_call(c, FK_PW_ACCUM, false); }
+int dem_compute_forces(dem_ctx* c) { return force_call(c, FK_PP_PW_FORCES, true); }
+
+static int kick_call(dem_ctx* c, int kick, int drift) {
+  if (int r = check_ctx(c)) return r;
+  StepArgs a = c->step_args();
+  hipError_t e = launch_kick(a, c->xr[c->cP], c->uw[c->cP], c->ft[c->cS], kick, drift, c->stream);
+  if (e != hipSuccess) return c->hip_fail(e, "stage launch");
+  return DEM_OK;
+}
+int dem_stage1(dem_ctx* c) { return kick_call(c, 1, 0); }
+int dem_stage2(dem_ctx* c) { return kick_call(c, 0, 1); }
+int dem_stage3(dem_ctx* c) { return kick_call(c, 1, 0); }
+
+int dem_set_diag_every(dem_ctx* c, int every) {
+  if (int r = check_ctx(c)) return r;
+  c->diag_every = every > 0 ? every : 0;
+  return DEM_OK;
+}
+
+int dem_step(dem_ctx* c, int nsteps) {
+  if (int r = check_ctx(c)) return r;
+  if (nsteps <= 0) return DEM_OK;
+  const int K = c->cfg.rebuild_every;
+  const int nb = force_blocks(c->n);
+  // diagnostic rows of this call
+  int rows = 0;
+  if (c->diag_every > 0)
+    for (int s = 0; s < nsteps; ++s)
+      if ((c->step + s) % c->diag_every == 0) ++rows;
+  if (rows > c->diag_rows_cap) {
+    hipFree(c->diag_rows);
+    c->diag_rows = nullptr;
+    hipError_t e = hipSuccess;
+    c->diag_rows = dalloc<double>((size_t)rows * 9, &e);
+    if (e != hipSuccess) return c->hip_fail(e, "hipMalloc diag rows");
+    c->diag_rows_cap = rows;
+  }
+  // stage1 + stage2 of the first step (later ones are fused into the force kernel)
+  StepArgs a0 = c->step_args();
+  hipError_t e = launch_kick(a0, c->xr[c->cP], c->uw[c->cP], c->ft[c->cS], 1, 1, c->stream);
+  if (e != hipSuccess) return c->hip_fail(e, "stage1/2 launch");
+  int row = 0;
+  for (int s = 0; s < nsteps; ++s) {
+    const int64_t gstep = c->step;
+    if (!c->list_valid || gstep % K == 0) {
+      int r = c->rebuild(!c->built_once, c->list_valid);
+      if (r) return r;
+    }
+    const bool sample = c->diag_every > 0 && gstep % c->diag_every == 0;
+    const bool last = (s == nsteps - 1);
+    StepArgs a = c->step_args();
+    c->mark_begin(0);
+    e = launch_force(last ? FK_STEP_LAST : FK_STEP_FUSED, a, sample, c->stream);
+    c->mark_end();
+    if (e != hipSuccess) return c->hip_fail(e, "step launch");
+    c->cP ^= 1;
+    c->cC ^= 1;
+    if (sample) {
+      e = launch_diag_final(c->diag_part, nb, c->diag_rows + 9 * row, (double)gstep, c->stream);
+      if (e != hipSuccess) return c->hip_fail(e, "diag launch");
+      ++row;
+    }
+    c->step++;
+  }
+  if (row > 0) {
+    std::vector<double> h((size_t)row * 9);
+    e = hipMemcpyAsync(h.data(), c->diag_rows, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return c->hip_fail(e, "diag read-back");
+    const double nD = (double)c->n;
+    for (int r = 0; r < row; ++r) {
+      double* q = &h[(size_t)r * 9];
+      q[6] /= nD; q[7] /= nD; q[8] /= nD; // sums -> per-particle means
+      c->diag_hist.insert(c->diag_hist.end(), q, q + 9);
+      c->last_sample[0] = q[6]; c->last_sample[1] = q[7]; c->last_sample[2] = q[8];
+    }
+  }
+  e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return c->hip_fail(e, "step sync");
+  c->collect_timing();
+  return c->check_and_clear_flags();
+}
+
+int dem_get_diag_history(dem_ctx* c, double* rows, int cap, int* nrows) {
+  if (int r = check_ctx(c)) return r;
+  const int total = (int)(c->diag_hist.size() / 9);
+  if (nrows) *nrows = total;
+  if (rows) {
+    const int m = std::min(cap, total);
+    std::copy(c->diag_hist.begin(), c->diag_hist.begin() + (size_t)m * 9, rows);
+  }
+  return DEM_OK;
+}
+
+int dem_diagnostics(dem_ctx* c, double out[5]) {
+  if (int r = check_ctx(c)) return r;
+  StepArgs a = c->step_args();
+  hipError_t e = launch_diag_current(a, c->uw[c->cP], c->stream);
+  if (e != hipSuccess) return c->hip_fail(e, "diag launch");
+  if (c->diag_rows_cap < 1) {
+    hipError_t ea = hipSuccess;
+    c->diag_rows = dalloc<double>(9, &ea);
+    if (ea != hipSuccess) return c->hip_fail(ea, "hipMalloc diag rows");
+    c->diag_rows_cap = 1;
+  }
+  e = launch_diag_final(c->diag_part, force_blocks(c->n), c->diag_rows, (double)c->step, c->stream);
+  double h[9];
+  if (e == hipSuccess) e = hipMemcpyAsync(h, c->diag_rows, sizeof(h), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return c->hip_fail(e, "diagnostics");
+  for (int k = 0; k < 5; ++k) out[k] = h[1 + k];
+  return DEM_OK;
+}
+
+int dem_get_counters(dem_ctx* c, dem_counters* out) {
+  if (int r = check_ctx(c)) return r;
+  if (!out) return c->fail(DEM_ERR_ARG, "null output");
+  DevFlags f{};
+  hipError_t e = hipMemcpyAsync(&f, c->flags, sizeof(f), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return c->hip_fail(e, "counters");
+  out->steps = c->step;
+  out->rebuilds = c->rebuilds;
+  out->max_neighbors_seen = std::max(c->max_nn_seen, f.max_nn);
+  out->neighbor_capacity = c->max_n;
+  out->mean_neighbors = c->last_sample[0];
+  out->mean_contacts = c->last_sample[1];
+  out->mean_wall_contacts = c->last_sample[2];
+  return DEM_OK;
+}
+
+int dem_synchronize(dem_ctx* c) {
+  if (int r = check_ctx(c)) return r;
+  hipError_t e = hipStreamSynchronize(c->stream);
+  return e == hipSuccess ? DEM_OK : c->hip_fail(e, "sync");
+}
+
+int dem_set_timing(dem_ctx* c, int enable) {
+  if (int r = check_ctx(c)) return r;
+  hipStreamSynchronize(c->stream);
+  c->collect_timing();
+  c->timing = enable != 0;
+  c->t_ms[0] = c->t_ms[1] = 0.;
+  c->t_launches[0] = c->t_launches[1] = 0;
+  return DEM_OK;
+}
+
+int dem_get_timing(dem_ctx* c, double ms[2], int64_t launches[2]) {
+  if (int r = check_ctx(c)) return r;
+  hipStreamSynchronize(c->stream);
+  c->collect_timing();
+  for (int k = 0; k < 2; ++k) {
+    if (ms) ms[k] = c->t_ms[k];
+    if (launches) launches[k] = c->t_launches[k];
+  }
+  return DEM_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,297 @@
+// dem_build.hip -- neighbour-list rebuild on the GPU.
+//
+// Replaces Cabana::VerletList<FullNeighborTag>::build as called at
+// src/CabanaDEM_Solver.hpp:60-63,87-88 (reference paths relative to the
+// ExaPhysics/CabanaDEM root).  The list is the set {j != i : dx^2+dy^2+dz^2
+// <= rc^2} with the squared distance summed in x, y, z order without fused
+// multiply-adds, exactly as the reference's Cabana build forms it, so the
+// sets are bit-exact.
+//
+// Sequence (one rebuild, all stream ordered):
+//   1. k_cell_keys   cell key of every particle; per-cell arrival rank by
+//                    atomic counting (also the skin-displacement check)
+//   2. scan          exclusive prefix over the cell counts -> cell starts
+//   3. k_scatter     counting-sort scatter: slot = start[key] + rank
+//   4. k_cell_order  within each cell, order by previous slot (the arrival
+//                    order of step 1 is not deterministic; this makes the
+//                    sort a stable, deterministic single-digit radix sort
+//                    on the cell key) and write the inverse permutation
+//   5. k_permute     gather the whole particle state into the new order and
+//                    re-map the tracked-contact lists (j -> inv[j], re-sorted)
+//   6. k_neighbors   27-cell search: for each (dz, dy) the three x-adjacent
+//                    cells are one contiguous slot range, so a row is nine
+//                    range scans and comes out in ascending j.
+#include "dem_internal.h"
+
+#include <climits>
+
+namespace demhip {
+namespace {
+
+__device__ inline int cell_coord(double v, const Grid& g, int d) {
+  double t = floor((v - g.lo[d]) / g.h[d]);
+  if (!(t >= 0.0)) return 0; // below the box, or NaN
+  if (t > (double)(g.nc[d] - 1)) return g.nc[d] - 1;
+  return (int)t;
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_int(int* p, long count, int value) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long stride = (long)gridDim.x * blockDim.x;
+  for (; i < count; i += stride) p[i] = value;
+}
+
+__global__ __launch_bounds__(kBlock) void k_cell_keys(BuildArgs a) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.n) return;
+  const double4 p = a.xr_in[i];
+  if (!(isfinite(p.x) && isfinite(p.y) && isfinite(p.z))) atomicOr(&a.flags->nonfinite, 1);
+  const int cx = cell_coord(p.x, a.grid, 0);
+  const int cy = cell_coord(p.y, a.grid, 1);
+  const int cz = cell_coord(p.z, a.grid, 2);
+  const int key = (cz * a.grid.nc[1] + cy) * a.grid.nc[0] + cx;
+  a.key[i] = key;
+  a.rank[i] = atomicAdd(&a.cell_count[key], 1);
+  if (a.check_skin) {
+    const double d0 = p.x - a.xb[3 * i + 0];
+    const double d1 = p.y - a.xb[3 * i + 1];
+    const double d2 = p.z - a.xb[3 * i + 2];
+    if (d0 * d0 + d1 * d1 + d2 * d2 > a.skin_half_sq) atomicOr(&a.flags->skin, 1);
+  }
+}
+
+// ---- exclusive scan of int counts (three launches) -------------------------
+constexpr int kScanThreads = 1024;
+constexpr int kScanItems = 4;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+__device__ inline int block_exclusive_scan(int v, int* lds, int* total) {
+  const int t = threadIdx.x;
+  lds[t] = v;
+  __syncthreads();
+  for (int off = 1; off < kScanThreads; off <<= 1) {
+    int add = (t >= off) ? lds[t - off] : 0;
+    __syncthreads();
+    lds[t] += add;
+    __syncthreads();
+  }
+  const int incl = lds[t];
+  *total = lds[kScanThreads - 1];
+  __syncthreads();
+  return incl - v;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_tiles(const int* in, int* out,
+                                                              int* tile_sums, long count) {
+  __shared__ int lds[kScanThreads];
+  const long base = (long)blockIdx.x * kScanTile + (long)threadIdx.x * kScanItems;
+  int v[kScanItems];
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    v[k] = (base + k < count) ? in[base + k] : 0;
+    s += v[k];
+  }
+  int total;
+  int ex = block_exclusive_scan(s, lds, &total);
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    if (base + k < count) out[base + k] = ex;
+    ex += v[k];
+  }
+  if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_top(int* tile_sums, int ntiles,
+                                                            int* out, long count) {
+  __shared__ int lds[kScanThreads];
+  int carry = 0;
+  for (int base = 0; base < ntiles; base += kScanThreads) {
+    const int t = base + (int)threadIdx.x;
+    const int v = (t < ntiles) ? tile_sums[t] : 0;
+    int total;
+    const int ex = block_exclusive_scan(v, lds, &total);
+    if (t < ntiles) tile_sums[t] = ex + carry;
+    carry += total;
+  }
+  if (threadIdx.x == 0) out[count] = carry;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_add(int* out, const int* tile_sums, long count) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) out[i] += tile_sums[i / kScanTile];
+}
+
+hipError_t exclusive_scan(const int* in, int* out, int* tmp, long count, hipStream_t s) {
+  const int ntiles = (int)((count + kScanTile - 1) / kScanTile);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(kScanThreads), 0, s, in, out, tmp, count);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, s, tmp, ntiles, out, count);
+  hipLaunchKernelGGL(k_scan_add, dim3((unsigned)((count + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                     s, out, tmp, count);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(kBlock) void k_scatter(BuildArgs a) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.n) return;
+  a.perm[a.cell_start[a.key[i]] + a.rank[i]] = i;
+}
+
+// One thread per cell: insertion-sort the cell's slots by previous index
+// (cells hold a handful of particles), then record old -> new.
+__global__ __launch_bounds__(kBlock) void k_cell_order(BuildArgs a) {
+  const long c = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (c >= a.ncell) return;
+  const int s = a.cell_start[c], e = a.cell_start[c + 1];
+  if (e - s > kMaxCellParticles) {
+    atomicOr(&a.flags->cell_overflow, 1);
+    for (int q = s; q < e; ++q) a.inv[a.perm[q]] = q;
+    return;
+  }
+  for (int q = s + 1; q < e; ++q) {
+    const int v = a.perm[q];
+    int r = q - 1;
+    while (r >= s && a.perm[r] > v) {
+      a.perm[r + 1] = a.perm[r];
+      --r;
+    }
+    a.perm[r + 1] = v;
+  }
+  for (int q = s; q < e; ++q) a.inv[a.perm[q]] = q;
+}
+
+// Gather every per-particle array into the new order.  The tracked-contact
+// list of a particle keeps its entries and histories; only the neighbour
+// indices change (j -> inv[j]) and the list is re-sorted by the new index,
+// which is the order the force kernel merges in.
+__global__ __launch_bounds__(kBlock) void k_permute(BuildArgs a) {
+  extern __shared__ int s_j[]; // [max_c][kBlock]
+  const int tid = threadIdx.x;
+  const int p = blockIdx.x * kBlock + tid;
+  if (p >= a.n) return;
+  const int o = a.perm[p];
+  const double4 xr = a.xr_in[o];
+  a.xr_out[p] = xr;
+  {
+    const double2* src = reinterpret_cast<const double2*>(a.uw_in + 6l * o);
+    double2* dst = reinterpret_cast<double2*>(a.uw_out + 6l * p);
+    dst[0] = src[0]; dst[1] = src[1]; dst[2] = src[2];
+    const double2* fs = reinterpret_cast<const double2*>(a.ft_in + 6l * o);
+    double2* fd = reinterpret_cast<double2*>(a.ft_out + 6l * p);
+    fd[0] = fs[0]; fd[1] = fs[1]; fd[2] = fs[2];
+  }
+  a.mat_out[p] = a.mat_in[o];
+  a.mi_out[p] = a.mi_in[o];
+  a.I_out[p] = a.I_in[o];
+  a.id_out[p] = a.id_in[o];
+  a.key_sorted[p] = a.key[o];
+  a.xb[3 * p + 0] = xr.x;
+  a.xb[3 * p + 1] = xr.y;
+  a.xb[3 * p + 2] = xr.z;
+  const int wm = a.wmask_in[o];
+  a.wmask_out[p] = wm;
+  if (wm) {
+    for (int w = 0; w < DEM_MAX_WALLS; ++w)
+      if (wm & (1 << w))
+        for (int c = 0; c < 3; ++c)
+          a.sw_out[(long)(w * 3 + c) * a.ld + p] = a.sw_in[(long)(w * 3 + c) * a.ld + o];
+  }
+  int oc = a.ccnt_in[o];
+  if (oc > a.max_c) oc = a.max_c; // counts are clamped on write; never index past the LDS rows
+  for (int m = 0; m < oc; ++m) s_j[m * kBlock + tid] = a.inv[a.cj_in[(long)m * a.ld + o]];
+  for (int m = 0; m < oc; ++m) {
+    const int jm = s_j[m * kBlock + tid];
+    int r = 0;
+    for (int q = 0; q < oc; ++q) r += (s_j[q * kBlock + tid] < jm) ? 1 : 0;
+    a.cj_out[(long)r * a.ld + p] = jm;
+    for (int c = 0; c < 3; ++c)
+      a.ch_out[(long)(r * 3 + c) * a.ld + p] = a.ch_in[(long)(m * 3 + c) * a.ld + o];
+  }
+  a.ccnt_out[p] = oc;
+}
+
+__device__ inline int wave_max_int(int v) {
+  for (int off = 32; off > 0; off >>= 1) {
+    const int o = __shfl_xor(v, off);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_neighbors(BuildArgs a) {
+  const int p = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  int cnt = 0;
+  if (p < a.n && !a.flags->cell_overflow) {
+    const int nx = a.grid.nc[0], ny = a.grid.nc[1], nz = a.grid.nc[2];
+    const int key = a.key_sorted[p];
+    const int cx = key % nx;
+    const int cy = (key / nx) % ny;
+    const int cz = key / (nx * ny);
+    const double4 xi = a.xr_out[p];
+    const int xlo = cx > 0 ? cx - 1 : 0;
+    const int xhi = cx < nx - 1 ? cx + 1 : nx - 1;
+    for (int dz = -1; dz <= 1; ++dz) {
+      const int z = cz + dz;
+      if (z < 0 || z >= nz) continue;
+      for (int dy = -1; dy <= 1; ++dy) {
+        const int y = cy + dy;
+        if (y < 0 || y >= ny) continue;
+        const long row = ((long)z * ny + y) * nx;
+        const int s = a.cell_start[row + xlo];
+        const int e = a.cell_start[row + xhi + 1];
+        for (int j = s; j < e; ++j) {
+          if (j == p) continue;
+          const double4 xj = a.xr_out[j];
+          const double d0 = xi.x - xj.x;
+          const double d1 = xi.y - xj.y;
+          const double d2 = xi.z - xj.z;
+          const double dsqr = d0 * d0 + d1 * d1 + d2 * d2;
+          if (dsqr <= a.rc2) {
+            if (cnt < a.max_n) a.nbr[(long)cnt * a.ld + p] = j;
+            ++cnt;
+          }
+        }
+      }
+    }
+    a.nn[p] = cnt < a.max_n ? cnt : a.max_n;
+  }
+  const int wmax = wave_max_int(cnt);
+  if ((threadIdx.x & 63) == 0 && wmax > 0) {
+    atomicMax(&a.flags->max_nn, wmax);
+    if (wmax > a.max_n) atomicMax(&a.flags->nbr_overflow, wmax);
+  }
+}
+
+inline unsigned nblk(long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+} // namespace
+
+hipError_t launch_fill_int(int* p, long count, int value, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  long blocks = (count + kBlock - 1) / kBlock;
+  if (blocks > 65535L * 8) blocks = 65535L * 8;
+  hipLaunchKernelGGL(k_fill_int, dim3((unsigned)blocks), dim3(kBlock), 0, s, p, count, value);
+  return hipGetLastError();
+}
+
+hipError_t launch_rebuild(const BuildArgs& a, hipStream_t s) {
+  hipError_t e = launch_fill_int(&a.flags->max_nn, 1, 0, s);
+  if (e != hipSuccess) return e;
+  e = launch_fill_int(a.cell_count, a.ncell, 0, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_cell_keys, dim3(nblk(a.n)), dim3(kBlock), 0, s, a);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = exclusive_scan(a.cell_count, a.cell_start, a.scan_tmp, a.ncell, s)) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_scatter, dim3(nblk(a.n)), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL(k_cell_order, dim3(nblk(a.ncell)), dim3(kBlock), 0, s, a);
+  const size_t lds = (size_t)(a.max_c > 0 ? a.max_c : 1) * kBlock * sizeof(int);
+  hipLaunchKernelGGL(k_permute, dim3(nblk(a.n)), dim3(kBlock), lds, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_neighbors(const BuildArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_neighbors, dim3(nblk(a.n)), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+} // namespace demhip

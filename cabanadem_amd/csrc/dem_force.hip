@@ -1,0 +1,529 @@
+// dem_force.hip -- contact forces and velocity-Verlet stages on the GPU.
+//
+// Reference paths are relative to the ExaPhysics/CabanaDEM root.
+//   pp contact    src/CabanaDEM_Force.hpp:73-327  (Hertz normal + tangential spring)
+//   tracking      src/CabanaDEM_Force.hpp:166-192, 343-445
+//   pw contact    src/CabanaDEM_Force.hpp:477-671
+//   stages        src/CabanaDEM_Integrator.hpp:31-103
+// All contact arithmetic is FP64 in the reference's operation order, with
+// the per-particle factors (1-nu^2)/E, (2-nu)/G, 1/r and 1/m precomputed
+// (the same IEEE operations the reference performs per pair) and
+// pow(x, 0.5) -> sqrt(x), pow(x, 1.5) -> x*sqrt(x), log(cor) hoisted.
+//
+// One thread owns particle i (full list, no atomics).  Phase 1 walks the
+// neighbour row and keeps the j with positive overlap in LDS; phase 2 runs
+// the contact arithmetic over that short list only, merging with the
+// previous step's tracked contacts (both ascending in j): a contact found in
+// the old list continues its tangential history, a new one starts from zero
+// (src/CabanaDEM_Force.hpp:180-192), and an old one that no longer overlaps
+// is dropped (src/CabanaDEM_Force.hpp:399-433).  The fused step kernel then
+// applies stage3 of this step and stage1 + stage2 of the next one, writing
+// the new state to the other buffer so neighbours still read the old one.
+#include "dem_internal.h"
+
+#include <climits>
+
+namespace demhip {
+namespace {
+
+__device__ inline void load6(const double* p, double* v) {
+  const double2* q = reinterpret_cast<const double2*>(p);
+  double2 a = q[0], b = q[1], c = q[2];
+  v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y; v[4] = c.x; v[5] = c.y;
+}
+__device__ inline void store6(double* p, const double* v) {
+  double2* q = reinterpret_cast<double2*>(p);
+  q[0] = make_double2(v[0], v[1]);
+  q[1] = make_double2(v[2], v[3]);
+  q[2] = make_double2(v[4], v[5]);
+}
+
+// Particle-particle contact of i with j (overlap > 0 already established).
+// h is the tangential history of (i, j), updated in place.  F (and, in
+// corrected mode, T) accumulate.  Line numbers: src/CabanaDEM_Force.hpp.
+template <bool REF>
+__device__ inline void pp_contact(const double4& xi, const double* ui, const double4& mi,
+                                  const double4& xj, const double* uj, const double4& mj,
+                                  double* h, double* F, double* T, const StepArgs& a) {
+  const double p0 = xi.x - xj.x, p1 = xi.y - xj.y, p2 = xi.z - xj.z; // :94-96
+  const double r2 = p0 * p0 + p1 * p1 + p2 * p2;                      // :99
+  const double rij = sqrt(r2);                                        // :101
+  const double overlap = xi.w + xj.w - rij;                           // :107
+  const double a_i = xi.w - overlap / 2.;                             // :109
+  const double a_j = xj.w - overlap / 2.;                             // :110
+  const double nx = p0 / rij, ny = p1 / rij, nz = p2 / rij;           // :113-115
+  double vi0, vi1, vi2, vj0, vj1, vj2;
+  if (REF) {
+    vi0 = ui[0] + (ui[4] * nz - ui[5] * ny) * a_i;                    // :119-126
+    vi1 = ui[1] + (ui[5] * nx - ui[3] * nz) * a_i;
+    vi2 = ui[2] + (ui[3] * ny - ui[4] * nx) * a_i;
+    vj0 = uj[0] + (-uj[4] * nz + uj[5] * ny) * a_j;                   // :130-137 (u_i in y, z: Q1)
+    vj1 = ui[1] + (-uj[5] * nx + uj[3] * nz) * a_j;
+    vj2 = ui[2] + (-uj[3] * ny + uj[4] * nx) * a_j;
+  } else {
+    vi0 = ui[0] - (ui[4] * nz - ui[5] * ny) * a_i;
+    vi1 = ui[1] - (ui[5] * nx - ui[3] * nz) * a_i;
+    vi2 = ui[2] - (ui[3] * ny - ui[4] * nx) * a_i;
+    vj0 = uj[0] + (uj[4] * nz - uj[5] * ny) * a_j;
+    vj1 = uj[1] + (uj[5] * nx - uj[3] * nz) * a_j;
+    vj2 = uj[2] + (uj[3] * ny - uj[4] * nx) * a_j;
+  }
+  const double v0 = vi0 - vj0, v1 = vi1 - vj1, v2 = vi2 - vj2;      // :141-143
+  const double vdn = v0 * nx + v1 * ny + v2 * nz;                   // :146
+  const double vn0 = vdn * nx, vn1 = vdn * ny, vn2 = vdn * nz;      // :147-149
+  const double vt0 = v0 - vn0, vt1 = v1 - vn1, vt2 = v2 - vn2;      // :152-154
+  const double E_eff = 1. / (mi.x + mj.x);                          // :196-198
+  const double R_eff = 1. / (mi.z + mj.z);                          // :199-201
+  const double kn = 4. / 3. * E_eff * sqrt(R_eff);                  // :203
+  const double sRo = sqrt(R_eff * overlap);
+  const double S_n = 2. * E_eff * sRo;                              // :211
+  const double m_eff = 1. / (mi.w + mj.w);                          // :212-214
+  const double eta_n = a.c_eta_pp * sqrt(S_n * m_eff);              // :215
+  const double G_eff = 1. / (mi.y + mj.y);                          // :218-220
+  const double kt = 8. * G_eff * sRo;                               // :222
+  const double eta_t = REF ? 0. : a.c_eta_pp * sqrt(kt * m_eff);    // :224 (5/6 == 0: Q2)
+  const double fn = kn * (overlap * sqrt(overlap));                 // :227
+  const double fn0 = fn * nx - eta_n * vn0;                         // :228-230
+  const double fn1 = fn * ny - eta_n * vn1;
+  const double fn2 = fn * nz - eta_n * vn2;
+  double tf0 = 0., tf1 = 0., tf2 = 0.;
+  const double vmag = sqrt(v0 * v0 + v1 * v1 + v2 * v2);            // :238
+  if (vmag < 1e-12) {                                               // :239-243
+    h[0] = 0.; h[1] = 0.; h[2] = 0.;
+  } else {
+    const double tx = v0 - nx * vdn, ty = v1 - ny * vdn, tz = v2 - nz * vdn; // :257-259
+    const double tmag = sqrt(tx * tx + ty * ty + tz * tz);          // :261
+    double ti0 = 0., ti1 = 0., ti2 = 0.;
+    if (tmag > 1e-12) { ti0 = tx / tmag; ti1 = ty / tmag; ti2 = tz / tmag; } // :267-271
+    const double d0 = h[0] + v0 * a.dt;                             // :274-276
+    const double d1 = h[1] + v1 * a.dt;
+    const double d2 = h[2] + v2 * a.dt;
+    const double ddt = (d0 * ti0 + d1 * ti1 + d2 * ti2);            // :278-280
+    h[0] = ddt * ti0; h[1] = ddt * ti1; h[2] = ddt * ti2;           // :282-284
+    const double fs0 = -kt * h[0] - eta_t * vt0;                    // :286-288
+    const double fs1 = -kt * h[1] - eta_t * vt1;
+    const double fs2 = -kt * h[2] - eta_t * vt2;
+    const double ftm = sqrt(fs0 * fs0 + fs1 * fs1 + fs2 * fs2);     // :290
+    const double fnm = sqrt(fn0 * fn0 + fn1 * fn1 + fn2 * fn2);     // :291
+    double ftms = ftm;                                              // :293-296
+    if (ftms > a.fr_pp * fnm) ftms = a.fr_pp * fnm;
+    tf0 = -ftms * ti0; tf1 = -ftms * ti1; tf2 = -ftms * ti2;        // :299-301
+    const double m0 = -tf0 / kt, m1 = -tf1 / kt, m2 = -tf2 / kt;    // :303-305
+    if (REF) {                                                      // :307-319 (Q3)
+      const double lm = sqrt(m0 * m0 + m1 * m1 + m2 * m2);
+      if (lm > 1e-12) { h[0] = m0 / lm; h[1] = m1 / lm; h[2] = m2 / lm; }
+      else { h[0] = 0.; h[1] = 0.; h[2] = 0.; }
+    } else {
+      h[0] = m0; h[1] = m1; h[2] = m2;
+    }
+  }
+  F[0] += fn0 + tf0;                                                // :324-326
+  F[1] += fn1 + tf1;
+  F[2] += fn2 + tf2;
+  if (!REF) {
+    T[0] += (ny * tf2 - nz * tf1) * (-a_i);
+    T[1] += (nz * tf0 - nx * tf2) * (-a_i);
+    T[2] += (nx * tf1 - ny * tf0) * (-a_i);
+  }
+}
+
+// Particle-wall contact (overlap > 0 already established).  sw is the
+// wall tangential history, updated in place.  src/CabanaDEM_Force.hpp.
+template <bool REF>
+__device__ inline void pw_contact(const double4& xi, const double* ui, const double4& mi,
+                                  const WallConst& W, double overlap, double* sw,
+                                  double* F, double* T, const StepArgs& a) {
+  const double nx = W.n[0], ny = W.n[1], nz = W.n[2];               // :509-511
+  const double a_i = xi.w - overlap / 2.;                           // :517
+  double v0, v1, v2;
+  if (REF) {
+    v0 = ui[0] + (ui[4] * nz - ui[5] * ny) * a_i;                   // :520-527
+    v1 = ui[1] + (ui[5] * nx - ui[3] * nz) * a_i;
+    v2 = ui[2] + (ui[3] * ny - ui[4] * nx) * a_i;
+  } else {
+    v0 = ui[0] - (ui[4] * nz - ui[5] * ny) * a_i;
+    v1 = ui[1] - (ui[5] * nx - ui[3] * nz) * a_i;
+    v2 = ui[2] - (ui[3] * ny - ui[4] * nx) * a_i;
+  }
+  const double vdn = v0 * nx + v1 * ny + v2 * nz;                   // :538
+  const double vn0 = vdn * nx, vn1 = vdn * ny, vn2 = vdn * nz;      // :539-541
+  const double vt0 = v0 - vn0, vt1 = v1 - vn1, vt2 = v2 - vn2;      // :544-546
+  const double E_eff = 1. / (mi.x + W.a1);                          // :561-563
+  const double R_eff = 1. / (mi.z + 0.);                            // :564-566
+  const double kn = 4. / 3. * E_eff * sqrt(R_eff);                  // :568
+  const double sRo = sqrt(R_eff * overlap);
+  const double S_n = 2. * E_eff * sRo;                              // :576
+  const double m_eff = 1. / (mi.w + 0.);                            // :577-579
+  const double eta_n = a.c_eta_pw * sqrt(S_n * m_eff);              // :580
+  const double fn = kn * (overlap * sqrt(overlap));                 // :583
+  const double fn0 = fn * nx - eta_n * vn0;                         // :584-586
+  const double fn1 = fn * ny - eta_n * vn1;
+  const double fn2 = fn * nz - eta_n * vn2;
+  double ft0 = 0., ft1 = 0., ft2 = 0.;
+  const double vmag = sqrt(v0 * v0 + v1 * v1 + v2 * v2);            // :597
+  if (vmag < 1e-12) {                                               // :599-603
+    sw[0] = 0.; sw[1] = 0.; sw[2] = 0.;
+  } else {
+    sw[0] += vt0 * a.dt; sw[1] += vt1 * a.dt; sw[2] += vt2 * a.dt;  // :607-609
+    const double G_eff = 1. / (mi.y + W.a2);                        // :612-614
+    const double kt = 8. * G_eff * sRo;                             // :616
+    const double eta_t = REF ? 0. : a.c_eta_pw * sqrt(kt * m_eff);  // :618 (Q2)
+    const double fs0 = -kt * sw[0] - eta_t * vt0;                   // :620-622
+    const double fs1 = -kt * sw[1] - eta_t * vt1;
+    const double fs2 = -kt * sw[2] - eta_t * vt2;
+    const double ftm = REF ? sqrt(fs0 * fs0 + fs1 * fs1 + fs2 * fs1) // :624 (Q6)
+                           : sqrt(fs0 * fs0 + fs1 * fs1 + fs2 * fs2);
+    double ti0 = 0., ti1 = 0., ti2 = 0.;
+    if (ftm > 1e-12) { ti0 = fs0 / ftm; ti1 = fs1 / ftm; ti2 = fs2 / ftm; } // :630-634
+    const double fnm = sqrt(fn0 * fn0 + fn1 * fn1 + fn2 * fn2);     // :636
+    const double lim = a.fr_pw * fnm;                               // :638 std::min(lim, ftm)
+    const double ftms = (ftm < lim) ? ftm : lim;
+    ft0 = ftms * ti0; ft1 = ftms * ti1; ft2 = ftms * ti2;           // :641-643
+    ft0 += eta_t * vt0; ft1 += eta_t * vt1; ft2 += eta_t * vt2;     // :646-648
+    sw[0] = -ft0 / kt; sw[1] = -ft1 / kt; sw[2] = -ft2 / kt;        // :651-653
+  }
+  F[0] += fn0 + ft0;                                                // :657-659
+  F[1] += fn1 + ft1;
+  F[2] += fn2 + ft2;
+  const double arm = REF ? a_i : -a_i;                              // :662-664 (Q5)
+  T[0] += (ny * ft2 - nz * ft1) * arm;
+  T[1] += (nz * ft0 - nx * ft2) * arm;
+  T[2] += (nx * ft1 - ny * ft0) * arm;
+}
+
+__device__ inline double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// Fixed-order block reduction of 8 doubles into part[blk][8].
+__device__ inline void block_reduce8(const double* v, double* part, int blk) {
+  __shared__ double s_red[kBlock / 64][8];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  double r[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) r[k] = wave_sum(v[k]);
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s_red[wid][k] = r[k];
+  __syncthreads();
+  if (threadIdx.x < 8) {
+    double s = 0.;
+    for (int w = 0; w < kBlock / 64; ++w) s += s_red[w][threadIdx.x];
+    part[(long)blk * 8 + threadIdx.x] = s;
+  }
+}
+
+// KICK: 0 = forces only (write F,T); 1 = + stage3 (write state, F,T);
+//       2 = + stage3 + stage1 + stage2 of the next step (write state).
+template <bool REF, bool PP, bool PW, int KICK, bool ACCUM, bool DIAG>
+__global__ __launch_bounds__(kBlock) void k_force(StepArgs a) {
+  extern __shared__ int s_cj[]; // [max_c][kBlock] contacts found in phase 1
+  const int tid = threadIdx.x;
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const int i = blk * kBlock + tid;
+  double dg[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
+  if (i < a.n) {
+    const long ld = a.ld;
+    const double4 xi = a.xr_in[i];
+    double ui[6];
+    load6(a.uw_in + 6l * i, ui);
+    const double4 mti = a.mat[i];
+    const double2 mI = a.mi[i];
+    double F[3], T[3];
+    if (ACCUM) {
+      double ft[6];
+      load6(a.ft + 6l * i, ft);
+      F[0] = ft[0]; F[1] = ft[1]; F[2] = ft[2];
+      T[0] = ft[3]; T[1] = ft[4]; T[2] = ft[5];
+    } else {
+      if (a.gravity_on) { F[0] = mI.x * a.g[0]; F[1] = mI.x * a.g[1]; F[2] = mI.x * a.g[2]; }
+      else { F[0] = 0.; F[1] = 0.; F[2] = 0.; }
+      T[0] = 0.; T[1] = 0.; T[2] = 0.;
+    }
+    int nnb = 0, nc = 0, nwc = 0;
+    if (PP) {
+      // Phase 1: contact detection over the neighbour row.
+      nnb = a.nn[i];
+      for (int k = 0; k < nnb; ++k) {
+        const int j = a.nbr[k * ld + i];
+        const double4 xj = a.xr_in[j];
+        const double d0 = xi.x - xj.x, d1 = xi.y - xj.y, d2 = xi.z - xj.z;
+        const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+        const double s = xi.w + xj.w;
+        // r2 < s^2 is necessary for sqrt(r2) < s; the 1e-12 margin covers
+        // the rounding of s*s, so no contact is ever filtered out here.
+        if (r2 < s * s * (1. + 1e-12)) {
+          if (s - sqrt(r2) > 0.) {
+            if (nc < a.max_c) s_cj[nc * kBlock + tid] = j;
+            ++nc;
+          }
+        }
+      }
+      if (nc > a.max_c) {
+        atomicMax(&a.flags->contact_overflow, nc);
+        nc = a.max_c;
+      }
+      // Phase 2: contact arithmetic, merged with last step's tracked list.
+      const int oc = a.ccnt_in[i];
+      int p = 0;
+      int jo = oc > 0 ? a.cj_in[i] : INT_MAX;
+      for (int m = 0; m < nc; ++m) {
+        const int j = s_cj[m * kBlock + tid];
+        while (jo < j) {
+          ++p;
+          jo = (p < oc) ? a.cj_in[p * ld + i] : INT_MAX;
+        }
+        double h[3] = {0., 0., 0.};
+        if (jo == j) {
+          h[0] = a.ch_in[(3 * p + 0) * ld + i];
+          h[1] = a.ch_in[(3 * p + 1) * ld + i];
+          h[2] = a.ch_in[(3 * p + 2) * ld + i];
+        }
+        const double4 xj = a.xr_in[j];
+        double uj[6];
+        load6(a.uw_in + 6l * j, uj);
+        const double4 mj = a.mat[j];
+        pp_contact<REF>(xi, ui, mti, xj, uj, mj, h, F, T, a);
+        a.cj_out[m * ld + i] = j;
+        a.ch_out[(3 * m + 0) * ld + i] = h[0];
+        a.ch_out[(3 * m + 1) * ld + i] = h[1];
+        a.ch_out[(3 * m + 2) * ld + i] = h[2];
+      }
+      a.ccnt_out[i] = nc;
+    }
+    if (PW) {
+      const int wm = a.wmask[i];
+      int wn = 0;
+      for (int w = 0; w < a.n_walls; ++w) {
+        const WallConst& W = a.walls[w];
+        const double q0 = xi.x - W.x[0], q1 = xi.y - W.x[1], q2 = xi.z - W.x[2]; // :497-499
+        const double tmp = q0 * W.n[0] + q1 * W.n[1] + q2 * W.n[2];              // :514
+        const double overlap = xi.w - tmp;                                        // :515
+        if (overlap > 0.) {                                                       // :556
+          double sw[3] = {0., 0., 0.};
+          if (wm & (1 << w)) {
+            sw[0] = a.sw[(3 * w + 0) * ld + i];
+            sw[1] = a.sw[(3 * w + 1) * ld + i];
+            sw[2] = a.sw[(3 * w + 2) * ld + i];
+          }
+          pw_contact<REF>(xi, ui, mti, W, overlap, sw, F, T, a);
+          a.sw[(3 * w + 0) * ld + i] = sw[0];
+          a.sw[(3 * w + 1) * ld + i] = sw[1];
+          a.sw[(3 * w + 2) * ld + i] = sw[2];
+          wn |= 1 << w;
+          ++nwc;
+        }
+        // no overlap: the history is zero (:666-670); the cleared bit says so
+      }
+      a.wmask[i] = wn;
+    }
+    if (KICK == 0) {
+      double ft[6] = {F[0], F[1], F[2], T[0], T[1], T[2]};
+      store6(a.ft + 6l * i, ft);
+      if (DIAG) {
+        const double I = a.Imom[i];
+        dg[0] = 0.5 * mI.x * (ui[0] * ui[0] + ui[1] * ui[1] + ui[2] * ui[2]);
+        dg[1] = 0.5 * I * (ui[3] * ui[3] + ui[4] * ui[4] + ui[5] * ui[5]);
+        dg[2] = mI.x * ui[0]; dg[3] = mI.x * ui[1]; dg[4] = mI.x * ui[2];
+      }
+    } else {
+      const double minv = mti.w, Iinv = mI.y;
+      double s[6];
+      // stage3: src/CabanaDEM_Integrator.hpp:89-98
+      s[0] = ui[0] + a.hdt * F[0] * minv;
+      s[1] = ui[1] + a.hdt * F[1] * minv;
+      s[2] = ui[2] + a.hdt * F[2] * minv;
+      s[3] = ui[3] + a.hdt * T[0] * Iinv;
+      s[4] = ui[4] + a.hdt * T[1] * Iinv;
+      s[5] = ui[5] + a.hdt * T[2] * Iinv;
+      if (DIAG) {
+        const double I = a.Imom[i];
+        dg[0] = 0.5 * mI.x * (s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+        dg[1] = 0.5 * I * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]);
+        dg[2] = mI.x * s[0]; dg[3] = mI.x * s[1]; dg[4] = mI.x * s[2];
+      }
+      if (KICK == 2) {
+        // stage1 of the next step (:42-51), then stage2 (:67-69)
+        for (int d = 0; d < 3; ++d) s[d] = s[d] + a.hdt * F[d] * minv;
+        for (int d = 0; d < 3; ++d) s[3 + d] = s[3 + d] + a.hdt * T[d] * Iinv;
+        double4 xo;
+        xo.x = xi.x + a.dt * s[0];
+        xo.y = xi.y + a.dt * s[1];
+        xo.z = xi.z + a.dt * s[2];
+        xo.w = xi.w;
+        a.xr_out[i] = xo;
+        store6(a.uw_out + 6l * i, s);
+      } else {
+        a.xr_out[i] = xi;
+        store6(a.uw_out + 6l * i, s);
+        double ft[6] = {F[0], F[1], F[2], T[0], T[1], T[2]};
+        store6(a.ft + 6l * i, ft);
+      }
+    }
+    dg[5] = (double)nnb;
+    dg[6] = (double)nc;
+    dg[7] = (double)nwc;
+  }
+  if (DIAG) block_reduce8(dg, a.diag_part, blk);
+}
+
+// F = m g, T = 0  (src/CabanaDEM_Force.hpp:39-47 plus the body force).
+__global__ __launch_bounds__(kBlock) void k_zero_forces(StepArgs a) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.n) return;
+  const double2 mI = a.mi[i];
+  double ft[6] = {0., 0., 0., 0., 0., 0.};
+  if (a.gravity_on) { ft[0] = mI.x * a.g[0]; ft[1] = mI.x * a.g[1]; ft[2] = mI.x * a.g[2]; }
+  store6(a.ft + 6l * i, ft);
+}
+
+// src/CabanaDEM_Force.hpp:343-445: keep the tracked contacts that still
+// overlap.  The list stays ascending in j; the reference's swap-remove slot
+// order is not kept (the history is compared as a map keyed by (i, j)).
+__global__ __launch_bounds__(kBlock) void k_update_tangential(StepArgs a) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.n) return;
+  const long ld = a.ld;
+  const double4 xi = a.xr_in[i];
+  const int oc = a.ccnt_in[i];
+  int m = 0;
+  for (int q = 0; q < oc; ++q) {
+    const int j = a.cj_in[q * ld + i];
+    const double4 xj = a.xr_in[j];
+    const double p0 = xi.x - xj.x, p1 = xi.y - xj.y, p2 = xi.z - xj.z; // :388-390
+    const double r2 = p0 * p0 + p1 * p1 + p2 * p2;
+    const double overlap = xi.w + xj.w - sqrt(r2);                    // :397
+    if (overlap <= 0.) continue;                                      // :399
+    a.cj_out[m * ld + i] = j;
+    for (int c = 0; c < 3; ++c) a.ch_out[(3 * m + c) * ld + i] = a.ch_in[(3 * q + c) * ld + i];
+    ++m;
+  }
+  a.ccnt_out[i] = m;
+}
+
+// Half kick (stage1/stage3) and/or drift (stage2), in place.
+__global__ __launch_bounds__(kBlock) void k_kick(StepArgs a, double4* xr, double* uw,
+                                                 const double* ftp, int do_kick, int do_drift) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.n) return;
+  double s[6];
+  load6(uw + 6l * i, s);
+  if (do_kick) {
+    double ft[6];
+    load6(ftp + 6l * i, ft);
+    const double minv = a.mat[i].w, Iinv = a.mi[i].y;
+    for (int d = 0; d < 3; ++d) s[d] = s[d] + a.hdt * ft[d] * minv;
+    for (int d = 0; d < 3; ++d) s[3 + d] = s[3 + d] + a.hdt * ft[3 + d] * Iinv;
+    store6(uw + 6l * i, s);
+  }
+  if (do_drift) {
+    double4 x = xr[i];
+    x.x = x.x + a.dt * s[0];
+    x.y = x.y + a.dt * s[1];
+    x.z = x.z + a.dt * s[2];
+    xr[i] = x;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_diag_current(StepArgs a, const double* uw) {
+  const int blk = blockIdx.x;
+  const int i = blk * kBlock + threadIdx.x;
+  double dg[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
+  if (i < a.n) {
+    double s[6];
+    load6(uw + 6l * i, s);
+    const double m = a.mi[i].x, I = a.Imom[i];
+    dg[0] = 0.5 * m * (s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+    dg[1] = 0.5 * I * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]);
+    dg[2] = m * s[0]; dg[3] = m * s[1]; dg[4] = m * s[2];
+    dg[5] = (double)a.nn[i];
+    dg[6] = (double)a.ccnt_in[i];
+    dg[7] = (double)__popc(a.wmask[i]);
+  }
+  block_reduce8(dg, a.diag_part, blk);
+}
+
+// Sum block partials in block order; row = (step, 8 sums).
+__global__ __launch_bounds__(kBlock) void k_diag_final(const double* part, int nblocks,
+                                                       double* row, double step) {
+  double v[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
+  for (int b = threadIdx.x; b < nblocks; b += kBlock)
+    for (int k = 0; k < 8; ++k) v[k] += part[(long)b * 8 + k];
+  __shared__ double tmp[8];
+  double* scratch = tmp;
+  // block_reduce8 writes part-style output; reuse it into a local slot.
+  block_reduce8(v, scratch, 0);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    row[0] = step;
+    for (int k = 0; k < 8; ++k) row[1 + k] = scratch[k];
+  }
+}
+
+inline unsigned nblk(long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+template <bool REF>
+hipError_t launch_force_mode(ForceKind kind, const StepArgs& a, bool sample, hipStream_t s) {
+  const size_t lds = (size_t)(a.max_c > 0 ? a.max_c : 1) * kBlock * sizeof(int);
+  const dim3 g(nblk(a.n)), b(kBlock);
+  switch (kind) {
+    case FK_PP_PW_FORCES:
+      if (sample) hipLaunchKernelGGL((k_force<REF, true, true, 0, false, true>), g, b, lds, s, a);
+      else hipLaunchKernelGGL((k_force<REF, true, true, 0, false, false>), g, b, lds, s, a);
+      break;
+    case FK_PP_ACCUM:
+      hipLaunchKernelGGL((k_force<REF, true, false, 0, true, false>), g, b, lds, s, a);
+      break;
+    case FK_PW_ACCUM:
+      hipLaunchKernelGGL((k_force<REF, false, true, 0, true, false>), g, b, lds, s, a);
+      break;
+    case FK_STEP_FUSED:
+      if (sample) hipLaunchKernelGGL((k_force<REF, true, true, 2, false, true>), g, b, lds, s, a);
+      else hipLaunchKernelGGL((k_force<REF, true, true, 2, false, false>), g, b, lds, s, a);
+      break;
+    case FK_STEP_LAST:
+      if (sample) hipLaunchKernelGGL((k_force<REF, true, true, 1, false, true>), g, b, lds, s, a);
+      else hipLaunchKernelGGL((k_force<REF, true, true, 1, false, false>), g, b, lds, s, a);
+      break;
+  }
+  return hipGetLastError();
+}
+
+} // namespace
+
+int force_blocks(int n) { return (int)nblk(n); }
+
+hipError_t launch_force(ForceKind kind, const StepArgs& a, bool sample, hipStream_t s) {
+  if (a.physics_mode == DEM_MODE_REFERENCE) return launch_force_mode<true>(kind, a, sample, s);
+  return launch_force_mode<false>(kind, a, sample, s);
+}
+
+hipError_t launch_zero_forces(const StepArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_forces, dim3(nblk(a.n)), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_update_tangential(const StepArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_update_tangential, dim3(nblk(a.n)), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_kick(const StepArgs& a, double4* xr, double* uw, const double* ft,
+                       int do_kick, int do_drift, hipStream_t s) {
+  hipLaunchKernelGGL(k_kick, dim3(nblk(a.n)), dim3(kBlock), 0, s, a, xr, uw, ft, do_kick, do_drift);
+  return hipGetLastError();
+}
+
+hipError_t launch_diag_current(const StepArgs& a, const double* uw, hipStream_t s) {
+  hipLaunchKernelGGL(k_diag_current, dim3(nblk(a.n)), dim3(kBlock), 0, s, a, uw);
+  return hipGetLastError();
+}
+
+hipError_t launch_diag_final(const double* part, int nblocks, double* row, double step,
+                             hipStream_t s) {
+  hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(kBlock), 0, s, part, nblocks, row, step);
+  return hipGetLastError();
+}
+
+} // namespace demhip

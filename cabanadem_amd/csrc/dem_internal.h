@@ -1,0 +1,164 @@
+// dem_internal.h -- device data layout and kernel launchers of the MI355X
+// DEM time-step library.  Not part of the public ABI (include/cabanadem.h).
+//
+// Layout in HBM (all FP64 state stays resident; particles are stored in
+// cell order and re-sorted at every neighbour rebuild):
+//   xr   double4[n]        x, y, z, radius               (gathered for neighbours)
+//   uw   double[n][6]      u, omega                      (gathered for contacts)
+//   mat  double4[n]        (1-nu^2)/E, (2-nu)/G, 1/r, 1/m (contact constants)
+//   mi   double2[n]        m, 1/I                        (own-particle only)
+//   Imom double[n]         I                              (diagnostics)
+//   ft   double[n][6]      F, T
+//   id   int[n]            original particle id of each sorted slot
+//   nbr  int[max_n][ld]    neighbour rows, slot-major (lane-coalesced), ascending j
+//   ccnt int[n], cj int[max_c][ld], ch double[max_c*3][ld]
+//                          tracked pp contacts: compact, ascending j, with the
+//                          tangential history of each
+//   wmask int[n], sw double[DEM_MAX_WALLS*3][ld]
+//                          wall-contact bits and wall tangential history
+// Arrays written by the fused step kernel or by the re-sort are double
+// buffered ([2]) and swapped by the host.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "../../include/cabanadem.h"
+
+namespace demhip {
+
+constexpr int kBlock = 256;      // threads per block of the particle kernels
+constexpr int kMaxContacts = 32; // upper bound of max_contacts
+constexpr int kXcds = 8;         // MI355X: 8 XCDs, workgroups dealt round-robin
+
+// Logical block index that gives each XCD one contiguous run of blocks, so
+// that the neighbour gathers of one XCD stay inside its own L2.  Workgroup b
+// of a launch goes to XCD b % 8; this maps XCD k to the k-th contiguous
+// share of [0, nb).  Bijective for any nb.
+__device__ inline int xcd_block(int b, int nb) {
+  const int q = nb / kXcds, r = nb % kXcds;
+  const int xcd = b % kXcds, local = b / kXcds;
+  return (xcd < r) ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
+}
+
+struct WallConst {
+  double x[3];
+  double n[3];
+  double a1; // (1 - nu_w^2) / E_w
+  double a2; // (2 - nu_w) / G_w
+};
+
+struct Grid {
+  double lo[3];
+  double inv_h[3]; // unused by the exact path; kept for diagnostics
+  double h[3];
+  int nc[3];
+  int pad;
+};
+
+struct DevFlags {
+  int nonfinite;
+  int nbr_overflow;   // largest row that did not fit
+  int max_nn;         // largest row of the last build
+  int contact_overflow;
+  int skin;
+  int bond_overflow;
+  int cell_overflow;  // a cell held more than kMaxCellParticles particles
+  int pad;
+};
+
+// Upper bound on particles in one cell; beyond it the binning box is far too
+// coarse for the cutoff and the rebuild stops with an error instead of
+// running quadratic work.
+constexpr int kMaxCellParticles = 4096;
+
+// Pointers and scalars of one launch, resolved for the current buffers.
+struct StepArgs {
+  int n, ld, n_walls, max_n, max_c;
+  int physics_mode;
+  double dt, hdt;
+  double g[3];
+  int gravity_on;
+  double c_eta_pp, c_eta_pw;   // (-2 sqrt(5/6)) * beta
+  double fr_pp, fr_pw;
+  WallConst walls[DEM_MAX_WALLS];
+
+  const double4* xr_in;
+  double4* xr_out;
+  const double* uw_in;
+  double* uw_out;
+  const double4* mat;
+  const double2* mi;
+  const double* Imom;
+  double* ft;
+  int* wmask;
+  double* sw;
+  const int* nn;
+  const int* nbr;
+  const int* ccnt_in;
+  const int* cj_in;
+  const double* ch_in;
+  int* ccnt_out;
+  int* cj_out;
+  double* ch_out;
+  DevFlags* flags;
+  double* diag_part; // [blocks][8] when sampling
+};
+
+struct BuildArgs {
+  int n, ld, max_n, max_c;
+  Grid grid;
+  long ncell;
+  double rc2;
+  double skin_half_sq;
+  int check_skin;
+  // old (cur) and new (nxt) buffers
+  const double4* xr_in; double4* xr_out;
+  const double* uw_in; double* uw_out;
+  const double4* mat_in; double4* mat_out;
+  const double2* mi_in; double2* mi_out;
+  const double* I_in; double* I_out;
+  const int* id_in; int* id_out;
+  const double* ft_in; double* ft_out;
+  const int* wmask_in; int* wmask_out;
+  const double* sw_in; double* sw_out;
+  const int* ccnt_in; int* ccnt_out;
+  const int* cj_in; int* cj_out;
+  const double* ch_in; double* ch_out;
+  double* xb;   // positions at the last build, sorted order
+  int* key;     // cell key, old order
+  int* rank;    // arrival rank in the cell, old order
+  int* key_sorted;
+  int* cell_count;
+  int* cell_start;
+  int* perm;    // new -> old
+  int* inv;     // old -> new
+  int* scan_tmp;
+  int* nn;
+  int* nbr;
+  DevFlags* flags;
+};
+
+// Launchers (defined in dem_build.hip / dem_force.hip); all enqueue on `s`.
+hipError_t launch_rebuild(const BuildArgs& a, hipStream_t s);
+hipError_t launch_neighbors(const BuildArgs& a, hipStream_t s);
+hipError_t launch_fill_int(int* p, long count, int value, hipStream_t s);
+
+enum ForceKind {
+  FK_PP_PW_FORCES = 0, // zero + update + pp + pw, write F,T
+  FK_PP_ACCUM = 1,     // F,T += pp (contact list updated)
+  FK_PW_ACCUM = 2,     // F,T += pw
+  FK_STEP_FUSED = 3,   // forces + stage3 + stage1 + stage2 of the next step
+  FK_STEP_LAST = 4     // forces + stage3, write F,T
+};
+hipError_t launch_force(ForceKind kind, const StepArgs& a, bool sample, hipStream_t s);
+hipError_t launch_zero_forces(const StepArgs& a, hipStream_t s);
+hipError_t launch_update_tangential(const StepArgs& a, hipStream_t s);
+hipError_t launch_kick(const StepArgs& a, double4* xr, double* uw, const double* ft,
+                       int do_kick, int do_drift, hipStream_t s);
+hipError_t launch_diag_current(const StepArgs& a, const double* uw, hipStream_t s);
+hipError_t launch_diag_final(const double* part, int nblocks, double* out_row,
+                             double step, hipStream_t s);
+int force_blocks(int n);
+
+} // namespace demhip

@@ -1,0 +1,202 @@
+/*
+ * cabanadem.h -- C ABI of the MI355X-native CabanaDEM time-step library
+ * (libcabanadem_hip.so, built from cabanadem_amd/csrc).
+ *
+ * The reference exposes a header-only C++ template API and no C ABI; each
+ * entry point below names the reference interface it stands in for
+ * (paths relative to the ExaPhysics/CabanaDEM root).  The C++ drop-in headers
+ * in this directory (CabanaDEM.hpp and friends) are the reference-shaped API
+ * built on top of these calls.
+ *
+ * Conventions
+ *  - Opaque context per simulated system; one host thread per context.
+ *  - All calls return 0 on success or a negative DEM_ERR_* code; the text of
+ *    the last error is available from dem_last_error().  No exception crosses
+ *    the ABI.
+ *  - Host arrays are in the reference layout: vectors xyz-interleaved
+ *    double[n][3], histories row-major T[n][max] (src/CabanaDEM_Particles.hpp:40-46).
+ *    Host pointers are borrowed for the duration of the call only.
+ *  - Device work is stream ordered on the context's own stream; calls that
+ *    return host data synchronise.  The context owns all device memory.
+ *  - physics_mode 0 ("reference") reproduces the reference's arithmetic,
+ *    including its quirks; 1 ("corrected") fixes them.  Both move the same
+ *    bytes.
+ */
+#ifndef CABANADEM_H
+#define CABANADEM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DEM_MAX_WALLS 8
+#define DEM_ABI_VERSION 1
+
+enum dem_status {
+  DEM_OK = 0,
+  DEM_ERR_ARG = -1,            /* invalid argument / configuration */
+  DEM_ERR_HIP = -2,            /* HIP runtime failure (incl. no device) */
+  DEM_ERR_NEIGH_OVERFLOW = -3, /* a neighbour row exceeded max_neighbors */
+  DEM_ERR_CONTACT_OVERFLOW = -4, /* more tracked contacts than max_contacts (MaxCnts) */
+  DEM_ERR_NONFINITE = -5,      /* non-finite position met by the cell binning */
+  DEM_ERR_SKIN = -6,           /* a particle moved more than skin/2 between rebuilds */
+  DEM_ERR_BOND_OVERFLOW = -7,  /* more bonds than max_bonds (MaxBonds) */
+  DEM_ERR_STATE = -8           /* call not valid in the current state */
+};
+
+enum dem_physics_mode { DEM_MODE_REFERENCE = 0, DEM_MODE_CORRECTED = 1 };
+
+typedef struct dem_ctx dem_ctx;
+
+/* Static configuration of one system.  Replaces the constructor arguments
+ * spread over Force (src/CabanaDEM_Force.hpp:23-28: cor_pp, cor_pw,
+ * friction_pp, friction_pw), Integrator (src/CabanaDEM_Integrator.hpp:22-27:
+ * dt), SolverDEM (src/CabanaDEM_Solver.hpp:32-64: delta, mesh_lo/hi,
+ * cell_ratio 1.0) and the Particles template capacities MaxCnts/MaxNoWalls
+ * (src/CabanaDEM_Particles.hpp:22). */
+typedef struct dem_config {
+  int n;               /* particles */
+  int n_walls;         /* infinite walls, <= DEM_MAX_WALLS */
+  int max_neighbors;   /* neighbour-row capacity; 0 = sized from the first build */
+  int max_contacts;    /* tracked pp contacts per particle (MaxCnts), <= 32 */
+  double dt;
+  double cor_pp, cor_pw, friction_pp, friction_pw;
+  double gravity[3];   /* body force per unit mass (0 reproduces the reference) */
+  double cutoff;       /* neighbour cutoff (the reference's `delta`) */
+  double skin;         /* > 0: flag DEM_ERR_SKIN if a particle moves > skin/2 between rebuilds */
+  double lo[3], hi[3]; /* binning box (mesh_lo / mesh_hi); outside particles are clamped in */
+  int rebuild_every;   /* neighbour rebuild cadence K in steps (1 = every step, as the reference) */
+  int physics_mode;    /* DEM_MODE_* */
+  int block_size;      /* threads per block for the particle kernels (0 = 256) */
+  int reserved;
+} dem_config;
+
+/* Host view of the particle state in the reference layout
+ * (fields of src/CabanaDEM_Particles.hpp:348-370).  NULL pointers are
+ * skipped on upload/download.  max_cnts / max_walls give the row length of
+ * the tangential-history arrays (the MaxCnts / MaxNoWalls template
+ * parameters). */
+typedef struct dem_particles_view {
+  int n;
+  int max_cnts;
+  int max_walls;
+  int reserved;
+  double *x, *u, *omega, *force, *torque; /* [n][3] */
+  double *m, *rad, *E, *nu, *G, *I;       /* [n] */
+  double *sw_x, *sw_y, *sw_z;             /* [n][max_walls] wall tangential history */
+  double *ss_x, *ss_y, *ss_z;             /* [n][max_cnts]  pp tangential history */
+  int *ss_idx;                            /* [n][max_cnts]  tracked neighbour ids, -1 = empty */
+  int *ss_cnt;                            /* [n]            tracked contacts */
+} dem_particles_view;
+
+/* Host view of the walls (fields of src/CabanaDEM_Wall.hpp:192-201; wall u
+ * and omega are not read by the reference force kernel, :468-469). */
+typedef struct dem_walls_view {
+  int n;
+  int reserved;
+  const double *x, *normal; /* [n][3] */
+  const double *E, *nu, *G; /* [n] */
+} dem_walls_view;
+
+/* Run-average counters (extension; the reference reports none). */
+typedef struct dem_counters {
+  int64_t steps;          /* steps taken by this context */
+  int64_t rebuilds;       /* neighbour rebuilds */
+  int max_neighbors_seen; /* largest neighbour row at the last rebuild */
+  int neighbor_capacity;  /* current row capacity */
+  double mean_neighbors;  /* n-bar at the last sampled step */
+  double mean_contacts;   /* c-bar (tracked pp contacts) at the last sampled step */
+  double mean_wall_contacts; /* w-bar at the last sampled step */
+} dem_counters;
+
+/* ---- lifecycle ---------------------------------------------------------- */
+/* Replaces: createSolverDEM / SolverDEM::SolverDEM (src/CabanaDEM_Solver.hpp:32-64,127-138)
+ * together with the device allocation of Particles::resize (src/CabanaDEM_Particles.hpp:273-297). */
+int dem_create(const dem_config* cfg, dem_ctx** out);
+void dem_destroy(dem_ctx* ctx);
+const char* dem_last_error(const dem_ctx* ctx); /* ctx may be NULL: last global error */
+int dem_abi_version(void);
+int dem_device_count(void);
+
+/* ---- transfer ------------------------------------------------------------ */
+/* Replaces: Particles::updateParticles (src/CabanaDEM_Particles.hpp:90-97) -- the
+ * user functor runs on the host mirror and the result is uploaded here. */
+int dem_upload_particles(dem_ctx* ctx, const dem_particles_view* v);
+/* Replaces: host access through the Particles slices after a run
+ * (slice getters src/CabanaDEM_Particles.hpp:99-271). */
+int dem_download_particles(dem_ctx* ctx, dem_particles_view* v);
+/* Replaces: Wall::updateParticles (src/CabanaDEM_Wall.hpp:75-82). */
+int dem_upload_walls(dem_ctx* ctx, const dem_walls_view* w);
+
+/* ---- neighbour list ------------------------------------------------------ */
+/* Replaces: Cabana::VerletList<...,FullNeighborTag,...>::build as called at
+ * src/CabanaDEM_Solver.hpp:87-88 (and the constructor call :60-63).  Sorts
+ * the particles by cell, rebuilds the full list (every j != i with
+ * |x_i - x_j|^2 <= cutoff^2) and carries the tangential history across. */
+int dem_build_neighbors(dem_ctx* ctx);
+/* Returns the current list in original particle ids, each row in ascending
+ * id order: counts[n], ids[n][max_n].  For parity checks. */
+int dem_get_neighbors(dem_ctx* ctx, int* counts, int* ids, int max_n);
+
+/* ---- force stages ---------------------------------------------------------- */
+/* Replaces: Force::makeForceTorqueZeroOnParticle (src/CabanaDEM_Force.hpp:39-47);
+ * F starts at m*gravity. */
+int dem_zero_forces(dem_ctx* ctx);
+/* Replaces: Force::update_tangential_contacts (src/CabanaDEM_Force.hpp:343-445). */
+int dem_update_tangential(dem_ctx* ctx);
+/* Replaces: Force::computeForceFullParticleParticle (src/CabanaDEM_Force.hpp:49-341). */
+int dem_force_pp(dem_ctx* ctx);
+/* Replaces: Force::computeForceFullParticleInfiniteWall (src/CabanaDEM_Force.hpp:447-678). */
+int dem_force_pw(dem_ctx* ctx);
+/* Replaces: computeForceParticleParticleInfiniteWall (src/CabanaDEM_Force.hpp:698-710):
+ * zero + update tangential + pp + pw in one kernel. */
+int dem_compute_forces(dem_ctx* ctx);
+
+/* ---- integrator -------------------------------------------------------------- */
+/* Replace: Integrator::stage1 / stage2 / stage3 (src/CabanaDEM_Integrator.hpp:31-56, 59-74, 77-103). */
+int dem_stage1(dem_ctx* ctx);
+int dem_stage2(dem_ctx* ctx);
+int dem_stage3(dem_ctx* ctx);
+
+/* ---- whole steps ------------------------------------------------------------- */
+/* Replaces: the body of SolverDEM::run (src/CabanaDEM_Solver.hpp:79-97) for
+ * nsteps steps: stage1, stage2, rebuild every rebuild_every steps, forces,
+ * stage3.  Consecutive steps are fused: one kernel computes the forces of
+ * step s, the closing half kick of s and the opening kick and drift of s+1. */
+int dem_step(dem_ctx* ctx, int nsteps);
+/* Sample KE/momentum (and n-bar, c-bar, w-bar) after stage3 of every step
+ * whose global index is a multiple of every (0 = off).  Extension. */
+int dem_set_diag_every(dem_ctx* ctx, int every);
+/* Rows of 9 doubles: step, KE_trans, KE_rot, px, py, pz, n-bar, c-bar, w-bar. */
+int dem_get_diag_history(dem_ctx* ctx, double* rows, int cap, int* nrows);
+/* Current KE_trans, KE_rot, px, py, pz (synchronises). */
+int dem_diagnostics(dem_ctx* ctx, double out[5]);
+int dem_get_counters(dem_ctx* ctx, dem_counters* out);
+int dem_synchronize(dem_ctx* ctx);
+
+/* ---- kernel timing (HIP events on the context's stream) ------------------ */
+/* Enables per-launch HIP-event timing of the kernels dem_step launches. */
+int dem_set_timing(dem_ctx* ctx, int enable);
+/* Accumulated milliseconds and launch counts for the fused step kernel
+ * (slot 0) and the rebuild sequence (slot 1). */
+int dem_get_timing(dem_ctx* ctx, double ms[2], int64_t launches[2]);
+
+/* ---- synthetic inputs (host-only, no device needed) ------------------------ */
+/* Jittered-lattice packing: nx*ny*nz particles at spacing `spacing` from
+ * origin lo, positions jittered U(-jitter, jitter), radii U(rmin, rmax),
+ * velocities N(0, vsigma) per component; m = 4/3 pi r^3 rho and
+ * I = 2/5 m r^2 as the reference drivers set them (examples/01_...cpp:106-107).
+ * Counter-based splitmix64 stream keyed by (seed, particle, draw). */
+int dem_generate_lattice(int nx, int ny, int nz, double spacing, const double lo[3],
+                         double jitter, double rmin, double rmax, double vsigma,
+                         double rho, double E, double nu, uint64_t seed,
+                         double* x, double* u, double* rad, double* m, double* I,
+                         double* Ey, double* nuv, double* G);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

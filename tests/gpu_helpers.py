@@ -1,0 +1,80 @@
+"""Shared set-ups for the GPU parity tests (the oracle is the checker)."""
+from __future__ import annotations
+
+import numpy as np
+
+import oracle_py as O
+from cabanadem_amd import abi
+from scenarios import box_walls
+
+
+def normwise(a, b):
+    """||a - b||_2 / ||b||_2 (0 when both vanish)."""
+    a = np.asarray(a, dtype=np.float64).ravel()
+    b = np.asarray(b, dtype=np.float64).ravel()
+    nb = np.linalg.norm(b)
+    d = np.linalg.norm(a - b)
+    return 0.0 if d == 0.0 else d / (nb if nb > 0 else 1.0)
+
+
+def history_map(st):
+    """(i, j) -> (hx, hy, hz) of the tracked pp contacts (order-free)."""
+    out = {}
+    for i in range(st.n):
+        for k in range(int(st.ss_cnt[i])):
+            j = int(st.ss_idx[i, k])
+            out[(i, j)] = (st.ss_x[i, k], st.ss_y[i, k], st.ss_z[i, k])
+    return out
+
+
+def compare_history(gpu_st, ref_st, tol):
+    hg, hr = history_map(gpu_st), history_map(ref_st)
+    assert set(hg) == set(hr), (len(set(hg) ^ set(hr)), "tracked contact sets differ")
+    keys = sorted(hr)
+    if keys:
+        a = np.array([hg[k] for k in keys])
+        b = np.array([hr[k] for k in keys])
+        assert normwise(a, b) <= tol, normwise(a, b)
+    return len(keys)
+
+
+def granular_box(nx, ny, nz, *, spacing_r=2.05, r=1e-3, rmin=None, rmax=None, jitter_r=0.02,
+                 vsigma=0.05, omega_sigma=0.0, seed=1, E=1e7, nu=0.3, rho=2600., max_cnts=16,
+                 wall_gap_r=1.0):
+    """Jittered lattice in a six-wall box (the plain-DEM small configuration's shape)."""
+    rmin = r if rmin is None else rmin
+    rmax = r if rmax is None else rmax
+    sp = spacing_r * r
+    g = abi.generate_lattice(nx, ny, nz, sp, (0., 0., 0.), jitter_r * r, rmin, rmax, vsigma,
+                             rho, E, nu, seed)
+    n = nx * ny * nz
+    st = abi.ParticleState(n, max_cnts=max_cnts, max_walls=6)
+    st.x[:] = g["x"]
+    st.u[:] = g["u"]
+    st.rad[:], st.m[:], st.I[:] = g["rad"], g["m"], g["I"]
+    st.E[:], st.nu[:], st.G[:] = g["E"], g["nu"], g["G"]
+    if omega_sigma > 0:
+        rng = np.random.default_rng(seed + 100)
+        st.omega[:] = rng.normal(0., omega_sigma, size=(n, 3))
+    lo = np.array([-wall_gap_r * sp] * 3)
+    hi = np.array([(nx - 1 + wall_gap_r) * sp, (ny - 1 + wall_gap_r) * sp, (nz - 1 + wall_gap_r) * sp])
+    w = box_walls(lo, hi, E, nu, E / (2 * (1 + nu)))
+    return st, w, lo, hi
+
+
+def make_system(st, w, *, dt, rc, lo, hi, cor_pp=0.9, cor_pw=0.9, fr_pp=0.3, fr_pw=0.3,
+                gravity=(0., 0., 0.), rebuild_every=1, mode=0, max_contacts=16, skin=0.0,
+                max_neighbors=0):
+    sysm = abi.System(st.n, dt=dt, cutoff=rc, lo=lo, hi=hi, n_walls=0 if w is None else w.n,
+                      max_contacts=max_contacts, cor_pp=cor_pp, cor_pw=cor_pw,
+                      friction_pp=fr_pp, friction_pw=fr_pw, gravity=gravity, skin=skin,
+                      rebuild_every=rebuild_every, physics_mode=mode, max_neighbors=max_neighbors)
+    if w is not None:
+        sysm.upload_walls(w)
+    sysm.upload(st)
+    return sysm
+
+
+def oracle_params(dt, cor_pp=0.9, cor_pw=0.9, fr_pp=0.3, fr_pw=0.3, gravity=(0., 0., 0.), mode=0):
+    return O.params(dt=dt, cor_pp=cor_pp, cor_pw=cor_pw, fr_pp=fr_pp, fr_pw=fr_pw,
+                    gravity=gravity, mode=mode)

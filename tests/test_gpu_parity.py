@@ -1,0 +1,168 @@
+"""GPU parity against the CPU oracle, through the C ABI.
+
+Bar (from the project's contract): neighbour sets bit-exact; per-step
+forces, torques and tangential histories within 1e-12 relative (normwise)
+on identical states; integrator stages bit-exact; kinetic-energy and
+momentum histories within 1e-8 relative over a run."""
+import numpy as np
+import pytest
+
+import oracle_py as O
+from cabanadem_amd import abi
+from gpu_helpers import (compare_history, granular_box, make_system, normwise,
+                         oracle_params)
+
+pytestmark = pytest.mark.gpu
+
+TOL_STEP = 1e-12
+TOL_RUN = 1e-8
+
+
+def _rows(counts, ids):
+    return [tuple(sorted(ids[i, : counts[i]].tolist())) for i in range(len(counts))]
+
+
+def _dense_state(seed=3, mode=0, steps=40):
+    """A packing that is mid-collision: overlapping contacts, non-zero
+    tangential histories and wall contacts, prepared by the oracle."""
+    st, w, lo, hi = granular_box(6, 6, 6, spacing_r=1.97, jitter_r=0.03, vsigma=0.2,
+                                 omega_sigma=50., seed=seed, wall_gap_r=0.98)
+    dt, rc = 1e-6, 2.4e-3
+    prm = oracle_params(dt, gravity=(0., 0., -9.81), mode=mode)
+    O.run(st, w, prm, nsteps=steps, rc=rc, lo=lo - 1e-3, hi=hi + 1e-3, rebuild_every=1, max_n=64)
+    return st, w, lo - 1e-3, hi + 1e-3, dt, rc
+
+
+def test_neighbor_sets_bit_exact(gpu_lib):
+    st, w, lo, hi = granular_box(12, 10, 9, spacing_r=2.05, jitter_r=0.02, seed=5)
+    rc = 2.2e-3
+    sysm = make_system(st, w, dt=1e-6, rc=rc, lo=lo, hi=hi)
+    sysm.build_neighbors()
+    c_g, i_g = sysm.neighbors(64)
+    c_o, i_o = O.neighbors_cell(st.x, rc, lo, hi)
+    c_b, i_b = O.neighbors_brute(st.x, rc)
+    assert _rows(c_g, i_g) == _rows(c_o, i_o) == _rows(c_b, i_b)
+    assert c_g.sum() > 0
+
+
+def test_neighbor_sets_ties_clamped_and_after_resort(gpu_lib):
+    # exact ties (spacing == cutoff in binary fractions), particles outside
+    # the binning box, then a re-sort after motion
+    rc = 0.25
+    g = np.arange(10) * rc
+    x = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    x = np.concatenate([x, [[4.0, 4.0, 4.0], [4.0, 4.0, 4.25], [-2.0, 0.0, 0.0]]])
+    n = len(x)
+    st = abi.ParticleState(n, max_cnts=16, max_walls=6)
+    st.x[:] = x
+    st.rad[:] = 0.1
+    sysm = make_system(st, None, dt=1e-3, rc=rc, lo=(0., 0., 0.), hi=(2., 2., 2.))
+    sysm.build_neighbors()
+    c_g, i_g = sysm.neighbors(64)
+    c_b, i_b = O.neighbors_brute(x, rc)
+    assert _rows(c_g, i_g) == _rows(c_b, i_b)
+    rng = np.random.default_rng(0)
+    st.x[:] = x + rng.uniform(-0.05, 0.05, size=x.shape)
+    sysm.upload(st, fields=("x",))
+    for _ in range(2):
+        sysm.build_neighbors()
+    c_g, i_g = sysm.neighbors(64)
+    c_b, i_b = O.neighbors_brute(st.x, rc)
+    assert _rows(c_g, i_g) == _rows(c_b, i_b)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_forces_torques_history_identical_state(gpu_lib, mode):
+    st, w, lo, hi, dt, rc = _dense_state(mode=mode)
+    assert st.ss_cnt.sum() > 100  # mid-collision, histories carried
+    ref = st.copy()
+    prm = oracle_params(dt, gravity=(0., 0., -9.81), mode=mode)
+    c, i = O.neighbors_cell(ref.x, rc, lo, hi)
+    O.compute_forces(ref, w, c, i, prm)
+
+    sysm = make_system(st, w, dt=dt, rc=rc, lo=lo, hi=hi, gravity=(0., 0., -9.81), mode=mode)
+    sysm.build_neighbors()
+    sysm.compute_forces()
+    got = sysm.download(abi.ParticleState(st.n, 16, 6))
+    assert normwise(got.force, ref.force) <= TOL_STEP
+    assert normwise(got.torque, ref.torque) <= TOL_STEP
+    assert np.abs(ref.torque).max() > 0  # wall contacts (and pp in corrected mode) act
+    ncont = compare_history(got, ref, TOL_STEP)
+    assert ncont > 100
+    for comp in ("sw_x", "sw_y", "sw_z"):
+        assert normwise(getattr(got, comp), getattr(ref, comp)) <= TOL_STEP
+    assert np.abs(ref.sw_x).max() + np.abs(ref.sw_y).max() > 0
+    assert np.array_equal(got.ss_cnt, ref.ss_cnt)
+
+
+def test_stagewise_calls_match_fused_forces(gpu_lib):
+    st, w, lo, hi, dt, rc = _dense_state(seed=4)
+    a = make_system(st, w, dt=dt, rc=rc, lo=lo, hi=hi, gravity=(0., 0., -9.81))
+    b = make_system(st, w, dt=dt, rc=rc, lo=lo, hi=hi, gravity=(0., 0., -9.81))
+    for s in (a, b):
+        s.build_neighbors()
+    a.compute_forces()
+    b.zero_forces()
+    b.update_tangential()
+    b.force_pp()
+    b.force_pw()
+    ga, gb = a.download(), b.download()
+    assert np.array_equal(ga.force, gb.force) and np.array_equal(ga.torque, gb.torque)
+    assert np.array_equal(ga.ss_cnt, gb.ss_cnt)
+
+
+def test_update_tangential_drops_separated_contacts(gpu_lib):
+    st, w, lo, hi, dt, rc = _dense_state(seed=6)
+    ref = st.copy()
+    st_sep = st.copy()
+    st_sep.x[:] *= 1.05  # pull every pair apart a little
+    ref.x[:] = st_sep.x
+    O.update_tangential(ref)
+    sysm = make_system(st_sep, w, dt=dt, rc=rc, lo=lo, hi=hi)
+    sysm.update_tangential()
+    got = sysm.download()
+    compare_history(got, ref, 0.0)
+
+
+def test_integrator_stages_bit_exact(gpu_lib):
+    st, w, lo, hi, dt, rc = _dense_state(seed=7)
+    rng = np.random.default_rng(1)
+    st.force[:] = rng.normal(0., 1e-2, size=st.force.shape)
+    st.torque[:] = rng.normal(0., 1e-6, size=st.torque.shape)
+    sysm = make_system(st, w, dt=dt, rc=rc, lo=lo, hi=hi)
+    ref = st.copy()
+    for stage in (1, 2, 3):
+        O.stage(ref, stage, dt)
+        getattr(sysm, f"stage{stage}")()
+    got = sysm.download()
+    assert np.array_equal(got.x, ref.x)
+    assert np.array_equal(got.u, ref.u)
+    assert np.array_equal(got.omega, ref.omega)
+
+
+@pytest.mark.parametrize("mode,K", [(0, 20), (1, 20), (0, 1)])
+def test_run_ke_momentum_history(gpu_lib, mode, K):
+    # the small plain-DEM configuration's physics at 12x12x14 = 2016 spheres
+    st, w, lo, hi = granular_box(12, 12, 14, spacing_r=2.05, jitter_r=0.02, vsigma=0.05,
+                                 seed=11)
+    dt, rc = 1e-6, 2.2e-3
+    nsteps, every = 600, 50
+    g = (0., 0., -9.81)
+    ref = st.copy()
+    diag_ref, _ = O.run(ref, w, oracle_params(dt, gravity=g, mode=mode), nsteps=nsteps, rc=rc,
+                        lo=lo, hi=hi, rebuild_every=K, max_n=64, diag_every=every)
+    sysm = make_system(st, w, dt=dt, rc=rc, lo=lo, hi=hi, gravity=g, rebuild_every=K, mode=mode)
+    sysm.set_diag_every(every)
+    sysm.step(nsteps)
+    diag = sysm.diag_history()
+    assert diag.shape[0] == diag_ref.shape[0] == nsteps // every
+    assert np.array_equal(diag[:, 0], diag_ref[:, 0])
+    for k in range(1, 6):  # KE_t, KE_r, px, py, pz
+        scale = np.abs(diag_ref[:, k]).max()
+        if scale == 0:
+            continue
+        assert np.abs(diag[:, k] - diag_ref[:, k]).max() <= TOL_RUN * scale, k
+    got = sysm.download()
+    assert normwise(got.x, ref.x) <= 1e-12
+    assert normwise(got.u, ref.u) <= TOL_RUN
+    assert ref.ss_cnt.sum() > 0
