@@ -40,7 +40,7 @@ def compare_history(gpu_st, ref_st, tol):
 
 def granular_box(nx, ny, nz, *, spacing_r=2.05, r=1e-3, rmin=None, rmax=None, jitter_r=0.02,
                  vsigma=0.05, omega_sigma=0.0, seed=1, E=1e7, nu=0.3, rho=2600., max_cnts=16,
-                 wall_gap_r=1.0):
+                 wall_offset=None):
     """Jittered lattice in a six-wall box (the plain-DEM small configuration's shape)."""
     rmin = r if rmin is None else rmin
     rmax = r if rmax is None else rmax
@@ -56,8 +56,9 @@ def granular_box(nx, ny, nz, *, spacing_r=2.05, r=1e-3, rmin=None, rmax=None, ji
     if omega_sigma > 0:
         rng = np.random.default_rng(seed + 100)
         st.omega[:] = rng.normal(0., omega_sigma, size=(n, 3))
-    lo = np.array([-wall_gap_r * sp] * 3)
-    hi = np.array([(nx - 1 + wall_gap_r) * sp, (ny - 1 + wall_gap_r) * sp, (nz - 1 + wall_gap_r) * sp])
+    off = sp if wall_offset is None else wall_offset  # wall distance from the outer lattice sites
+    lo = np.array([-off] * 3)
+    hi = np.array([(nx - 1) * sp + off, (ny - 1) * sp + off, (nz - 1) * sp + off])
     w = box_walls(lo, hi, E, nu, E / (2 * (1 + nu)))
     return st, w, lo, hi
 

@@ -26,7 +26,7 @@ def _dense_state(seed=3, mode=0, steps=40):
     """A packing that is mid-collision: overlapping contacts, non-zero
     tangential histories and wall contacts, prepared by the oracle."""
     st, w, lo, hi = granular_box(6, 6, 6, spacing_r=1.97, jitter_r=0.03, vsigma=0.2,
-                                 omega_sigma=50., seed=seed, wall_gap_r=0.98)
+                                 omega_sigma=50., seed=seed, wall_offset=0.97e-3)
     dt, rc = 1e-6, 2.4e-3
     prm = oracle_params(dt, gravity=(0., 0., -9.81), mode=mode)
     O.run(st, w, prm, nsteps=steps, rc=rc, lo=lo - 1e-3, hi=hi + 1e-3, rebuild_every=1, max_n=64)
@@ -163,6 +163,16 @@ def test_run_ke_momentum_history(gpu_lib, mode, K):
             continue
         assert np.abs(diag[:, k] - diag_ref[:, k]).max() <= TOL_RUN * scale, k
     got = sysm.download()
-    assert normwise(got.x, ref.x) <= 1e-12
-    assert normwise(got.u, ref.u) <= TOL_RUN
     assert ref.ss_cnt.sum() > 0
+    # Per-particle states: rounding-order differences grow through the
+    # contacts over the run.  The bar is the oracle's own sensitivity: the
+    # oracle restarted from a state one ulp away in u (10x margin), or the
+    # run-level 1e-8, whichever is larger.
+    pert = st.copy()
+    pert.u[:] = np.nextafter(pert.u, np.inf)
+    O.run(pert, w, oracle_params(dt, gravity=g, mode=mode), nsteps=nsteps, rc=rc, lo=lo, hi=hi,
+          rebuild_every=K, max_n=64)
+    self_u = normwise(pert.u, ref.u)
+    self_x = normwise(pert.x - st.x, ref.x - st.x)
+    assert normwise(got.u, ref.u) <= max(TOL_RUN, 10. * self_u)
+    assert normwise(got.x - st.x, ref.x - st.x) <= max(TOL_RUN, 10. * self_x)
