@@ -12,7 +12,8 @@ from pathlib import Path
 
 import numpy as np
 
-LIB_PATH = Path(__file__).resolve().parent / "lib" / "libcabanadem_hip.so"
+LIB_PATH = Path(os.environ.get("CABANADEM_LIB") or
+                Path(__file__).resolve().parent / "lib" / "libcabanadem_hip.so")
 
 DEM_MAX_WALLS = 8
 DEM_OK = 0
@@ -77,6 +78,7 @@ SYMBOLS = {
     "dem_destroy": (None, [C.c_void_p]),
     "dem_last_error": (C.c_char_p, [C.c_void_p]),
     "dem_abi_version": (C.c_int, []),
+    "dem_set_params": (C.c_int, [C.c_void_p, C.POINTER(DemConfig)]),
     "dem_device_count": (C.c_int, []),
     "dem_upload_particles": (C.c_int, [C.c_void_p, C.POINTER(DemParticlesView)]),
     "dem_download_particles": (C.c_int, [C.c_void_p, C.POINTER(DemParticlesView)]),

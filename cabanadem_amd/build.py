@@ -48,14 +48,43 @@ def build_library(extra_flags=None) -> Path:
     return LIB
 
 
+def build_copy_probe() -> Path:
+    """Streaming-copy probe (known bytes) used to calibrate the HBM counters."""
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    exe = LIB.parent / "hbm_copy"
+    flags = [f for f in HIPCC_FLAGS if f not in ("-shared", "-fPIC")]
+    _run([hipcc] + flags + [CSRC / "hbm_copy.hip", "-o", exe], cwd=CSRC)
+    return exe
+
+
 def build_oracle() -> Path:
     _run(["make", "-C", ORACLE_DIR, "-B"])
     return ORACLE_DIR / "_build" / "libdem_oracle.so"
 
 
+EXAMPLES = {"dem_validation": "examples/dem_validation.cpp"}
+
+
+def build_examples() -> list:
+    """Host programs on the reference-shaped C++ API, linked to the library
+    (rpath relative to the binary, so the tree can move)."""
+    out_dir = ROOT / "examples" / "_build"
+    out_dir.mkdir(parents=True, exist_ok=True)
+    built = []
+    for name, src in EXAMPLES.items():
+        exe = out_dir / name
+        _run(["g++", "-std=c++17", "-O2", "-Wall", "-I", ROOT / "include", ROOT / src,
+              "-L", LIB.parent, "-lcabanadem_hip", "-Wl,-rpath,$ORIGIN/../../cabanadem_amd/lib",
+              "-o", exe])
+        built.append(exe)
+    return built
+
+
 def build_all():
     build_oracle()
     build_library()
+    build_examples()
+    build_copy_probe()
 
 
 if __name__ == "__main__":
@@ -64,5 +93,7 @@ if __name__ == "__main__":
         build_library()
     elif what == "oracle":
         build_oracle()
+    elif what == "examples":
+        build_examples()
     else:
         build_all()

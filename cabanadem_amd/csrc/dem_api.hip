@@ -60,13 +60,11 @@ struct dem_ctx {
   double4* xr[2] = {nullptr, nullptr};
   double* uw[2] = {nullptr, nullptr};
   double4* mat[2] = {nullptr, nullptr};
-  double2* mi[2] = {nullptr, nullptr};
-  double* Imom[2] = {nullptr, nullptr};
+  double2* mI[2] = {nullptr, nullptr};
   int* id[2] = {nullptr, nullptr};
   double* ft[2] = {nullptr, nullptr};
-  int* wmask[2] = {nullptr, nullptr};
   double* sw[2] = {nullptr, nullptr};
-  int* ccnt[2] = {nullptr, nullptr};
+  int* cmeta[2] = {nullptr, nullptr};
   int* cj[2] = {nullptr, nullptr};
   double* ch[2] = {nullptr, nullptr};
   int cP = 0, cS = 0, cC = 0;
@@ -110,9 +108,9 @@ struct dem_ctx {
 
   ~dem_ctx() {
     for (int b = 0; b < 2; ++b) {
-      hipFree(xr[b]); hipFree(uw[b]); hipFree(mat[b]); hipFree(mi[b]); hipFree(Imom[b]);
-      hipFree(id[b]); hipFree(ft[b]); hipFree(wmask[b]); hipFree(sw[b]);
-      hipFree(ccnt[b]); hipFree(cj[b]); hipFree(ch[b]);
+      hipFree(xr[b]); hipFree(uw[b]); hipFree(mat[b]); hipFree(mI[b]);
+      hipFree(id[b]); hipFree(ft[b]); hipFree(sw[b]);
+      hipFree(cmeta[b]); hipFree(cj[b]); hipFree(ch[b]);
     }
     hipFree(nn); hipFree(nbr); hipFree(xb); hipFree(key); hipFree(rank); hipFree(key_sorted);
     hipFree(cell_count); hipFree(cell_start); hipFree(perm); hipFree(inv); hipFree(scan_tmp);
@@ -145,18 +143,20 @@ struct dem_ctx {
     for (int w = 0; w < DEM_MAX_WALLS; ++w) a.walls[w] = walls[w];
     a.xr_in = xr[cP]; a.xr_out = xr[cP ^ 1];
     a.uw_in = uw[cP]; a.uw_out = uw[cP ^ 1];
-    a.mat = mat[cS]; a.mi = mi[cS]; a.Imom = Imom[cS]; a.ft = ft[cS];
-    a.wmask = wmask[cS]; a.sw = sw[cS];
+    a.mat = mat[cS]; a.mI = mI[cS]; a.ft = ft[cS];
+    a.sw = sw[cS];
     a.nn = nn; a.nbr = nbr;
-    a.ccnt_in = ccnt[cC]; a.cj_in = cj[cC]; a.ch_in = ch[cC];
-    a.ccnt_out = ccnt[cC ^ 1]; a.cj_out = cj[cC ^ 1]; a.ch_out = ch[cC ^ 1];
+    a.cmeta_in = cmeta[cC]; a.cj_in = cj[cC]; a.ch_in = ch[cC];
+    a.cmeta_out = cmeta[cC ^ 1]; a.cj_out = cj[cC ^ 1]; a.ch_out = ch[cC ^ 1];
     a.flags = flags;
     a.diag_part = diag_part;
     return a;
   }
 
-  BuildArgs build_args(bool check_skin) const {
+  BuildArgs build_args(bool check_skin, bool permute_ft = true) const {
     BuildArgs b{};
+    b.keep_xb = cfg.skin > 0. ? 1 : 0;
+    b.permute_ft = permute_ft ? 1 : 0;
     b.n = n; b.ld = ld; b.max_n = max_n; b.max_c = max_c;
     b.grid = grid; b.ncell = ncell; b.rc2 = rc2;
     double sh = 0.5 * cfg.skin;
@@ -165,13 +165,11 @@ struct dem_ctx {
     b.xr_in = xr[cP]; b.xr_out = xr[cP ^ 1];
     b.uw_in = uw[cP]; b.uw_out = uw[cP ^ 1];
     b.mat_in = mat[cS]; b.mat_out = mat[cS ^ 1];
-    b.mi_in = mi[cS]; b.mi_out = mi[cS ^ 1];
-    b.I_in = Imom[cS]; b.I_out = Imom[cS ^ 1];
+    b.mI_in = mI[cS]; b.mI_out = mI[cS ^ 1];
     b.id_in = id[cS]; b.id_out = id[cS ^ 1];
     b.ft_in = ft[cS]; b.ft_out = ft[cS ^ 1];
-    b.wmask_in = wmask[cS]; b.wmask_out = wmask[cS ^ 1];
     b.sw_in = sw[cS]; b.sw_out = sw[cS ^ 1];
-    b.ccnt_in = ccnt[cC]; b.ccnt_out = ccnt[cC ^ 1];
+    b.cmeta_in = cmeta[cC]; b.cmeta_out = cmeta[cC ^ 1];
     b.cj_in = cj[cC]; b.cj_out = cj[cC ^ 1];
     b.ch_in = ch[cC]; b.ch_out = ch[cC ^ 1];
     b.xb = xb; b.key = key; b.rank = rank; b.key_sorted = key_sorted;
@@ -259,9 +257,11 @@ struct dem_ctx {
 
   // Rebuild: re-sort into cell order, gather, search.  Grows the rows on
   // overflow when max_neighbors was left to auto sizing.
-  int rebuild(bool sync_check, bool check_skin) {
+  // permute_ft: F and T are live across this rebuild (false inside dem_step,
+  // where the fused kernel has already applied them).
+  int rebuild(bool sync_check, bool check_skin, bool permute_ft = true) {
     mark_begin(1);
-    BuildArgs b = build_args(check_skin && built_once);
+    BuildArgs b = build_args(check_skin && built_once, permute_ft);
     hipError_t e = launch_rebuild(b, stream);
     if (e != hipSuccess) return hip_fail(e, "rebuild launch");
     cP ^= 1; cS ^= 1; cC ^= 1;
@@ -328,7 +328,8 @@ Grid make_grid(const dem_config& c, long* ncell) {
     if (f > 1e6) f = 1e6;
     nc[d] = (int)f;
   }
-  while ((long)nc[0] * nc[1] * nc[2] > cap) {
+  auto padded = [&]() { return (long)nc[0] * ((nc[1] + kTileRows - 1) / kTileRows) * kTileRows * nc[2]; };
+  while (padded() > cap) {
     int d = (nc[0] >= nc[1] && nc[0] >= nc[2]) ? 0 : (nc[1] >= nc[2] ? 1 : 2);
     nc[d] = std::max(1, nc[d] / 2);
   }
@@ -339,9 +340,9 @@ Grid make_grid(const dem_config& c, long* ncell) {
     g.h[d] = ext / nc[d];
     double cs = c.cutoff * (1.0 + 1e-9);
     if (g.h[d] < cs) g.h[d] = cs;
-    g.inv_h[d] = 1.0 / g.h[d];
   }
-  *ncell = (long)nc[0] * nc[1] * nc[2];
+  g.nty = (nc[1] + kTileRows - 1) / kTileRows;
+  *ncell = cell_key_count(g);
   return g;
 }
 
@@ -398,13 +399,11 @@ int dem_create(const dem_config* cfg, dem_ctx** out) {
     c->xr[b] = dalloc<double4>(n, &e);
     c->uw[b] = dalloc<double>(6 * n, &e);
     c->mat[b] = dalloc<double4>(n, &e);
-    c->mi[b] = dalloc<double2>(n, &e);
-    c->Imom[b] = dalloc<double>(n, &e);
+    c->mI[b] = dalloc<double2>(n, &e);
     c->id[b] = dalloc<int>(n, &e);
     c->ft[b] = dalloc<double>(6 * n, &e);
-    c->wmask[b] = dalloc<int>(n, &e);
     c->sw[b] = dalloc<double>((size_t)DEM_MAX_WALLS * 3 * ld, &e);
-    c->ccnt[b] = dalloc<int>(n, &e);
+    c->cmeta[b] = dalloc<int>(n, &e);
     c->cj[b] = dalloc<int>((size_t)c->max_c * ld, &e);
     c->ch[b] = dalloc<double>((size_t)c->max_c * 3 * ld, &e);
   }
@@ -427,8 +426,7 @@ int dem_create(const dem_config* cfg, dem_ctx** out) {
   std::vector<int> ids(n);
   for (size_t i = 0; i < n; ++i) ids[i] = (int)i;
   e = hipMemcpy(c->id[0], ids.data(), n * sizeof(int), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemset(c->ccnt[0], 0, n * sizeof(int));
-  if (e == hipSuccess) e = hipMemset(c->wmask[0], 0, n * sizeof(int));
+  if (e == hipSuccess) e = hipMemset(c->cmeta[0], 0, n * sizeof(int));
   if (e == hipSuccess) e = hipMemset(c->ft[0], 0, 6 * n * sizeof(double));
   if (e == hipSuccess) e = hipMemset(c->uw[0], 0, 6 * n * sizeof(double));
   if (e == hipSuccess) e = hipMemset(c->xr[0], 0, n * sizeof(double4));
@@ -447,9 +445,28 @@ void dem_destroy(dem_ctx* ctx) {
   delete ctx;
 }
 
+int dem_set_params(dem_ctx* c, const dem_config* cfg) {
+  if (int r = check_ctx(c)) return r;
+  std::string msg;
+  if (int r = validate(cfg, &msg)) return c->fail(r, msg);
+  const dem_config& o = c->cfg;
+  bool same = cfg->n == o.n && cfg->max_contacts == o.max_contacts && cfg->cutoff == o.cutoff;
+  for (int d = 0; d < 3; ++d) same = same && cfg->lo[d] == o.lo[d] && cfg->hi[d] == o.hi[d];
+  if (!same) return c->fail(DEM_ERR_ARG, "dem_set_params cannot change n, max_contacts, cutoff or the box");
+  hipStreamSynchronize(c->stream);
+  dem_config keep = c->cfg;
+  c->cfg = *cfg;
+  c->cfg.max_neighbors = keep.max_neighbors;
+  c->cfg.n_walls = keep.n_walls;
+  if (c->cfg.rebuild_every <= 0) c->cfg.rebuild_every = 1;
+  return DEM_OK;
+}
+
 int dem_upload_walls(dem_ctx* c, const dem_walls_view* w) {
   if (int r = check_ctx(c)) return r;
-  if (!w || w->n != c->nw) return c->fail(DEM_ERR_ARG, "wall count differs from dem_config.n_walls");
+  if (!w || w->n < 0 || w->n > DEM_MAX_WALLS) return c->fail(DEM_ERR_ARG, "wall count must be in [0, 8]");
+  c->nw = w->n;
+  c->cfg.n_walls = w->n;
   for (int j = 0; j < c->nw; ++j) {
     WallConst& W = c->walls[j];
     for (int d = 0; d < 3; ++d) {
@@ -471,10 +488,11 @@ int dem_upload_particles(dem_ctx* c, const dem_particles_view* v) {
   hipError_t e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return c->hip_fail(e, "sync");
   // current device state, sorted order
-  std::vector<int> id(n);
+  std::vector<int> id(n), meta(n);
   std::vector<double4> xr(n);
   std::vector<double> uw(6 * (size_t)n), ft(6 * (size_t)n);
   e = hipMemcpy(id.data(), c->id[c->cS], n * sizeof(int), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(meta.data(), c->cmeta[c->cC], n * sizeof(int), hipMemcpyDeviceToHost);
   if (e == hipSuccess) e = hipMemcpy(xr.data(), c->xr[c->cP], n * sizeof(double4), hipMemcpyDeviceToHost);
   if (e == hipSuccess) e = hipMemcpy(uw.data(), c->uw[c->cP], 6 * n * sizeof(double), hipMemcpyDeviceToHost);
   if (e == hipSuccess) e = hipMemcpy(ft.data(), c->ft[c->cS], 6 * n * sizeof(double), hipMemcpyDeviceToHost);
@@ -488,8 +506,7 @@ int dem_upload_particles(dem_ctx* c, const dem_particles_view* v) {
   setv(c->h_m, v->m); setv(c->h_rad, v->rad); setv(c->h_E, v->E);
   setv(c->h_nu, v->nu); setv(c->h_G, v->G); setv(c->h_I, v->I);
   std::vector<double4> mat(n);
-  std::vector<double2> mi(n);
-  std::vector<double> Im(n);
+  std::vector<double2> mI(n);
   for (int p = 0; p < n; ++p) {
     const int I = id[p];
     const double nu = c->h_nu[I];
@@ -497,10 +514,9 @@ int dem_upload_particles(dem_ctx* c, const dem_particles_view* v) {
     mat[p].x = (1. - (nu * nu)) / c->h_E[I];
     mat[p].y = (2. - nu) / c->h_G[I];
     mat[p].z = 1. / c->h_rad[I];
-    mat[p].w = 1. / c->h_m[I];              // src/CabanaDEM_Integrator.hpp:42
-    mi[p].x = c->h_m[I];
-    mi[p].y = 1. / c->h_I[I];               // src/CabanaDEM_Integrator.hpp:48
-    Im[p] = c->h_I[I];
+    mat[p].w = 1. / c->h_m[I];
+    mI[p].x = c->h_m[I];
+    mI[p].y = c->h_I[I];
     xr[p].w = c->h_rad[I];
     for (int d = 0; d < 3; ++d) {
       if (v->x) (&xr[p].x)[d] = v->x[3 * I + d];
@@ -514,26 +530,25 @@ int dem_upload_particles(dem_ctx* c, const dem_particles_view* v) {
   if (e == hipSuccess) e = hipMemcpy(c->uw[c->cP], uw.data(), 6 * n * sizeof(double), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(c->ft[c->cS], ft.data(), 6 * n * sizeof(double), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(c->mat[c->cS], mat.data(), n * sizeof(double4), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(c->mi[c->cS], mi.data(), n * sizeof(double2), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(c->Imom[c->cS], Im.data(), n * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(c->mI[c->cS], mI.data(), n * sizeof(double2), hipMemcpyHostToDevice);
   if (e != hipSuccess) return c->hip_fail(e, "upload");
   // wall history: a set bit marks a (possibly) non-zero history
   if (v->sw_x && v->sw_y && v->sw_z) {
     const int MW = v->max_walls;
-    std::vector<int> wm(n, 0);
     std::vector<double> sw((size_t)DEM_MAX_WALLS * 3 * ld, 0.);
     for (int p = 0; p < n; ++p) {
       const int I = id[p];
+      int wm = 0;
       for (int w = 0; w < std::min(MW, DEM_MAX_WALLS); ++w) {
         double h[3] = {v->sw_x[(size_t)I * MW + w], v->sw_y[(size_t)I * MW + w], v->sw_z[(size_t)I * MW + w]};
         if (h[0] != 0. || h[1] != 0. || h[2] != 0.) {
-          wm[p] |= 1 << w;
+          wm |= 1 << w;
           for (int q = 0; q < 3; ++q) sw[(size_t)(w * 3 + q) * ld + p] = h[q];
         }
       }
+      meta[p] = make_meta(meta_count(meta[p]), wm);
     }
-    e = hipMemcpy(c->wmask[c->cS], wm.data(), n * sizeof(int), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(c->sw[c->cS], sw.data(), sw.size() * sizeof(double), hipMemcpyHostToDevice);
+    e = hipMemcpy(c->sw[c->cS], sw.data(), sw.size() * sizeof(double), hipMemcpyHostToDevice);
     if (e != hipSuccess) return c->hip_fail(e, "upload wall history");
   }
   // tracked pp contacts: reference rows (ids, history) -> compact rows in
@@ -541,7 +556,7 @@ int dem_upload_particles(dem_ctx* c, const dem_particles_view* v) {
   // as the reference's search does (src/CabanaDEM_Force.hpp:168-174).
   if (v->ss_idx && v->ss_cnt) {
     const int M = v->max_cnts;
-    std::vector<int> cnt(n, 0), cjh((size_t)c->max_c * ld, 0);
+    std::vector<int> cjh((size_t)c->max_c * ld, 0);
     std::vector<double> chh((size_t)c->max_c * 3 * ld, 0.);
     for (int p = 0; p < n; ++p) {
       const int I = id[p];
@@ -557,7 +572,7 @@ int dem_upload_particles(dem_ctx* c, const dem_particles_view* v) {
       std::sort(ent.begin(), ent.end());
       if ((int)ent.size() > c->max_c)
         return c->fail(DEM_ERR_CONTACT_OVERFLOW, "uploaded tracked contacts exceed max_contacts");
-      cnt[p] = (int)ent.size();
+      meta[p] = make_meta((int)ent.size(), meta_walls(meta[p]));
       for (size_t m = 0; m < ent.size(); ++m) {
         cjh[m * ld + p] = ent[m].first;
         const size_t src = (size_t)I * M + ent[m].second;
@@ -566,11 +581,12 @@ int dem_upload_particles(dem_ctx* c, const dem_particles_view* v) {
         for (int q = 0; q < 3; ++q) chh[(m * 3 + q) * ld + p] = h[q];
       }
     }
-    e = hipMemcpy(c->ccnt[c->cC], cnt.data(), n * sizeof(int), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(c->cj[c->cC], cjh.data(), cjh.size() * sizeof(int), hipMemcpyHostToDevice);
+    e = hipMemcpy(c->cj[c->cC], cjh.data(), cjh.size() * sizeof(int), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(c->ch[c->cC], chh.data(), chh.size() * sizeof(double), hipMemcpyHostToDevice);
     if (e != hipSuccess) return c->hip_fail(e, "upload contacts");
   }
+  e = hipMemcpy(c->cmeta[c->cC], meta.data(), n * sizeof(int), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return c->hip_fail(e, "upload contact counts");
   if (v->x) c->list_valid = false;
   return DEM_OK;
 }
@@ -582,10 +598,11 @@ int dem_download_particles(dem_ctx* c, dem_particles_view* v) {
   const size_t ld = c->ld;
   hipError_t e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return c->hip_fail(e, "sync");
-  std::vector<int> id(n);
+  std::vector<int> id(n), meta(n);
   std::vector<double4> xr(n);
   std::vector<double> uw(6 * (size_t)n), ft(6 * (size_t)n);
   e = hipMemcpy(id.data(), c->id[c->cS], n * sizeof(int), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(meta.data(), c->cmeta[c->cC], n * sizeof(int), hipMemcpyDeviceToHost);
   if (e == hipSuccess) e = hipMemcpy(xr.data(), c->xr[c->cP], n * sizeof(double4), hipMemcpyDeviceToHost);
   if (e == hipSuccess) e = hipMemcpy(uw.data(), c->uw[c->cP], 6 * n * sizeof(double), hipMemcpyDeviceToHost);
   if (e == hipSuccess) e = hipMemcpy(ft.data(), c->ft[c->cS], 6 * n * sizeof(double), hipMemcpyDeviceToHost);
@@ -607,16 +624,15 @@ int dem_download_particles(dem_ctx* c, dem_particles_view* v) {
   getv(v->nu, c->h_nu); getv(v->G, c->h_G); getv(v->I, c->h_I);
   if (v->sw_x || v->sw_y || v->sw_z) {
     const int MW = v->max_walls;
-    std::vector<int> wm(n);
     std::vector<double> sw((size_t)DEM_MAX_WALLS * 3 * ld);
-    e = hipMemcpy(wm.data(), c->wmask[c->cS], n * sizeof(int), hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(sw.data(), c->sw[c->cS], sw.size() * sizeof(double), hipMemcpyDeviceToHost);
+    e = hipMemcpy(sw.data(), c->sw[c->cS], sw.size() * sizeof(double), hipMemcpyDeviceToHost);
     if (e != hipSuccess) return c->hip_fail(e, "download wall history");
     for (int p = 0; p < n; ++p) {
       const int I = id[p];
+      const int wm = meta_walls(meta[p]);
       for (int w = 0; w < MW; ++w) {
         double h[3] = {0., 0., 0.};
-        if (w < DEM_MAX_WALLS && (wm[p] & (1 << w)))
+        if (w < DEM_MAX_WALLS && (wm & (1 << w)))
           for (int q = 0; q < 3; ++q) h[q] = sw[(size_t)(w * 3 + q) * ld + p];
         if (v->sw_x) v->sw_x[(size_t)I * MW + w] = h[0];
         if (v->sw_y) v->sw_y[(size_t)I * MW + w] = h[1];
@@ -626,16 +642,16 @@ int dem_download_particles(dem_ctx* c, dem_particles_view* v) {
   }
   if (v->ss_idx || v->ss_cnt || v->ss_x) {
     const int M = v->max_cnts;
-    std::vector<int> cnt(n), cjh((size_t)c->max_c * ld);
+    std::vector<int> cjh((size_t)c->max_c * ld);
     std::vector<double> chh((size_t)c->max_c * 3 * ld);
-    e = hipMemcpy(cnt.data(), c->ccnt[c->cC], n * sizeof(int), hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(cjh.data(), c->cj[c->cC], cjh.size() * sizeof(int), hipMemcpyDeviceToHost);
+    e = hipMemcpy(cjh.data(), c->cj[c->cC], cjh.size() * sizeof(int), hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(chh.data(), c->ch[c->cC], chh.size() * sizeof(double), hipMemcpyDeviceToHost);
     if (e != hipSuccess) return c->hip_fail(e, "download contacts");
     for (int p = 0; p < n; ++p) {
       const int I = id[p];
+      const int cnt = meta_count(meta[p]);
       int k = 0;
-      for (; k < cnt[p] && k < M; ++k) {
+      for (; k < cnt && k < M; ++k) {
         const size_t dst = (size_t)I * M + k;
         if (v->ss_idx) v->ss_idx[dst] = id[cjh[(size_t)k * ld + p]];
         if (v->ss_x) v->ss_x[dst] = chh[(size_t)(k * 3 + 0) * ld + p];
@@ -649,7 +665,7 @@ int dem_download_particles(dem_ctx* c, dem_particles_view* v) {
         if (v->ss_y) v->ss_y[dst] = 0.;
         if (v->ss_z) v->ss_z[dst] = 0.;
       }
-      if (v->ss_cnt) v->ss_cnt[I] = std::min(cnt[p], M);
+      if (v->ss_cnt) v->ss_cnt[I] = std::min(cnt, M);
     }
   }
   return DEM_OK;
@@ -757,7 +773,7 @@ int dem_step(dem_ctx* c, int nsteps) {
   for (int s = 0; s < nsteps; ++s) {
     const int64_t gstep = c->step;
     if (!c->list_valid || gstep % K == 0) {
-      int r = c->rebuild(!c->built_once, c->list_valid);
+      int r = c->rebuild(!c->built_once, c->list_valid, false);
       if (r) return r;
     }
     const bool sample = c->diag_every > 0 && gstep % c->diag_every == 0;

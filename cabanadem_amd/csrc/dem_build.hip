@@ -8,19 +8,21 @@
 // sets are bit-exact.
 //
 // Sequence (one rebuild, all stream ordered):
-//   1. k_cell_keys   cell key of every particle; per-cell arrival rank by
-//                    atomic counting (also the skin-displacement check)
+//   1. k_cell_keys   cell key of every particle (tiled cell order, see
+//                    dem_internal.h); per-cell arrival rank by atomic
+//                    counting; optional skin-displacement check
 //   2. scan          exclusive prefix over the cell counts -> cell starts
 //   3. k_scatter     counting-sort scatter: slot = start[key] + rank
 //   4. k_cell_order  within each cell, order by previous slot (the arrival
 //                    order of step 1 is not deterministic; this makes the
 //                    sort a stable, deterministic single-digit radix sort
 //                    on the cell key) and write the inverse permutation
-//   5. k_permute     gather the whole particle state into the new order and
+//   5. k_permute     gather the particle state into the new order and
 //                    re-map the tracked-contact lists (j -> inv[j], re-sorted)
 //   6. k_neighbors   27-cell search: for each (dz, dy) the three x-adjacent
-//                    cells are one contiguous slot range, so a row is nine
-//                    range scans and comes out in ascending j.
+//                    cells are one contiguous slot range; the nine ranges are
+//                    visited in ascending slot order, so a row comes out in
+//                    ascending j (the order the force kernel merges in).
 #include "dem_internal.h"
 
 #include <climits>
@@ -49,7 +51,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_keys(BuildArgs a) {
   const int cx = cell_coord(p.x, a.grid, 0);
   const int cy = cell_coord(p.y, a.grid, 1);
   const int cz = cell_coord(p.z, a.grid, 2);
-  const int key = (cz * a.grid.nc[1] + cy) * a.grid.nc[0] + cx;
+  const int key = (int)cell_key(a.grid, cx, cy, cz);
   a.key[i] = key;
   a.rank[i] = atomicAdd(&a.cell_count[key], 1);
   if (a.check_skin) {
@@ -160,10 +162,17 @@ __global__ __launch_bounds__(kBlock) void k_cell_order(BuildArgs a) {
   for (int q = s; q < e; ++q) a.inv[a.perm[q]] = q;
 }
 
-// Gather every per-particle array into the new order.  The tracked-contact
+__device__ inline void copy6(const double* src, double* dst) {
+  const double2* s = reinterpret_cast<const double2*>(src);
+  double2* d = reinterpret_cast<double2*>(dst);
+  d[0] = s[0];
+  d[1] = s[1];
+  d[2] = s[2];
+}
+
+// Gather the per-particle arrays into the new order.  The tracked-contact
 // list of a particle keeps its entries and histories; only the neighbour
-// indices change (j -> inv[j]) and the list is re-sorted by the new index,
-// which is the order the force kernel merges in.
+// indices change (j -> inv[j]) and the list is re-sorted by the new index.
 __global__ __launch_bounds__(kBlock) void k_permute(BuildArgs a) {
   extern __shared__ int s_j[]; // [max_c][kBlock]
   const int tid = threadIdx.x;
@@ -172,31 +181,27 @@ __global__ __launch_bounds__(kBlock) void k_permute(BuildArgs a) {
   const int o = a.perm[p];
   const double4 xr = a.xr_in[o];
   a.xr_out[p] = xr;
-  {
-    const double2* src = reinterpret_cast<const double2*>(a.uw_in + 6l * o);
-    double2* dst = reinterpret_cast<double2*>(a.uw_out + 6l * p);
-    dst[0] = src[0]; dst[1] = src[1]; dst[2] = src[2];
-    const double2* fs = reinterpret_cast<const double2*>(a.ft_in + 6l * o);
-    double2* fd = reinterpret_cast<double2*>(a.ft_out + 6l * p);
-    fd[0] = fs[0]; fd[1] = fs[1]; fd[2] = fs[2];
-  }
+  copy6(a.uw_in + 6l * o, a.uw_out + 6l * p);
+  if (a.permute_ft) copy6(a.ft_in + 6l * o, a.ft_out + 6l * p);
   a.mat_out[p] = a.mat_in[o];
-  a.mi_out[p] = a.mi_in[o];
-  a.I_out[p] = a.I_in[o];
+  a.mI_out[p] = a.mI_in[o];
   a.id_out[p] = a.id_in[o];
   a.key_sorted[p] = a.key[o];
-  a.xb[3 * p + 0] = xr.x;
-  a.xb[3 * p + 1] = xr.y;
-  a.xb[3 * p + 2] = xr.z;
-  const int wm = a.wmask_in[o];
-  a.wmask_out[p] = wm;
+  if (a.keep_xb) {
+    a.xb[3 * p + 0] = xr.x;
+    a.xb[3 * p + 1] = xr.y;
+    a.xb[3 * p + 2] = xr.z;
+  }
+  const int meta = a.cmeta_in[o];
+  a.cmeta_out[p] = meta;
+  const int wm = meta_walls(meta);
   if (wm) {
     for (int w = 0; w < DEM_MAX_WALLS; ++w)
       if (wm & (1 << w))
         for (int c = 0; c < 3; ++c)
           a.sw_out[(long)(w * 3 + c) * a.ld + p] = a.sw_in[(long)(w * 3 + c) * a.ld + o];
   }
-  int oc = a.ccnt_in[o];
+  int oc = meta_count(meta);
   if (oc > a.max_c) oc = a.max_c; // counts are clamped on write; never index past the LDS rows
   for (int m = 0; m < oc; ++m) s_j[m * kBlock + tid] = a.inv[a.cj_in[(long)m * a.ld + o]];
   for (int m = 0; m < oc; ++m) {
@@ -207,7 +212,6 @@ __global__ __launch_bounds__(kBlock) void k_permute(BuildArgs a) {
     for (int c = 0; c < 3; ++c)
       a.ch_out[(long)(r * 3 + c) * a.ld + p] = a.ch_in[(long)(m * 3 + c) * a.ld + o];
   }
-  a.ccnt_out[p] = oc;
 }
 
 __device__ inline int wave_max_int(int v) {
@@ -222,42 +226,62 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(BuildArgs a) {
   const int p = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   int cnt = 0;
   if (p < a.n && !a.flags->cell_overflow) {
-    const int nx = a.grid.nc[0], ny = a.grid.nc[1], nz = a.grid.nc[2];
-    const int key = a.key_sorted[p];
+    const Grid& g = a.grid;
+    const int nx = g.nc[0], ny = g.nc[1], nz = g.nc[2];
+    int key = a.key_sorted[p];
     const int cx = key % nx;
-    const int cy = (key / nx) % ny;
-    const int cz = key / (nx * ny);
+    key /= nx;
+    const int ly = key % kTileRows;
+    key /= kTileRows;
+    const int cz = key % nz;
+    const int cy = (key / nz) * kTileRows + ly;
     const double4 xi = a.xr_out[p];
     const int xlo = cx > 0 ? cx - 1 : 0;
     const int xhi = cx < nx - 1 ? cx + 1 : nx - 1;
-    for (int dz = -1; dz <= 1; ++dz) {
-      const int z = cz + dz;
-      if (z < 0 || z >= nz) continue;
-      for (int dy = -1; dy <= 1; ++dy) {
-        const int y = cy + dy;
-        if (y < 0 || y >= ny) continue;
-        const long row = ((long)z * ny + y) * nx;
-        const int s = a.cell_start[row + xlo];
-        const int e = a.cell_start[row + xhi + 1];
-        for (int j = s; j < e; ++j) {
-          if (j == p) continue;
-          const double4 xj = a.xr_out[j];
-          const double d0 = xi.x - xj.x;
-          const double d1 = xi.y - xj.y;
-          const double d2 = xi.z - xj.z;
-          const double dsqr = d0 * d0 + d1 * d1 + d2 * d2;
-          if (dsqr <= a.rc2) {
-            if (cnt < a.max_n) a.nbr[(long)cnt * a.ld + p] = j;
-            ++cnt;
-          }
+    // the nine slot ranges, ascending by start (they never overlap)
+    int rs[9], re[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+      const int z = cz + q / 3 - 1, y = cy + q % 3 - 1;
+      if (z < 0 || z >= nz || y < 0 || y >= ny) {
+        rs[q] = INT_MAX;
+        re[q] = INT_MAX;
+      } else {
+        const long k0 = cell_key(g, xlo, y, z);
+        rs[q] = a.cell_start[k0];
+        re[q] = a.cell_start[k0 + (xhi - xlo) + 1];
+      }
+    }
+#pragma unroll
+    for (int q = 1; q < 9; ++q)
+#pragma unroll
+      for (int r = q; r > 0; --r)
+        if (rs[r] < rs[r - 1]) {
+          int t = rs[r]; rs[r] = rs[r - 1]; rs[r - 1] = t;
+          t = re[r]; re[r] = re[r - 1]; re[r - 1] = t;
+        }
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+      if (rs[q] == INT_MAX) break;
+      for (int j = rs[q]; j < re[q]; ++j) {
+        if (j == p) continue;
+        const double4 xj = a.xr_out[j];
+        const double d0 = xi.x - xj.x;
+        const double d1 = xi.y - xj.y;
+        const double d2 = xi.z - xj.z;
+        const double dsqr = d0 * d0 + d1 * d1 + d2 * d2;
+        if (dsqr <= a.rc2) {
+          if (cnt < a.max_n) a.nbr[(long)cnt * a.ld + p] = j;
+          ++cnt;
         }
       }
     }
     a.nn[p] = cnt < a.max_n ? cnt : a.max_n;
   }
+  // one global atomic per wave, and only when it can raise the maximum
   const int wmax = wave_max_int(cnt);
   if ((threadIdx.x & 63) == 0 && wmax > 0) {
-    atomicMax(&a.flags->max_nn, wmax);
+    if (wmax > *(volatile int*)&a.flags->max_nn) atomicMax(&a.flags->max_nn, wmax);
     if (wmax > a.max_n) atomicMax(&a.flags->nbr_overflow, wmax);
   }
 }

@@ -26,6 +26,14 @@
 namespace demhip {
 namespace {
 
+#ifndef DEM_PHASE1_BATCH
+#define DEM_PHASE1_BATCH 4
+#endif
+#ifndef DEM_FORCE_MIN_WAVES
+#define DEM_FORCE_MIN_WAVES 1
+#endif
+constexpr int kPhase1Batch = DEM_PHASE1_BATCH; // neighbour gathers issued together
+
 __device__ inline void load6(const double* p, double* v) {
   const double2* q = reinterpret_cast<const double2*>(p);
   double2 a = q[0], b = q[1], c = q[2];
@@ -217,7 +225,7 @@ __device__ inline void block_reduce8(const double* v, double* part, int blk) {
 // KICK: 0 = forces only (write F,T); 1 = + stage3 (write state, F,T);
 //       2 = + stage3 + stage1 + stage2 of the next step (write state).
 template <bool REF, bool PP, bool PW, int KICK, bool ACCUM, bool DIAG>
-__global__ __launch_bounds__(kBlock) void k_force(StepArgs a) {
+__global__ __launch_bounds__(kBlock, DEM_FORCE_MIN_WAVES) void k_force(StepArgs a) {
   extern __shared__ int s_cj[]; // [max_c][kBlock] contacts found in phase 1
   const int tid = threadIdx.x;
   const int blk = xcd_block(blockIdx.x, gridDim.x);
@@ -228,8 +236,10 @@ __global__ __launch_bounds__(kBlock) void k_force(StepArgs a) {
     const double4 xi = a.xr_in[i];
     double ui[6];
     load6(a.uw_in + 6l * i, ui);
-    const double4 mti = a.mat[i];
-    const double2 mI = a.mi[i];
+    const double2 mI = a.mI[i]; // m, I
+    // (1-nu^2)/E, (2-nu)/G, 1/r, 1/m of i: read once, on the first contact
+    double4 mti = make_double4(0., 0., 0., 0.);
+    bool have_mti = false;
     double F[3], T[3];
     if (ACCUM) {
       double ft[6];
@@ -241,22 +251,36 @@ __global__ __launch_bounds__(kBlock) void k_force(StepArgs a) {
       else { F[0] = 0.; F[1] = 0.; F[2] = 0.; }
       T[0] = 0.; T[1] = 0.; T[2] = 0.;
     }
-    int nnb = 0, nc = 0, nwc = 0;
+    const int meta = a.cmeta_in[i];
+    int nnb = 0, nc = meta_count(meta), nwc = 0;
     if (PP) {
       // Phase 1: contact detection over the neighbour row.
       nnb = a.nn[i];
-      for (int k = 0; k < nnb; ++k) {
-        const int j = a.nbr[k * ld + i];
-        const double4 xj = a.xr_in[j];
-        const double d0 = xi.x - xj.x, d1 = xi.y - xj.y, d2 = xi.z - xj.z;
-        const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
-        const double s = xi.w + xj.w;
-        // r2 < s^2 is necessary for sqrt(r2) < s; the 1e-12 margin covers
-        // the rounding of s*s, so no contact is ever filtered out here.
-        if (r2 < s * s * (1. + 1e-12)) {
-          if (s - sqrt(r2) > 0.) {
-            if (nc < a.max_c) s_cj[nc * kBlock + tid] = j;
-            ++nc;
+      nc = 0;
+      // Batches of kPhase1Batch independent row loads, then gathers, then
+      // tests: several gathers in flight per lane instead of one chain.
+      for (int k0 = 0; k0 < nnb; k0 += kPhase1Batch) {
+        int jb[kPhase1Batch];
+        double4 xb[kPhase1Batch];
+#pragma unroll
+        for (int q = 0; q < kPhase1Batch; ++q)
+          jb[q] = (k0 + q < nnb) ? a.nbr[(k0 + q) * ld + i] : i;
+#pragma unroll
+        for (int q = 0; q < kPhase1Batch; ++q) xb[q] = a.xr_in[jb[q]];
+#pragma unroll
+        for (int q = 0; q < kPhase1Batch; ++q) {
+          if (k0 + q >= nnb) break;
+          const double4 xj = xb[q];
+          const double d0 = xi.x - xj.x, d1 = xi.y - xj.y, d2 = xi.z - xj.z;
+          const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+          const double s = xi.w + xj.w;
+          // r2 < s^2 is necessary for sqrt(r2) < s; the 1e-12 margin covers
+          // the rounding of s*s, so no contact is ever filtered out here.
+          if (r2 < s * s * (1. + 1e-12)) {
+            if (s - sqrt(r2) > 0.) {
+              if (nc < a.max_c) s_cj[nc * kBlock + tid] = jb[q];
+              ++nc;
+            }
           }
         }
       }
@@ -265,9 +289,10 @@ __global__ __launch_bounds__(kBlock) void k_force(StepArgs a) {
         nc = a.max_c;
       }
       // Phase 2: contact arithmetic, merged with last step's tracked list.
-      const int oc = a.ccnt_in[i];
+      const int oc = meta_count(meta);
       int p = 0;
       int jo = oc > 0 ? a.cj_in[i] : INT_MAX;
+      if (nc > 0) { mti = a.mat[i]; have_mti = true; }
       for (int m = 0; m < nc; ++m) {
         const int j = s_cj[m * kBlock + tid];
         while (jo < j) {
@@ -290,17 +315,18 @@ __global__ __launch_bounds__(kBlock) void k_force(StepArgs a) {
         a.ch_out[(3 * m + 1) * ld + i] = h[1];
         a.ch_out[(3 * m + 2) * ld + i] = h[2];
       }
-      a.ccnt_out[i] = nc;
     }
+    int wn = meta_walls(meta);
     if (PW) {
-      const int wm = a.wmask[i];
-      int wn = 0;
+      const int wm = wn;
+      wn = 0;
       for (int w = 0; w < a.n_walls; ++w) {
         const WallConst& W = a.walls[w];
         const double q0 = xi.x - W.x[0], q1 = xi.y - W.x[1], q2 = xi.z - W.x[2]; // :497-499
         const double tmp = q0 * W.n[0] + q1 * W.n[1] + q2 * W.n[2];              // :514
         const double overlap = xi.w - tmp;                                        // :515
         if (overlap > 0.) {                                                       // :556
+          if (!have_mti) { mti = a.mat[i]; have_mti = true; }
           double sw[3] = {0., 0., 0.};
           if (wm & (1 << w)) {
             sw[0] = a.sw[(3 * w + 0) * ld + i];
@@ -316,19 +342,20 @@ __global__ __launch_bounds__(kBlock) void k_force(StepArgs a) {
         }
         // no overlap: the history is zero (:666-670); the cleared bit says so
       }
-      a.wmask[i] = wn;
     }
+    if (PP) a.cmeta_out[i] = make_meta(nc, wn);
+    else if (PW) a.cmeta_in[i] = make_meta(nc, wn); // walls only: in place
     if (KICK == 0) {
       double ft[6] = {F[0], F[1], F[2], T[0], T[1], T[2]};
       store6(a.ft + 6l * i, ft);
       if (DIAG) {
-        const double I = a.Imom[i];
         dg[0] = 0.5 * mI.x * (ui[0] * ui[0] + ui[1] * ui[1] + ui[2] * ui[2]);
-        dg[1] = 0.5 * I * (ui[3] * ui[3] + ui[4] * ui[4] + ui[5] * ui[5]);
+        dg[1] = 0.5 * mI.y * (ui[3] * ui[3] + ui[4] * ui[4] + ui[5] * ui[5]);
         dg[2] = mI.x * ui[0]; dg[3] = mI.x * ui[1]; dg[4] = mI.x * ui[2];
       }
     } else {
-      const double minv = mti.w, Iinv = mI.y;
+      // 1/m and 1/I per use, src/CabanaDEM_Integrator.hpp:42,48
+      const double minv = 1. / mI.x, Iinv = 1. / mI.y;
       double s[6];
       // stage3: src/CabanaDEM_Integrator.hpp:89-98
       s[0] = ui[0] + a.hdt * F[0] * minv;
@@ -338,9 +365,8 @@ __global__ __launch_bounds__(kBlock) void k_force(StepArgs a) {
       s[4] = ui[4] + a.hdt * T[1] * Iinv;
       s[5] = ui[5] + a.hdt * T[2] * Iinv;
       if (DIAG) {
-        const double I = a.Imom[i];
         dg[0] = 0.5 * mI.x * (s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
-        dg[1] = 0.5 * I * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]);
+        dg[1] = 0.5 * mI.y * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]);
         dg[2] = mI.x * s[0]; dg[3] = mI.x * s[1]; dg[4] = mI.x * s[2];
       }
       if (KICK == 2) {
@@ -372,7 +398,7 @@ __global__ __launch_bounds__(kBlock) void k_force(StepArgs a) {
 __global__ __launch_bounds__(kBlock) void k_zero_forces(StepArgs a) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= a.n) return;
-  const double2 mI = a.mi[i];
+  const double2 mI = a.mI[i];
   double ft[6] = {0., 0., 0., 0., 0., 0.};
   if (a.gravity_on) { ft[0] = mI.x * a.g[0]; ft[1] = mI.x * a.g[1]; ft[2] = mI.x * a.g[2]; }
   store6(a.ft + 6l * i, ft);
@@ -386,7 +412,8 @@ __global__ __launch_bounds__(kBlock) void k_update_tangential(StepArgs a) {
   if (i >= a.n) return;
   const long ld = a.ld;
   const double4 xi = a.xr_in[i];
-  const int oc = a.ccnt_in[i];
+  const int meta = a.cmeta_in[i];
+  const int oc = meta_count(meta);
   int m = 0;
   for (int q = 0; q < oc; ++q) {
     const int j = a.cj_in[q * ld + i];
@@ -399,7 +426,7 @@ __global__ __launch_bounds__(kBlock) void k_update_tangential(StepArgs a) {
     for (int c = 0; c < 3; ++c) a.ch_out[(3 * m + c) * ld + i] = a.ch_in[(3 * q + c) * ld + i];
     ++m;
   }
-  a.ccnt_out[i] = m;
+  a.cmeta_out[i] = make_meta(m, meta_walls(meta));
 }
 
 // Half kick (stage1/stage3) and/or drift (stage2), in place.
@@ -412,7 +439,8 @@ __global__ __launch_bounds__(kBlock) void k_kick(StepArgs a, double4* xr, double
   if (do_kick) {
     double ft[6];
     load6(ftp + 6l * i, ft);
-    const double minv = a.mat[i].w, Iinv = a.mi[i].y;
+    const double2 mI = a.mI[i];
+    const double minv = 1. / mI.x, Iinv = 1. / mI.y; // src/CabanaDEM_Integrator.hpp:42,48
     for (int d = 0; d < 3; ++d) s[d] = s[d] + a.hdt * ft[d] * minv;
     for (int d = 0; d < 3; ++d) s[3 + d] = s[3 + d] + a.hdt * ft[3 + d] * Iinv;
     store6(uw + 6l * i, s);
@@ -433,13 +461,13 @@ __global__ __launch_bounds__(kBlock) void k_diag_current(StepArgs a, const doubl
   if (i < a.n) {
     double s[6];
     load6(uw + 6l * i, s);
-    const double m = a.mi[i].x, I = a.Imom[i];
+    const double m = a.mI[i].x, I = a.mI[i].y;
     dg[0] = 0.5 * m * (s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
     dg[1] = 0.5 * I * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]);
     dg[2] = m * s[0]; dg[3] = m * s[1]; dg[4] = m * s[2];
     dg[5] = (double)a.nn[i];
-    dg[6] = (double)a.ccnt_in[i];
-    dg[7] = (double)__popc(a.wmask[i]);
+    dg[6] = (double)meta_count(a.cmeta_in[i]);
+    dg[7] = (double)__popc(meta_walls(a.cmeta_in[i]));
   }
   block_reduce8(dg, a.diag_part, blk);
 }

@@ -3,19 +3,24 @@
 //
 // Layout in HBM (all FP64 state stays resident; particles are stored in
 // cell order and re-sorted at every neighbour rebuild):
-//   xr   double4[n]        x, y, z, radius               (gathered for neighbours)
-//   uw   double[n][6]      u, omega                      (gathered for contacts)
-//   mat  double4[n]        (1-nu^2)/E, (2-nu)/G, 1/r, 1/m (contact constants)
-//   mi   double2[n]        m, 1/I                        (own-particle only)
-//   Imom double[n]         I                              (diagnostics)
-//   ft   double[n][6]      F, T
-//   id   int[n]            original particle id of each sorted slot
-//   nbr  int[max_n][ld]    neighbour rows, slot-major (lane-coalesced), ascending j
-//   ccnt int[n], cj int[max_c][ld], ch double[max_c*3][ld]
-//                          tracked pp contacts: compact, ascending j, with the
-//                          tangential history of each
-//   wmask int[n], sw double[DEM_MAX_WALLS*3][ld]
-//                          wall-contact bits and wall tangential history
+//   xr    double4[n]        x, y, z, radius               (gathered for neighbours)
+//   uw    double[n][6]      u, omega                      (gathered for contacts)
+//   mI    double2[n]        m, I  (1/m and 1/I are formed per use, as the
+//                           reference does, src/CabanaDEM_Integrator.hpp:42,48)
+//   mat   double4[n]        (1-nu^2)/E, (2-nu)/G, 1/r, 1/m (read only on contact)
+//   ft    double[n][6]      F, T
+//   id    int[n]            original particle id of each sorted slot
+//   nn    int[n], nbr int[max_n][ld]
+//                           neighbour rows, slot-major (lane-coalesced), ascending j
+//   cmeta int[n]            tracked-contact count (bits 0-7) | wall-contact bits (8-15)
+//   cj    int[max_c][ld], ch double[max_c*3][ld]
+//                           tracked pp contacts: compact, ascending j, with the
+//                           tangential history of each
+//   sw    double[DEM_MAX_WALLS*3][ld]  wall tangential history
+// Cell order: cells are numbered in y-tiles of kTileRows full x-rows,
+// z-major inside a tile, so the z-neighbours of a particle sit a tile's
+// area of cells away (tens of kB) instead of a whole x-y layer; the
+// neighbour gathers then hit L2.
 // Arrays written by the fused step kernel or by the re-sort are double
 // buffered ([2]) and swapped by the host.
 #pragma once
@@ -27,9 +32,11 @@
 
 namespace demhip {
 
-constexpr int kBlock = 256;      // threads per block of the particle kernels
-constexpr int kMaxContacts = 32; // upper bound of max_contacts
-constexpr int kXcds = 8;         // MI355X: 8 XCDs, workgroups dealt round-robin
+constexpr int kBlock = 256;       // threads per block of the particle kernels
+constexpr int kMaxContacts = 32;  // upper bound of max_contacts
+constexpr int kMaxNeighbors = 255;
+constexpr int kXcds = 8;          // MI355X: 8 XCDs, workgroups dealt round-robin
+constexpr int kTileRows = 8;      // y-rows per cell tile
 
 // Logical block index that gives each XCD one contiguous run of blocks, so
 // that the neighbour gathers of one XCD stay inside its own L2.  Workgroup b
@@ -50,11 +57,20 @@ struct WallConst {
 
 struct Grid {
   double lo[3];
-  double inv_h[3]; // unused by the exact path; kept for diagnostics
   double h[3];
-  int nc[3];
-  int pad;
+  int nc[3];  // cells per dimension
+  int nty;    // y-tiles
 };
+
+// Cell key of cell (cx, cy, cz): tile-major, then z, then y inside the
+// tile, then x.  Keys of x-adjacent cells of one row are consecutive.
+__host__ __device__ inline long cell_key(const Grid& g, int cx, int cy, int cz) {
+  const int ty = cy / kTileRows, ly = cy % kTileRows;
+  return (((long)ty * g.nc[2] + cz) * kTileRows + ly) * g.nc[0] + cx;
+}
+__host__ __device__ inline long cell_key_count(const Grid& g) {
+  return (long)g.nty * g.nc[2] * kTileRows * g.nc[0];
+}
 
 struct DevFlags {
   int nonfinite;
@@ -72,6 +88,10 @@ struct DevFlags {
 // running quadratic work.
 constexpr int kMaxCellParticles = 4096;
 
+__host__ __device__ inline int meta_count(int m) { return m & 0xFF; }
+__host__ __device__ inline int meta_walls(int m) { return (m >> 8) & 0xFF; }
+__host__ __device__ inline int make_meta(int count, int walls) { return count | (walls << 8); }
+
 // Pointers and scalars of one launch, resolved for the current buffers.
 struct StepArgs {
   int n, ld, n_walls, max_n, max_c;
@@ -88,17 +108,15 @@ struct StepArgs {
   const double* uw_in;
   double* uw_out;
   const double4* mat;
-  const double2* mi;
-  const double* Imom;
+  const double2* mI;
   double* ft;
-  int* wmask;
   double* sw;
   const int* nn;
   const int* nbr;
-  const int* ccnt_in;
+  int* cmeta_in;   // written in place only by the walls-only kernel
   const int* cj_in;
   const double* ch_in;
-  int* ccnt_out;
+  int* cmeta_out;
   int* cj_out;
   double* ch_out;
   DevFlags* flags;
@@ -108,21 +126,21 @@ struct StepArgs {
 struct BuildArgs {
   int n, ld, max_n, max_c;
   Grid grid;
-  long ncell;
+  long ncell;        // number of cell keys (padded to whole tiles)
   double rc2;
   double skin_half_sq;
-  int check_skin;
+  int check_skin;    // compare with the positions of the last build
+  int keep_xb;       // record the positions of this build
+  int permute_ft;    // F and T are live across this rebuild
   // old (cur) and new (nxt) buffers
   const double4* xr_in; double4* xr_out;
   const double* uw_in; double* uw_out;
   const double4* mat_in; double4* mat_out;
-  const double2* mi_in; double2* mi_out;
-  const double* I_in; double* I_out;
+  const double2* mI_in; double2* mI_out;
   const int* id_in; int* id_out;
   const double* ft_in; double* ft_out;
-  const int* wmask_in; int* wmask_out;
   const double* sw_in; double* sw_out;
-  const int* ccnt_in; int* ccnt_out;
+  const int* cmeta_in; int* cmeta_out;
   const int* cj_in; int* cj_out;
   const double* ch_in; double* ch_out;
   double* xb;   // positions at the last build, sorted order

@@ -90,17 +90,26 @@ def plain_large(seed: int = 1, dims=(200, 200, 100)) -> Workload:
                     (0., 0., -9.81), 20, extra={"dims": dims})
 
 
-# Compulsory HBM bytes per particle-step (8 B per double, 4 B per id/count),
-# with n = neighbours, c = tracked contacts, w = wall contacts per particle.
-def bytes_fused_step(nbar, cbar, wbar):
-    """The fused step kernel: reads x,r (32) u,omega (48) contact constants (32)
-    m,1/I (16) row count (4) row ids (4n) contact count (4) old contacts (28c)
-    wall bits (4) wall history (24w); writes x,r (32) u,omega (48) contact
-    count (4) new contacts (28c) wall bits (4) wall history (24w)."""
-    return 228. + 4. * nbar + 56. * cbar + 48. * wbar
+# Bytes per particle-step (8 B per double, 4 B per id/count), with n = list
+# neighbours, c = tracked contacts, w = wall contacts per particle.
+def bytes_reference_passes(nbar, cbar, wbar):
+    """Algorithmic bytes of the work the fused step kernel does, in the
+    reference data model: kick + drift pass 208 B (read x,u,omega,F,T 120 +
+    m,I 16, write x,u,omega 72) plus force pass 172 + 4n + 56c + 48w B
+    (read x,u,omega 72 + r,m,E,nu,G 40 + list 4 + 4n + history 4 + 28c,
+    write F,T 48 + history 4 + 28c, wall history 48w)."""
+    return 380. + 4. * nbar + 56. * cbar + 48. * wbar
 
 
 def bytes_reference_model(nbar, cbar, wbar, K):
-    """Two-pass reference data model: kick+drift 208 B, force pass
-    172 + 4n + 56c + 48w B, plus (36 + 4n)/K for the rebuild."""
-    return 380. + 4. * nbar + 56. * cbar + 48. * wbar + (36. + 4. * nbar) / K
+    """Whole step in the reference data model: the two passes plus the
+    amortised rebuild (read x,r 32, write count and ids 4 + 4n) every K steps."""
+    return bytes_reference_passes(nbar, cbar, wbar) + (36. + 4. * nbar) / K
+
+
+def bytes_fused_step(nbar, cbar, wbar):
+    """Compulsory traffic of this library's fused step kernel: reads x,r (32),
+    u,omega (48), m,I (16), row count (4), row ids (4n), contact word (4), old
+    contacts (28c), wall history (24w); writes x,r (32), u,omega (48),
+    contact word (4), new contacts (28c), wall history (24w)."""
+    return 188. + 4. * nbar + 56. * cbar + 48. * wbar

@@ -118,6 +118,12 @@ typedef struct dem_counters {
 int dem_create(const dem_config* cfg, dem_ctx** out);
 void dem_destroy(dem_ctx* ctx);
 const char* dem_last_error(const dem_ctx* ctx); /* ctx may be NULL: last global error */
+/* Updates the non-structural parameters of a live context: dt, restitution
+ * and friction coefficients, gravity, skin, rebuild_every, physics_mode.
+ * n, max_contacts, cutoff, lo and hi must equal the creation values.
+ * Replaces: constructing a new Force (src/CabanaDEM_Force.hpp:23-28) or
+ * Integrator (src/CabanaDEM_Integrator.hpp:22-27) with other scalars. */
+int dem_set_params(dem_ctx* ctx, const dem_config* cfg);
 int dem_abi_version(void);
 int dem_device_count(void);
 
@@ -128,7 +134,8 @@ int dem_upload_particles(dem_ctx* ctx, const dem_particles_view* v);
 /* Replaces: host access through the Particles slices after a run
  * (slice getters src/CabanaDEM_Particles.hpp:99-271). */
 int dem_download_particles(dem_ctx* ctx, dem_particles_view* v);
-/* Replaces: Wall::updateParticles (src/CabanaDEM_Wall.hpp:75-82). */
+/* Replaces: Wall::updateParticles (src/CabanaDEM_Wall.hpp:75-82).  Sets the
+ * number of walls in use (0 .. DEM_MAX_WALLS). */
 int dem_upload_walls(dem_ctx* ctx, const dem_walls_view* w);
 
 /* ---- neighbour list ------------------------------------------------------ */

@@ -1,0 +1,90 @@
+"""The reference's validation set-ups through the drop-in C++ API on the GPU
+(examples/dem_validation.cpp, built by build()), checked against the oracle
+running the same set-up and against the closed forms."""
+import json
+import math
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import scenarios as S
+from gpu_helpers import normwise
+from oracle_driver import solver_run
+
+pytestmark = pytest.mark.gpu
+
+ROOT = Path(__file__).resolve().parents[1]
+EXE = ROOT / "examples" / "_build" / "dem_validation"
+INPUTS = ROOT / "tests" / "golden" / "inputs"
+
+
+def run_driver(case, json_name, out_dir, extra=None):
+    assert EXE.exists(), "build() compiles examples/_build/dem_validation"
+    cmd = [str(EXE), case, str(INPUTS / json_name), str(out_dir)] + ([extra] if extra else [])
+    print("driver:", " ".join(cmd), flush=True)
+    p = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+    return json.loads(p.stdout.strip().splitlines()[-1])
+
+
+def state_arrays(res):
+    ps = res["particles"]
+    return {k: np.array([p[k] for p in ps]) for k in ("x", "u", "omega", "force")}
+
+
+def compare_to_oracle(res, st, tol=1e-9):
+    g = state_arrays(res)
+    assert normwise(g["x"], st.x) <= tol
+    assert normwise(g["u"], st.u) <= tol
+    if np.abs(st.omega).max() > 0:
+        assert normwise(g["omega"], st.omega) <= tol
+
+
+def test_head_on_matches_oracle_and_hertz(gpu_lib, tmp_path):
+    res = run_driver("head_on", "01_elastic_normal_impact_of_two_identical_particles.json", tmp_path)
+    st, w, prm, run, inp = S.ex01()
+    solver_run(st, w, prm, run)
+    compare_to_oracle(res, st)
+    # the solver's CSV output every 30 steps: peak contact force near the Hertz value
+    peak = 0.0
+    for f in tmp_path.glob("particles_*.csv"):
+        a = np.loadtxt(f, delimiter=",", skiprows=2, ndmin=2)
+        peak = max(peak, a[1, 9])  # forces_0 of particle 1
+    assert abs(peak - 10.697e3) / 10.697e3 < 0.02  # sampled every 3 us of a 40 us contact
+    assert len(list(tmp_path.glob("particles_*.csv"))) == 800 // 30 + 1
+
+
+def test_sphere_wall_matches_oracle(gpu_lib, tmp_path):
+    res = run_driver("sphere_wall", "02_elastic_normal_impact_of_particle_wall.json", tmp_path)
+    st, w, prm, run, inp = S.ex02()
+    solver_run(st, w, prm, run)
+    compare_to_oracle(res, st)
+
+
+@pytest.mark.parametrize("cor", [0.2, 0.6])
+def test_restitution_matches_oracle(gpu_lib, tmp_path, cor):
+    res = run_driver("cor", "03_normal_particle_wall_different_cor.json", tmp_path, f"--cor-pw={cor}")
+    st, w, prm, run, inp = S.ex03(cor)
+    solver_run(st, w, prm, run)
+    compare_to_oracle(res, st)
+    assert abs(res["particles"][0]["u"][1] / inp["velocity_p"] - cor) < 0.02 + 0.02 * cor
+
+
+def test_oblique_matches_oracle(gpu_lib, tmp_path):
+    res = run_driver("oblique", "04_oblique_particle_wall_different_angles.json", tmp_path, "--angle=30.")
+    st, w, prm, run, inp = S.ex04(30.)
+    solver_run(st, w, prm, run)
+    compare_to_oracle(res, st)
+    assert abs(res["particles"][0]["omega"][2] - st.omega[0, 2]) <= 1e-9 * abs(st.omega[0, 2])
+
+
+def test_one_hits_two_matches_oracle(gpu_lib, tmp_path):
+    res = run_driver("one_hits_two", "tst_03_multiple_particles_contacts.json", tmp_path)
+    st, w, prm, run, inp = S.tst03()
+    solver_run(st, w, prm, run)
+    compare_to_oracle(res, st)
+    g = state_arrays(res)
+    p = (st.m[:, None] * g["u"]).sum(0)
+    assert abs(p[0] - st.m[0] * inp["velocity_p"]) < 1e-12 * st.m[0] * inp["velocity_p"]
