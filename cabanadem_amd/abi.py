@@ -15,7 +15,12 @@ import numpy as np
 LIB_PATH = Path(os.environ.get("CABANADEM_LIB") or
                 Path(__file__).resolve().parent / "lib" / "libcabanadem_hip.so")
 
+DEM_ABI_VERSION = 2  # include/cabanadem.h DEM_ABI_VERSION
 DEM_MAX_WALLS = 8
+DEM_MAX_GROUPS = 8
+DEM_MAX_BONDS = 32
+GROUP_VELOCITY = 0
+GROUP_FIXED_FORCE = 1
 DEM_OK = 0
 ERRORS = {
     -1: "DEM_ERR_ARG",
@@ -69,7 +74,25 @@ class DemCounters(C.Structure):
         ("steps", C.c_int64), ("rebuilds", C.c_int64),
         ("max_neighbors_seen", C.c_int), ("neighbor_capacity", C.c_int),
         ("mean_neighbors", C.c_double), ("mean_contacts", C.c_double),
-        ("mean_wall_contacts", C.c_double),
+        ("mean_wall_contacts", C.c_double), ("bonds_broken", C.c_int64),
+    ]
+
+
+class DemBondConfig(C.Structure):
+    _fields_ = [
+        ("max_bonds", C.c_int), ("unbonded_contacts", C.c_int),
+        ("beta_bond", C.c_double), ("radius_factor", C.c_double),
+        ("tensile_strength", C.c_double),
+    ]
+
+
+class DemBondsView(C.Structure):
+    _fields_ = [
+        ("n", C.c_int), ("max_bonds", C.c_int), ("idx", _ip), ("count", _ip),
+        ("init_len", _dp),
+        ("ft_x", _dp), ("ft_y", _dp), ("ft_z", _dp),
+        ("mn_x", _dp), ("mn_y", _dp), ("mn_z", _dp),
+        ("mt_x", _dp), ("mt_y", _dp), ("mt_z", _dp),
     ]
 
 
@@ -101,6 +124,13 @@ SYMBOLS = {
     "dem_synchronize": (C.c_int, [C.c_void_p]),
     "dem_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "dem_get_timing": (C.c_int, [C.c_void_p, _dp, C.POINTER(C.c_int64)]),
+    "dem_enable_bonds": (C.c_int, [C.c_void_p, C.POINTER(DemBondConfig)]),
+    "dem_setup_bonds": (C.c_int, [C.c_void_p, C.c_double]),
+    "dem_upload_bonds": (C.c_int, [C.c_void_p, C.POINTER(DemBondsView)]),
+    "dem_download_bonds": (C.c_int, [C.c_void_p, C.POINTER(DemBondsView)]),
+    "dem_bond_forces": (C.c_int, [C.c_void_p]),
+    "dem_set_external_force": (C.c_int, [C.c_void_p, _dp]),
+    "dem_set_groups": (C.c_int, [C.c_void_p, _ip, C.c_int, _ip, _dp]),
     "dem_generate_lattice": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, _dp, C.c_double,
                                        C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                        C.c_double, C.c_uint64, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
@@ -119,6 +149,9 @@ def load_library(path: Path | str | None = None) -> C.CDLL:
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
+    if lib.dem_abi_version() != DEM_ABI_VERSION:
+        raise RuntimeError(f"{p} has ABI version {lib.dem_abi_version()}, this binding expects "
+                           f"{DEM_ABI_VERSION}: rebuild the library")
     return lib
 
 
@@ -218,6 +251,38 @@ class Walls:
         return v
 
 
+class BondState:
+    """Host bond state in the reference layout: rows [n][max_bonds] per field."""
+
+    HIST = ("ft_x", "ft_y", "ft_z", "mn_x", "mn_y", "mn_z", "mt_x", "mt_y", "mt_z")
+
+    def __init__(self, n: int, max_bonds: int):
+        self.n, self.max_bonds = n, max_bonds
+        self.idx = -np.ones((n, max_bonds), dtype=np.int32)
+        self.count = np.zeros(n, dtype=np.int32)
+        self.init_len = np.zeros((n, max_bonds))
+        for f in self.HIST:
+            setattr(self, f, np.zeros((n, max_bonds)))
+
+    def history(self) -> np.ndarray:
+        """(n, max_bonds, 9): ft, mn, mt."""
+        return np.stack([getattr(self, f) for f in self.HIST], axis=-1)
+
+    def view(self) -> DemBondsView:
+        for f in ("idx", "count", "init_len") + self.HIST:
+            a = getattr(self, f)
+            want = np.int32 if f in ("idx", "count") else np.float64
+            if a.dtype != want or not a.flags.c_contiguous:
+                setattr(self, f, np.ascontiguousarray(a, dtype=want))
+        v = DemBondsView()
+        v.n, v.max_bonds = self.n, self.max_bonds
+        v.idx, v.count = _ptr(self.idx, C.c_int), _ptr(self.count, C.c_int)
+        v.init_len = _ptr(self.init_len)
+        for f in self.HIST:
+            setattr(v, f, _ptr(getattr(self, f)))
+        return v
+
+
 class System:
     """One DEM context on the GPU (a thin owner of a dem_ctx*)."""
 
@@ -310,6 +375,48 @@ class System:
         c = DemCounters()
         self._check(self._lib.dem_get_counters(self._h, C.byref(c)))
         return c
+
+    # ---- bonded DEM, groups, external force ----
+    def enable_bonds(self, max_bonds, *, beta_bond=0.1, radius_factor=1.0, tensile_strength=0.0,
+                     unbonded_contacts=False):
+        b = DemBondConfig()
+        b.max_bonds, b.unbonded_contacts = max_bonds, 1 if unbonded_contacts else 0
+        b.beta_bond, b.radius_factor, b.tensile_strength = beta_bond, radius_factor, tensile_strength
+        self._check(self._lib.dem_enable_bonds(self._h, C.byref(b)))
+        self.max_bonds = max_bonds
+
+    def setup_bonds(self, factor=3.0):
+        self._check(self._lib.dem_setup_bonds(self._h, factor))
+
+    def upload_bonds(self, b: BondState):
+        self._check(self._lib.dem_upload_bonds(self._h, C.byref(b.view())))
+
+    def download_bonds(self, b: BondState | None = None) -> BondState:
+        if b is None:
+            b = BondState(self.n, self.max_bonds)
+        self._check(self._lib.dem_download_bonds(self._h, C.byref(b.view())))
+        return b
+
+    def bond_forces(self):
+        self._check(self._lib.dem_bond_forces(self._h))
+
+    def set_external_force(self, f):
+        if f is None:
+            self._check(self._lib.dem_set_external_force(self._h, None))
+            return
+        a = np.ascontiguousarray(np.asarray(f, dtype=np.float64).reshape(self.n, 3))
+        self._check(self._lib.dem_set_external_force(self._h, _ptr(a)))
+
+    def set_groups(self, group, kinds=None, vel=None):
+        """group[i] in 0..len(kinds) (0 = none); kinds[g] = GROUP_*; vel[g] = (vx, vy, vz)."""
+        if group is None:
+            self._check(self._lib.dem_set_groups(self._h, None, 0, None, None))
+            return
+        g = np.ascontiguousarray(np.asarray(group, dtype=np.int32))
+        k = np.ascontiguousarray(np.asarray(kinds, dtype=np.int32))
+        v = None if vel is None else np.ascontiguousarray(np.asarray(vel, dtype=np.float64).reshape(-1, 3))
+        self._check(self._lib.dem_set_groups(self._h, _ptr(g, C.c_int), len(k), _ptr(k, C.c_int),
+                                             _ptr(v)))
 
     def set_timing(self, on: bool):
         self._check(self._lib.dem_set_timing(self._h, 1 if on else 0))

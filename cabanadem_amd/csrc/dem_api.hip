@@ -69,6 +69,21 @@ struct dem_ctx {
   double* ch[2] = {nullptr, nullptr};
   int cP = 0, cS = 0, cC = 0;
 
+  // bonded DEM, groups and external force: group S (permuted at rebuilds)
+  bool bonds_on = false;
+  dem_bond_config bcfg{};
+  int max_b = 0;
+  int* nbond[2] = {nullptr, nullptr};
+  int* bj[2] = {nullptr, nullptr};
+  double* bl[2] = {nullptr, nullptr};
+  double* bh[2] = {nullptr, nullptr};
+  double2* bEG[2] = {nullptr, nullptr};
+  int* vgroup[2] = {nullptr, nullptr};
+  int n_groups = 0;
+  int gkind[DEM_MAX_GROUPS] = {};
+  double gvel[DEM_MAX_GROUPS][3] = {};
+  double* fext[2] = {nullptr, nullptr};
+
   int* nn = nullptr;
   int* nbr = nullptr;
   double* xb = nullptr;
@@ -111,6 +126,8 @@ struct dem_ctx {
       hipFree(xr[b]); hipFree(uw[b]); hipFree(mat[b]); hipFree(mI[b]);
       hipFree(id[b]); hipFree(ft[b]); hipFree(sw[b]);
       hipFree(cmeta[b]); hipFree(cj[b]); hipFree(ch[b]);
+      hipFree(nbond[b]); hipFree(bj[b]); hipFree(bl[b]); hipFree(bh[b]); hipFree(bEG[b]);
+      hipFree(vgroup[b]); hipFree(fext[b]);
     }
     hipFree(nn); hipFree(nbr); hipFree(xb); hipFree(key); hipFree(rank); hipFree(key_sorted);
     hipFree(cell_count); hipFree(cell_start); hipFree(perm); hipFree(inv); hipFree(scan_tmp);
@@ -150,6 +167,23 @@ struct dem_ctx {
     a.cmeta_out = cmeta[cC ^ 1]; a.cj_out = cj[cC ^ 1]; a.ch_out = ch[cC ^ 1];
     a.flags = flags;
     a.diag_part = diag_part;
+    if (bonds_on) {
+      a.max_b = max_b;
+      a.nbond = nbond[cS]; a.bj = bj[cS]; a.bl = bl[cS]; a.bh = bh[cS]; a.bEG = bEG[cS];
+      a.beta_bond = bcfg.beta_bond;
+      a.bond_radius_factor = bcfg.radius_factor;
+      a.tensile_strength = bcfg.tensile_strength;
+      a.unbonded_contacts = bcfg.unbonded_contacts;
+    }
+    if (n_groups > 0) {
+      a.vgroup = vgroup[cS];
+      a.n_vgroups = n_groups;
+      for (int g = 0; g < DEM_MAX_GROUPS; ++g) {
+        a.vgkind[g] = gkind[g];
+        for (int d = 0; d < 3; ++d) a.vgv[g][d] = gvel[g][d];
+      }
+    }
+    a.fext = fext[cS];
     return a;
   }
 
@@ -176,6 +210,17 @@ struct dem_ctx {
     b.cell_count = cell_count; b.cell_start = cell_start;
     b.perm = perm; b.inv = inv; b.scan_tmp = scan_tmp;
     b.nn = nn; b.nbr = nbr; b.flags = flags;
+    b.bonds_on = bonds_on ? 1 : 0;
+    b.max_b = max_b;
+    if (bonds_on) {
+      b.nbond_in = nbond[cS]; b.nbond_out = nbond[cS ^ 1];
+      b.bj_in = bj[cS]; b.bj_out = bj[cS ^ 1];
+      b.bl_in = bl[cS]; b.bl_out = bl[cS ^ 1];
+      b.bh_in = bh[cS]; b.bh_out = bh[cS ^ 1];
+      b.bEG_in = bEG[cS]; b.bEG_out = bEG[cS ^ 1];
+    }
+    if (n_groups > 0) { b.vgroup_in = vgroup[cS]; b.vgroup_out = vgroup[cS ^ 1]; }
+    if (fext[cS]) { b.fext_in = fext[cS]; b.fext_out = fext[cS ^ 1]; }
     return b;
   }
 
@@ -214,12 +259,13 @@ struct dem_ctx {
     ev_marks.clear();
   }
 
-  int check_flags() {
+  int check_flags(DevFlags* out = nullptr) {
     DevFlags f;
     hipError_t e = hipMemcpyAsync(&f, flags, sizeof(f), hipMemcpyDeviceToHost, stream);
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
     if (e != hipSuccess) return hip_fail(e, "flag read-back");
     max_nn_seen = std::max(max_nn_seen, f.max_nn);
+    if (out) *out = f;
     if (f.nonfinite) return fail(DEM_ERR_NONFINITE, "non-finite particle position met by the cell binning");
     if (f.cell_overflow)
       return fail(DEM_ERR_ARG, "more than 4096 particles in one cell: the binning box (lo/hi) "
@@ -232,17 +278,38 @@ struct dem_ctx {
                                                 " overlapping neighbours exceeds max_contacts = " +
                                                 std::to_string(max_c));
     if (f.skin) return fail(DEM_ERR_SKIN, "a particle moved more than skin/2 between neighbour rebuilds");
+    if (f.bond_overflow)
+      return fail(DEM_ERR_BOND_OVERFLOW, "particle with " + std::to_string(f.bond_overflow) +
+                                             " bond partners exceeds max_bonds = " + std::to_string(max_b));
     return DEM_OK;
   }
   // Reports and clears the sticky device error flags.
   int check_and_clear_flags() {
-    int r = check_flags();
+    DevFlags f{};
+    int r = check_flags(&f);
     if (r != DEM_OK && r != DEM_ERR_HIP) {
       DevFlags z{};
+      z.bonds_broken = f.bonds_broken; // a running count, not an error
       hipMemcpyAsync(flags, &z, sizeof(z), hipMemcpyHostToDevice, stream);
       hipStreamSynchronize(stream);
     }
     return r;
+  }
+
+  // original id of each sorted slot (host copy of the current order)
+  int sorted_ids(std::vector<int>* id) {
+    id->resize(n);
+    hipError_t e = hipStreamSynchronize(stream);
+    if (e == hipSuccess) e = hipMemcpy(id->data(), this->id[cS], n * sizeof(int), hipMemcpyDeviceToHost);
+    return e == hipSuccess ? DEM_OK : hip_fail(e, "id read-back");
+  }
+
+  // E, G of each sorted slot for the bond stiffnesses.
+  int upload_bEG(const std::vector<int>& id) {
+    std::vector<double2> eg(n);
+    for (int p = 0; p < n; ++p) { eg[p].x = h_E[id[p]]; eg[p].y = h_G[id[p]]; }
+    hipError_t e = hipMemcpy(bEG[cS], eg.data(), n * sizeof(double2), hipMemcpyHostToDevice);
+    return e == hipSuccess ? DEM_OK : hip_fail(e, "upload bond moduli");
   }
 
   int alloc_nbr(int cap) {
@@ -283,6 +350,7 @@ struct dem_ctx {
       if (rc) return rc;
       DevFlags z{};
       z.max_nn = f.max_nn;
+      z.bonds_broken = f.bonds_broken;
       hipMemcpyAsync(flags, &z, sizeof(z), hipMemcpyHostToDevice, stream);
       b.max_n = max_n;
       b.nbr = nbr;
@@ -587,6 +655,8 @@ int dem_upload_particles(dem_ctx* c, const dem_particles_view* v) {
   }
   e = hipMemcpy(c->cmeta[c->cC], meta.data(), n * sizeof(int), hipMemcpyHostToDevice);
   if (e != hipSuccess) return c->hip_fail(e, "upload contact counts");
+  if (c->bonds_on && (v->E || v->G))
+    if (int r = c->upload_bEG(id)) return r;
   if (v->x) c->list_valid = false;
   return DEM_OK;
 }
@@ -722,7 +792,7 @@ static int force_call(dem_ctx* c, ForceKind kind, bool needs_list) {
   if (needs_list && !c->list_valid) return c->fail(DEM_ERR_STATE, "neighbour list not built for the current positions");
   hipError_t e = launch_force(kind, c->step_args(), false, c->stream);
   if (e != hipSuccess) return c->hip_fail(e, "force launch");
-  if (kind != FK_PW_ACCUM) c->cC ^= 1;
+  if (kind != FK_PW_ACCUM && kind != FK_BOND_ACCUM) c->cC ^= 1;
   return c->check_and_clear_flags();
 }
 
@@ -856,6 +926,7 @@ int dem_get_counters(dem_ctx* c, dem_counters* out) {
   out->mean_neighbors = c->last_sample[0];
   out->mean_contacts = c->last_sample[1];
   out->mean_wall_contacts = c->last_sample[2];
+  out->bonds_broken = f.bonds_broken;
   return DEM_OK;
 }
 
@@ -883,6 +954,200 @@ int dem_get_timing(dem_ctx* c, double ms[2], int64_t launches[2]) {
     if (ms) ms[k] = c->t_ms[k];
     if (launches) launches[k] = c->t_launches[k];
   }
+  return DEM_OK;
+}
+
+// ---- bonded DEM, groups, external force -----------------------------------
+
+int dem_enable_bonds(dem_ctx* c, const dem_bond_config* b) {
+  if (int r = check_ctx(c)) return r;
+  if (!b) return c->fail(DEM_ERR_ARG, "null bond config");
+  if (b->max_bonds <= 0 || b->max_bonds > DEM_MAX_BONDS)
+    return c->fail(DEM_ERR_ARG, "max_bonds must be in [1, 32]");
+  if (!(b->beta_bond >= 0.) || !(b->radius_factor > 0.))
+    return c->fail(DEM_ERR_ARG, "beta_bond must be >= 0 and radius_factor > 0");
+  if (c->bonds_on) {
+    if (b->max_bonds != c->max_b) return c->fail(DEM_ERR_ARG, "max_bonds cannot change once bonds are enabled");
+    hipStreamSynchronize(c->stream);
+    c->bcfg = *b;
+    return DEM_OK;
+  }
+  std::vector<int> id;
+  if (int r = c->sorted_ids(&id)) return r;
+  const size_t n = (size_t)c->n, ld = (size_t)c->ld, mb = (size_t)b->max_bonds;
+  hipError_t e = hipSuccess;
+  for (int k = 0; k < 2; ++k) {
+    c->nbond[k] = dalloc<int>(n, &e);
+    c->bj[k] = dalloc<int>(mb * ld, &e);
+    c->bl[k] = dalloc<double>(mb * ld, &e);
+    c->bh[k] = dalloc<double>(9 * mb * ld, &e);
+    c->bEG[k] = dalloc<double2>(n, &e);
+  }
+  if (e == hipSuccess) e = hipMemset(c->nbond[c->cS], 0, n * sizeof(int));
+  if (e != hipSuccess) {
+    for (int k = 0; k < 2; ++k) {
+      hipFree(c->nbond[k]); hipFree(c->bj[k]); hipFree(c->bl[k]); hipFree(c->bh[k]); hipFree(c->bEG[k]);
+      c->nbond[k] = c->bj[k] = nullptr; c->bl[k] = c->bh[k] = nullptr; c->bEG[k] = nullptr;
+    }
+    return c->hip_fail(e, "hipMalloc bonds");
+  }
+  c->max_b = b->max_bonds;
+  c->bcfg = *b;
+  c->bonds_on = true;
+  return c->upload_bEG(id);
+}
+
+int dem_setup_bonds(dem_ctx* c, double factor) {
+  if (int r = check_ctx(c)) return r;
+  if (!c->bonds_on) return c->fail(DEM_ERR_STATE, "bonds are not enabled (dem_enable_bonds)");
+  if (!(factor > 0.)) return c->fail(DEM_ERR_ARG, "bond factor must be positive");
+  const double rmax = *std::max_element(c->h_rad.begin(), c->h_rad.end());
+  if (factor * rmax > c->cfg.cutoff)
+    return c->fail(DEM_ERR_ARG, "factor * max radius exceeds the neighbour cutoff: bonds would be missed");
+  if (!c->list_valid)
+    if (int r = c->rebuild(true, false)) return r;
+  BondSetupArgs a{};
+  a.n = c->n; a.ld = c->ld; a.max_b = c->max_b; a.factor = factor;
+  a.xr = c->xr[c->cP]; a.nn = c->nn; a.nbr = c->nbr;
+  a.nbond = c->nbond[c->cS]; a.bj = c->bj[c->cS]; a.bl = c->bl[c->cS]; a.bh = c->bh[c->cS];
+  a.flags = c->flags;
+  hipError_t e = launch_setup_bonds(a, c->stream);
+  if (e != hipSuccess) return c->hip_fail(e, "bond set-up launch");
+  return c->check_and_clear_flags();
+}
+
+int dem_upload_bonds(dem_ctx* c, const dem_bonds_view* v) {
+  if (int r = check_ctx(c)) return r;
+  if (!c->bonds_on) return c->fail(DEM_ERR_STATE, "bonds are not enabled (dem_enable_bonds)");
+  if (!v || v->n != c->n || v->max_bonds <= 0 || !v->idx || !v->count)
+    return c->fail(DEM_ERR_ARG, "bond view needs n, max_bonds, idx and count");
+  std::vector<int> id;
+  if (int r = c->sorted_ids(&id)) return r;
+  const int n = c->n, M = v->max_bonds, MB = c->max_b;
+  const size_t ld = c->ld;
+  std::vector<int> pos(n);
+  for (int p = 0; p < n; ++p) pos[id[p]] = p;
+  std::vector<int> nb(n, 0), bjh((size_t)MB * ld, -1);
+  std::vector<double> blh((size_t)MB * ld, 0.), bhh((size_t)9 * MB * ld, 0.);
+  const double* hist[9] = {v->ft_x, v->ft_y, v->ft_z, v->mn_x, v->mn_y, v->mn_z, v->mt_x, v->mt_y, v->mt_z};
+  for (int p = 0; p < n; ++p) {
+    const int I = id[p];
+    const int cnt = v->count[I];
+    if (cnt < 0 || cnt > M) return c->fail(DEM_ERR_ARG, "bond count outside [0, max_bonds] of the view");
+    if (cnt > MB) return c->fail(DEM_ERR_BOND_OVERFLOW, "uploaded bonds exceed max_bonds");
+    nb[p] = cnt;
+    for (int k = 0; k < cnt; ++k) {
+      const size_t src = (size_t)I * M + k;
+      const int J = v->idx[src];
+      if (J >= n) return c->fail(DEM_ERR_ARG, "bond partner id out of range");
+      bjh[(size_t)k * ld + p] = J >= 0 ? pos[J] : -1;
+      blh[(size_t)k * ld + p] = v->init_len ? v->init_len[src] : 0.;
+      for (int q = 0; q < 9; ++q) bhh[(size_t)(9 * k + q) * ld + p] = hist[q] ? hist[q][src] : 0.;
+    }
+  }
+  hipError_t e = hipMemcpy(c->nbond[c->cS], nb.data(), n * sizeof(int), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(c->bj[c->cS], bjh.data(), bjh.size() * sizeof(int), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(c->bl[c->cS], blh.data(), blh.size() * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(c->bh[c->cS], bhh.data(), bhh.size() * sizeof(double), hipMemcpyHostToDevice);
+  return e == hipSuccess ? DEM_OK : c->hip_fail(e, "upload bonds");
+}
+
+int dem_download_bonds(dem_ctx* c, dem_bonds_view* v) {
+  if (int r = check_ctx(c)) return r;
+  if (!c->bonds_on) return c->fail(DEM_ERR_STATE, "bonds are not enabled (dem_enable_bonds)");
+  if (!v || v->n != c->n || v->max_bonds <= 0) return c->fail(DEM_ERR_ARG, "bond view needs n and max_bonds");
+  std::vector<int> id;
+  if (int r = c->sorted_ids(&id)) return r;
+  const int n = c->n, M = v->max_bonds, MB = c->max_b;
+  const size_t ld = c->ld;
+  std::vector<int> nb(n), bjh((size_t)MB * ld);
+  std::vector<double> blh((size_t)MB * ld), bhh((size_t)9 * MB * ld);
+  hipError_t e = hipMemcpy(nb.data(), c->nbond[c->cS], n * sizeof(int), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(bjh.data(), c->bj[c->cS], bjh.size() * sizeof(int), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(blh.data(), c->bl[c->cS], blh.size() * sizeof(double), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(bhh.data(), c->bh[c->cS], bhh.size() * sizeof(double), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return c->hip_fail(e, "download bonds");
+  double* hist[9] = {v->ft_x, v->ft_y, v->ft_z, v->mn_x, v->mn_y, v->mn_z, v->mt_x, v->mt_y, v->mt_z};
+  for (int p = 0; p < n; ++p) {
+    if (nb[p] > M) return c->fail(DEM_ERR_ARG, "view max_bonds is smaller than a particle's bond count");
+  }
+  for (int p = 0; p < n; ++p) {
+    const int I = id[p];
+    for (int k = 0; k < M; ++k) {
+      const size_t dst = (size_t)I * M + k;
+      const bool live = k < nb[p];
+      const int j = live ? bjh[(size_t)k * ld + p] : -1;
+      if (v->idx) v->idx[dst] = j >= 0 ? id[j] : -1;
+      if (v->init_len) v->init_len[dst] = live ? blh[(size_t)k * ld + p] : 0.;
+      for (int q = 0; q < 9; ++q)
+        if (hist[q]) hist[q][dst] = live ? bhh[(size_t)(9 * k + q) * ld + p] : 0.;
+    }
+    if (v->count) v->count[I] = nb[p];
+  }
+  return DEM_OK;
+}
+
+int dem_bond_forces(dem_ctx* c) {
+  if (int r = check_ctx(c)) return r;
+  if (!c->bonds_on) return c->fail(DEM_ERR_STATE, "bonds are not enabled (dem_enable_bonds)");
+  return force_call(c, FK_BOND_ACCUM, false);
+}
+
+int dem_set_external_force(dem_ctx* c, const double* f) {
+  if (int r = check_ctx(c)) return r;
+  hipError_t e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return c->hip_fail(e, "sync");
+  if (!f) {
+    for (int k = 0; k < 2; ++k) { hipFree(c->fext[k]); c->fext[k] = nullptr; }
+    return DEM_OK;
+  }
+  std::vector<int> id;
+  if (int r = c->sorted_ids(&id)) return r;
+  const size_t n = (size_t)c->n;
+  for (int k = 0; k < 2; ++k)
+    if (!c->fext[k]) c->fext[k] = dalloc<double>(3 * n, &e);
+  if (e != hipSuccess) return c->hip_fail(e, "hipMalloc external force");
+  std::vector<double> h(3 * n);
+  for (size_t p = 0; p < n; ++p)
+    for (int d = 0; d < 3; ++d) h[3 * p + d] = f[3 * (size_t)id[p] + d];
+  e = hipMemcpy(c->fext[c->cS], h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice);
+  return e == hipSuccess ? DEM_OK : c->hip_fail(e, "upload external force");
+}
+
+int dem_set_groups(dem_ctx* c, const int* group, int ngroups, const int* kind, const double* vel) {
+  if (int r = check_ctx(c)) return r;
+  hipError_t e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return c->hip_fail(e, "sync");
+  if (!group) {
+    for (int k = 0; k < 2; ++k) { hipFree(c->vgroup[k]); c->vgroup[k] = nullptr; }
+    c->n_groups = 0;
+    return DEM_OK;
+  }
+  if (ngroups < 1 || ngroups > DEM_MAX_GROUPS || !kind)
+    return c->fail(DEM_ERR_ARG, "ngroups must be in [1, 8] with a kind per group");
+  for (int g = 0; g < ngroups; ++g) {
+    if (kind[g] != DEM_GROUP_VELOCITY && kind[g] != DEM_GROUP_FIXED_FORCE)
+      return c->fail(DEM_ERR_ARG, "unknown group kind");
+    if (kind[g] == DEM_GROUP_VELOCITY && !vel)
+      return c->fail(DEM_ERR_ARG, "a velocity group needs vel");
+  }
+  const int n = c->n;
+  for (int i = 0; i < n; ++i)
+    if (group[i] < 0 || group[i] > ngroups) return c->fail(DEM_ERR_ARG, "group index outside [0, ngroups]");
+  std::vector<int> id;
+  if (int r = c->sorted_ids(&id)) return r;
+  for (int k = 0; k < 2; ++k)
+    if (!c->vgroup[k]) c->vgroup[k] = dalloc<int>((size_t)n, &e);
+  if (e != hipSuccess) return c->hip_fail(e, "hipMalloc groups");
+  std::vector<int> h(n);
+  for (int p = 0; p < n; ++p) h[p] = group[id[p]];
+  e = hipMemcpy(c->vgroup[c->cS], h.data(), n * sizeof(int), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return c->hip_fail(e, "upload groups");
+  for (int g = 0; g < DEM_MAX_GROUPS; ++g) {
+    c->gkind[g] = g < ngroups ? kind[g] : 0;
+    for (int d = 0; d < 3; ++d) c->gvel[g][d] = (g < ngroups && vel) ? vel[3 * g + d] : 0.;
+  }
+  c->n_groups = ngroups;
   return DEM_OK;
 }
 

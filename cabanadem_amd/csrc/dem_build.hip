@@ -201,6 +201,24 @@ __global__ __launch_bounds__(kBlock) void k_permute(BuildArgs a) {
         for (int c = 0; c < 3; ++c)
           a.sw_out[(long)(w * 3 + c) * a.ld + p] = a.sw_in[(long)(w * 3 + c) * a.ld + o];
   }
+  if (a.bonds_on) {
+    // bond rows keep their slot order; partners follow the permutation
+    const int nb = a.nbond_in[o];
+    a.nbond_out[p] = nb;
+    a.bEG_out[p] = a.bEG_in[o];
+    if (a.vgroup_in) a.vgroup_out[p] = a.vgroup_in[o];
+    for (int b = 0; b < nb; ++b) {
+      const int j = a.bj_in[(long)b * a.ld + o];
+      a.bj_out[(long)b * a.ld + p] = j >= 0 ? a.inv[j] : -1;
+      a.bl_out[(long)b * a.ld + p] = a.bl_in[(long)b * a.ld + o];
+      for (int c = 0; c < 9; ++c)
+        a.bh_out[(long)(9 * b + c) * a.ld + p] = a.bh_in[(long)(9 * b + c) * a.ld + o];
+    }
+  } else if (a.vgroup_in) {
+    a.vgroup_out[p] = a.vgroup_in[o];
+  }
+  if (a.fext_in)
+    for (int d = 0; d < 3; ++d) a.fext_out[3l * p + d] = a.fext_in[3l * o + d];
   int oc = meta_count(meta);
   if (oc > a.max_c) oc = a.max_c; // counts are clamped on write; never index past the LDS rows
   for (int m = 0; m < oc; ++m) s_j[m * kBlock + tid] = a.inv[a.cj_in[(long)m * a.ld + o]];
@@ -286,9 +304,43 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(BuildArgs a) {
   }
 }
 
+// src/CabanaBondedDEM_Particles.hpp:118-137: bond i to every list neighbour
+// j with r_ij < factor * r_i (the reference hard-codes factor 3), storing the
+// partner and the initial length in the next slot.  Rows are ascending in j.
+__global__ __launch_bounds__(kBlock) void k_setup_bonds(BondSetupArgs a) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.n) return;
+  const double4 xi = a.xr[i];
+  const double cond = a.factor * xi.w;                 // :130
+  int nb = 0;
+  for (int k = 0; k < a.nn[i]; ++k) {
+    const int j = a.nbr[(long)k * a.ld + i];
+    const double4 xj = a.xr[j];
+    const double p0 = xi.x - xj.x, p1 = xi.y - xj.y, p2 = xi.z - xj.z; // :120-122
+    const double r2 = p0 * p0 + p1 * p1 + p2 * p2;                      // :124
+    const double rij = sqrt(r2);                                        // :126
+    if (rij < cond) {                                                   // :132
+      if (nb < a.max_b) {
+        a.bj[(long)nb * a.ld + i] = j;                                  // :133
+        a.bl[(long)nb * a.ld + i] = rij;                                // :134
+        for (int c = 0; c < 9; ++c) a.bh[(long)(9 * nb + c) * a.ld + i] = 0.;
+      } else {
+        atomicMax(&a.flags->bond_overflow, nb + 1);
+      }
+      ++nb;                                                             // :135
+    }
+  }
+  a.nbond[i] = nb < a.max_b ? nb : a.max_b;
+}
+
 inline unsigned nblk(long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 } // namespace
+
+hipError_t launch_setup_bonds(const BondSetupArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_setup_bonds, dim3(nblk(a.n)), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
 
 hipError_t launch_fill_int(int* p, long count, int value, hipStream_t s) {
   if (count <= 0) return hipSuccess;

@@ -199,6 +199,142 @@ __device__ inline void pw_contact(const double4& xi, const double* ui, const dou
   T[2] += (nx * ft1 - ny * ft0) * arm;
 }
 
+// Beam bond between i and j (src/CabanaBondedDEM_Force.hpp:86-341; line
+// numbers below), with the history h = (ft, mn, mt) and the rest length L0
+// of the bond's slot (the reference indexes both by the partner id j, quirk
+// Q9).  Returns true, contributing nothing, when the bond's tensile stress
+// E_i (r - L0) / (r_i + r_j) exceeds the strength (breakage, an extension).
+template <bool REF>
+__device__ inline bool bond_force(const double4& xi, const double* ui, double2 EGi,
+                                  const double4& xj, const double* uj, double L0, double* h,
+                                  double* F, double* T, const StepArgs& a) {
+  const double p0 = xi.x - xj.x, p1 = xi.y - xj.y, p2 = xi.z - xj.z; // :106-108
+  const double r2 = p0 * p0 + p1 * p1 + p2 * p2;                      // :111
+  const double rij = sqrt(r2);                                        // :113
+  const double rad_sum = xi.w + xj.w;                                 // :182
+  if (a.tensile_strength > 0.) {
+    const double stretch = rij - L0;
+    if (stretch > 0. && EGi.x * stretch / rad_sum > a.tensile_strength) return true;
+  }
+  const double g_c = rij - xi.w - xj.w;                               // :115
+  const double a_i = xi.w + g_c / 2.;                                 // :116
+  const double a_j = xj.w + g_c / 2.;                                 // :117
+  const double n0 = p0 / rij, n1 = p1 / rij, n2 = p2 / rij;           // :122
+  const double vi0 = ui[0] + (ui[4] * n2 - ui[5] * n1) * a_i;         // :130-137
+  const double vi1 = ui[1] + (ui[5] * n0 - ui[3] * n2) * a_i;
+  const double vi2 = ui[2] + (ui[3] * n1 - ui[4] * n0) * a_i;
+  const double vj0 = uj[0] + (-uj[4] * n2 + uj[5] * n1) * a_j;        // :139-146 (Q1)
+  const double vj1 = (REF ? ui[1] : uj[1]) + (-uj[5] * n0 + uj[3] * n2) * a_j;
+  const double vj2 = (REF ? ui[2] : uj[2]) + (-uj[3] * n1 + uj[4] * n0) * a_j;
+  const double vr0 = vi0 - vj0, vr1 = vi1 - vj1, vr2 = vi2 - vj2;    // :148-150
+  const double vdp = vr0 * n0 + vr1 * n1 + vr2 * n2;                  // :153
+  const double vn0 = vdp * n0, vn1 = vdp * n1, vn2 = vdp * n2;        // :154-156
+  const double vt0 = vr0 - vn0, vt1 = vr1 - vn1, vt2 = vr2 - vn2;    // :158-160
+  const double wr0 = ui[3] - uj[3], wr1 = ui[4] - uj[4], wr2 = ui[5] - uj[5]; // :163-165
+  const double wdp = wr0 * n0 + wr1 * n1 + wr2 * n2;                  // :167
+  const double wn0 = wdp * n0, wn1 = wdp * n1, wn2 = wdp * n2;        // :168
+  const double wt0 = wr0 - wn0, wt1 = wr1 - wn1, wt2 = wr2 - wn2;    // :169
+  const double beta = a.beta_bond;                                    // :186
+  const double rb = (xi.w < xj.w ? xi.w : xj.w) * a.bond_radius_factor; // :188
+  const double A_bond = M_PI * (rb * rb);                             // :189
+  const double I_bond = M_PI * ((rb * rb) * (rb * rb)) / 4.0;         // :190
+  const double I_p_bond = I_bond * 2.0;                               // :191
+  const double k_n = EGi.x * A_bond / rad_sum;                        // :195-198
+  const double k_t = EGi.y * A_bond / rad_sum;
+  const double k_tor = EGi.y * I_p_bond / rad_sum;
+  const double k_ben = EGi.x * I_bond / rad_sum;
+  const double overlap_n = rij - L0;                                  // :207
+  const double fn0 = -k_n * overlap_n * n0;                           // :208-210
+  const double fn1 = -k_n * overlap_n * n1;
+  const double fn2 = -k_n * overlap_n * n2;
+  double mx = (vn0 >= 0.0) ? 1.0 : -1.0;                              // :216-218
+  double my = (vn1 >= 0.0) ? 1.0 : -1.0;
+  double mz = (vn2 >= 0.0) ? 1.0 : -1.0;
+  // :220-222 add +beta|fn| sign(v_n), which feeds energy in (quirk Q15);
+  // corrected mode damps, as the tangential and moment terms do (:258-260)
+  const double bs = REF ? beta : -beta;
+  const double fd0 = fn0 + bs * fabs(fn0) * mx;
+  const double fd1 = fn1 + bs * fabs(fn1) * my;
+  const double fd2 = fn2 + bs * fabs(fn2) * mz;
+  double f0 = h[0], f1 = h[1], f2 = h[2];                             // :231-233
+  const double fdp = f0 * n0 + f1 * n1 + f2 * n2;                     // :235-239
+  f0 -= fdp * n0; f1 -= fdp * n1; f2 -= fdp * n2;
+  double tmp = k_t * a.dt;                                            // :241-246
+  f0 += -vt0 * tmp; f1 += -vt1 * tmp; f2 += -vt2 * tmp;
+  mx = (vt0 >= 0.0) ? 1.0 : -1.0;                                     // :251-256
+  my = (vt1 >= 0.0) ? 1.0 : -1.0;
+  mz = (vt2 >= 0.0) ? 1.0 : -1.0;
+  h[0] = f0 - beta * fabs(f0) * mx;                                   // :258-260
+  h[1] = f1 - beta * fabs(f1) * my;
+  h[2] = f2 - beta * fabs(f2) * mz;
+  double q0 = h[3], q1 = h[4], q2 = h[5];                             // :267-274
+  double tdp = q0 * n0 + q1 * n1 + q2 * n2;
+  q0 = tdp * n0; q1 = tdp * n1; q2 = tdp * n2;
+  tmp = k_tor * a.dt;                                                 // :276-280
+  q0 += -wn0 * tmp; q1 += -wn1 * tmp; q2 += -wn2 * tmp;
+  mx = (wn0 >= 0.0) ? 1.0 : -1.0;                                     // :282-287
+  my = (wn1 >= 0.0) ? 1.0 : -1.0;
+  mz = (wn2 >= 0.0) ? 1.0 : -1.0;
+  h[3] = q0 - beta * fabs(q0) * mx;                                   // :289-291
+  h[4] = q1 - beta * fabs(q1) * my;
+  h[5] = q2 - beta * fabs(q2) * mz;
+  double t0 = h[6], t1 = h[7], t2 = h[8];                             // :298-304
+  tdp = t0 * n0 + t1 * n1 + t2 * n2;
+  t0 -= tdp * n0; t1 -= tdp * n1; t2 -= tdp * n2;
+  tmp = k_ben * a.dt;                                                 // :306-310
+  t0 += -wt0 * tmp; t1 += -wt1 * tmp; t2 += -wt2 * tmp;
+  mx = (wt0 >= 0.0) ? 1.0 : -1.0;                                     // :313-318
+  my = (wt1 >= 0.0) ? 1.0 : -1.0;
+  mz = (wt2 >= 0.0) ? 1.0 : -1.0;
+  h[6] = t0 - beta * fabs(t0) * mx;                                   // :320-322
+  h[7] = t1 - beta * fabs(t1) * my;
+  h[8] = t2 - beta * fabs(t2) * mz;
+  F[0] += fd0 + h[0];                                                 // :334-336
+  F[1] += fd1 + h[1];
+  F[2] += fd2 + h[2];
+  T[0] += h[3] + h[6];                                                // :338-340 (the cross
+  T[1] += h[4] + h[7];                                                //  term is cri * 0)
+  T[2] += h[5] + h[8];
+  return false;
+}
+
+__device__ inline bool bonded_to(const StepArgs& a, int i, int nb, int j) {
+  for (int b = 0; b < nb; ++b)
+    if (a.bj[(long)b * a.ld + i] == j) return true;
+  return false;
+}
+
+// u = v_g, omega = 0 for particles of a prescribed-velocity group.
+__device__ inline void apply_vgroup(const StepArgs& a, int i, double* s) {
+  if (!a.vgroup) return;
+  const int g = a.vgroup[i];
+  if (g <= 0 || g > a.n_vgroups || a.vgkind[g - 1] != DEM_GROUP_VELOCITY) return;
+  s[0] = a.vgv[g - 1][0]; s[1] = a.vgv[g - 1][1]; s[2] = a.vgv[g - 1][2];
+  s[3] = 0.; s[4] = 0.; s[5] = 0.;
+}
+
+// F = 0 for particles of a fixed-force group, after the whole force pass
+// (examples/tst_01_BDEM_two_particles_tensile_3D.cpp:57-59, applied between
+// the bond forces and stage3, :306-309).
+__device__ inline void apply_force_bc(const StepArgs& a, int i, double* F) {
+  if (!a.vgroup) return;
+  const int g = a.vgroup[i];
+  if (g <= 0 || g > a.n_vgroups || a.vgkind[g - 1] != DEM_GROUP_FIXED_FORCE) return;
+  F[0] = 0.; F[1] = 0.; F[2] = 0.;
+}
+
+// Initial force of the force pass: m g (the body-force extension), then the
+// external force (examples/tst_01_BDEM_two_particles_tensile_3D.cpp:110-112).
+__device__ inline void initial_force(const StepArgs& a, int i, double m, double* F) {
+  if (a.gravity_on) { F[0] = m * a.g[0]; F[1] = m * a.g[1]; F[2] = m * a.g[2]; }
+  else { F[0] = 0.; F[1] = 0.; F[2] = 0.; }
+  if (a.fext) {
+    F[0] += a.fext[3l * i + 0];
+    F[1] += a.fext[3l * i + 1];
+    F[2] += a.fext[3l * i + 2];
+  }
+}
+
 __device__ inline double wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
   return v;
@@ -224,7 +360,7 @@ __device__ inline void block_reduce8(const double* v, double* part, int blk) {
 
 // KICK: 0 = forces only (write F,T); 1 = + stage3 (write state, F,T);
 //       2 = + stage3 + stage1 + stage2 of the next step (write state).
-template <bool REF, bool PP, bool PW, int KICK, bool ACCUM, bool DIAG>
+template <bool REF, bool PP, bool PW, int KICK, bool ACCUM, bool DIAG, bool BOND>
 __global__ __launch_bounds__(kBlock, DEM_FORCE_MIN_WAVES) void k_force(StepArgs a) {
   extern __shared__ int s_cj[]; // [max_c][kBlock] contacts found in phase 1
   const int tid = threadIdx.x;
@@ -247,13 +383,39 @@ __global__ __launch_bounds__(kBlock, DEM_FORCE_MIN_WAVES) void k_force(StepArgs 
       F[0] = ft[0]; F[1] = ft[1]; F[2] = ft[2];
       T[0] = ft[3]; T[1] = ft[4]; T[2] = ft[5];
     } else {
-      if (a.gravity_on) { F[0] = mI.x * a.g[0]; F[1] = mI.x * a.g[1]; F[2] = mI.x * a.g[2]; }
-      else { F[0] = 0.; F[1] = 0.; F[2] = 0.; }
+      initial_force(a, i, mI.x, F);
       T[0] = 0.; T[1] = 0.; T[2] = 0.;
+    }
+    int nb = 0;
+    if (BOND) {
+      // bond forces first, in slot order (the contacts follow)
+      nb = a.nbond[i];
+      const double2 EGi = a.bEG[i];
+      for (int b = 0; b < nb; ++b) {
+        const int j = a.bj[(long)b * ld + i];
+        if (j < 0) continue;
+        double h[9];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) h[c] = a.bh[(9 * b + c) * ld + i];
+        const double4 xj = a.xr_in[j];
+        double uj[6];
+        load6(a.uw_in + 6l * j, uj);
+        if (bond_force<REF>(xi, ui, EGi, xj, uj, a.bl[(long)b * ld + i], h, F, T, a)) {
+          a.bj[(long)b * ld + i] = -1;
+          atomicAdd(&a.flags->bonds_broken, 1);
+          continue;
+        }
+#pragma unroll
+        for (int c = 0; c < 9; ++c) a.bh[(9 * b + c) * ld + i] = h[c];
+      }
     }
     const int meta = a.cmeta_in[i];
     int nnb = 0, nc = meta_count(meta), nwc = 0;
-    if (PP) {
+    // in a bonded run, contacts only between unbonded pairs (or none at all)
+    const int nbk = a.nbond ? (BOND ? nb : a.nbond[i]) : 0;
+    const bool pp_on = !a.nbond || a.unbonded_contacts;
+    if (PP && !pp_on) nc = 0;
+    if (PP && pp_on) {
       // Phase 1: contact detection over the neighbour row.
       nnb = a.nn[i];
       nc = 0;
@@ -277,7 +439,7 @@ __global__ __launch_bounds__(kBlock, DEM_FORCE_MIN_WAVES) void k_force(StepArgs 
           // r2 < s^2 is necessary for sqrt(r2) < s; the 1e-12 margin covers
           // the rounding of s*s, so no contact is ever filtered out here.
           if (r2 < s * s * (1. + 1e-12)) {
-            if (s - sqrt(r2) > 0.) {
+            if (s - sqrt(r2) > 0. && !(nbk > 0 && bonded_to(a, i, nbk, jb[q]))) {
               if (nc < a.max_c) s_cj[nc * kBlock + tid] = jb[q];
               ++nc;
             }
@@ -345,6 +507,7 @@ __global__ __launch_bounds__(kBlock, DEM_FORCE_MIN_WAVES) void k_force(StepArgs 
     }
     if (PP) a.cmeta_out[i] = make_meta(nc, wn);
     else if (PW) a.cmeta_in[i] = make_meta(nc, wn); // walls only: in place
+    if (!ACCUM) apply_force_bc(a, i, F);
     if (KICK == 0) {
       double ft[6] = {F[0], F[1], F[2], T[0], T[1], T[2]};
       store6(a.ft + 6l * i, ft);
@@ -364,6 +527,7 @@ __global__ __launch_bounds__(kBlock, DEM_FORCE_MIN_WAVES) void k_force(StepArgs 
       s[3] = ui[3] + a.hdt * T[0] * Iinv;
       s[4] = ui[4] + a.hdt * T[1] * Iinv;
       s[5] = ui[5] + a.hdt * T[2] * Iinv;
+      apply_vgroup(a, i, s);
       if (DIAG) {
         dg[0] = 0.5 * mI.x * (s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
         dg[1] = 0.5 * mI.y * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]);
@@ -373,6 +537,7 @@ __global__ __launch_bounds__(kBlock, DEM_FORCE_MIN_WAVES) void k_force(StepArgs 
         // stage1 of the next step (:42-51), then stage2 (:67-69)
         for (int d = 0; d < 3; ++d) s[d] = s[d] + a.hdt * F[d] * minv;
         for (int d = 0; d < 3; ++d) s[3 + d] = s[3 + d] + a.hdt * T[d] * Iinv;
+        apply_vgroup(a, i, s);
         double4 xo;
         xo.x = xi.x + a.dt * s[0];
         xo.y = xi.y + a.dt * s[1];
@@ -394,13 +559,14 @@ __global__ __launch_bounds__(kBlock, DEM_FORCE_MIN_WAVES) void k_force(StepArgs 
   if (DIAG) block_reduce8(dg, a.diag_part, blk);
 }
 
-// F = m g, T = 0  (src/CabanaDEM_Force.hpp:39-47 plus the body force).
+// F = m g + f_ext, T = 0  (src/CabanaDEM_Force.hpp:39-47 plus the body and
+// external forces).
 __global__ __launch_bounds__(kBlock) void k_zero_forces(StepArgs a) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= a.n) return;
   const double2 mI = a.mI[i];
   double ft[6] = {0., 0., 0., 0., 0., 0.};
-  if (a.gravity_on) { ft[0] = mI.x * a.g[0]; ft[1] = mI.x * a.g[1]; ft[2] = mI.x * a.g[2]; }
+  initial_force(a, i, mI.x, ft);
   store6(a.ft + 6l * i, ft);
 }
 
@@ -443,6 +609,7 @@ __global__ __launch_bounds__(kBlock) void k_kick(StepArgs a, double4* xr, double
     const double minv = 1. / mI.x, Iinv = 1. / mI.y; // src/CabanaDEM_Integrator.hpp:42,48
     for (int d = 0; d < 3; ++d) s[d] = s[d] + a.hdt * ft[d] * minv;
     for (int d = 0; d < 3; ++d) s[3 + d] = s[3 + d] + a.hdt * ft[3 + d] * Iinv;
+    apply_vgroup(a, i, s);
     store6(uw + 6l * i, s);
   }
   if (do_drift) {
@@ -491,28 +658,31 @@ __global__ __launch_bounds__(kBlock) void k_diag_final(const double* part, int n
 
 inline unsigned nblk(long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
-template <bool REF>
+template <bool REF, bool BOND>
 hipError_t launch_force_mode(ForceKind kind, const StepArgs& a, bool sample, hipStream_t s) {
   const size_t lds = (size_t)(a.max_c > 0 ? a.max_c : 1) * kBlock * sizeof(int);
   const dim3 g(nblk(a.n)), b(kBlock);
   switch (kind) {
     case FK_PP_PW_FORCES:
-      if (sample) hipLaunchKernelGGL((k_force<REF, true, true, 0, false, true>), g, b, lds, s, a);
-      else hipLaunchKernelGGL((k_force<REF, true, true, 0, false, false>), g, b, lds, s, a);
+      if (sample) hipLaunchKernelGGL((k_force<REF, true, true, 0, false, true, BOND>), g, b, lds, s, a);
+      else hipLaunchKernelGGL((k_force<REF, true, true, 0, false, false, BOND>), g, b, lds, s, a);
       break;
     case FK_PP_ACCUM:
-      hipLaunchKernelGGL((k_force<REF, true, false, 0, true, false>), g, b, lds, s, a);
+      hipLaunchKernelGGL((k_force<REF, true, false, 0, true, false, false>), g, b, lds, s, a);
       break;
     case FK_PW_ACCUM:
-      hipLaunchKernelGGL((k_force<REF, false, true, 0, true, false>), g, b, lds, s, a);
+      hipLaunchKernelGGL((k_force<REF, false, true, 0, true, false, false>), g, b, lds, s, a);
+      break;
+    case FK_BOND_ACCUM:
+      hipLaunchKernelGGL((k_force<REF, false, false, 0, true, false, true>), g, b, lds, s, a);
       break;
     case FK_STEP_FUSED:
-      if (sample) hipLaunchKernelGGL((k_force<REF, true, true, 2, false, true>), g, b, lds, s, a);
-      else hipLaunchKernelGGL((k_force<REF, true, true, 2, false, false>), g, b, lds, s, a);
+      if (sample) hipLaunchKernelGGL((k_force<REF, true, true, 2, false, true, BOND>), g, b, lds, s, a);
+      else hipLaunchKernelGGL((k_force<REF, true, true, 2, false, false, BOND>), g, b, lds, s, a);
       break;
     case FK_STEP_LAST:
-      if (sample) hipLaunchKernelGGL((k_force<REF, true, true, 1, false, true>), g, b, lds, s, a);
-      else hipLaunchKernelGGL((k_force<REF, true, true, 1, false, false>), g, b, lds, s, a);
+      if (sample) hipLaunchKernelGGL((k_force<REF, true, true, 1, false, true, BOND>), g, b, lds, s, a);
+      else hipLaunchKernelGGL((k_force<REF, true, true, 1, false, false, BOND>), g, b, lds, s, a);
       break;
   }
   return hipGetLastError();
@@ -523,8 +693,11 @@ hipError_t launch_force_mode(ForceKind kind, const StepArgs& a, bool sample, hip
 int force_blocks(int n) { return (int)nblk(n); }
 
 hipError_t launch_force(ForceKind kind, const StepArgs& a, bool sample, hipStream_t s) {
-  if (a.physics_mode == DEM_MODE_REFERENCE) return launch_force_mode<true>(kind, a, sample, s);
-  return launch_force_mode<false>(kind, a, sample, s);
+  const bool ref = a.physics_mode == DEM_MODE_REFERENCE;
+  if (a.nbond) return ref ? launch_force_mode<true, true>(kind, a, sample, s)
+                          : launch_force_mode<false, true>(kind, a, sample, s);
+  return ref ? launch_force_mode<true, false>(kind, a, sample, s)
+             : launch_force_mode<false, false>(kind, a, sample, s);
 }
 
 hipError_t launch_zero_forces(const StepArgs& a, hipStream_t s) {

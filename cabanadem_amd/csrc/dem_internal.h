@@ -80,7 +80,7 @@ struct DevFlags {
   int skin;
   int bond_overflow;
   int cell_overflow;  // a cell held more than kMaxCellParticles particles
-  int pad;
+  int bonds_broken;   // bonds broken since the flags were last cleared
 };
 
 // Upper bound on particles in one cell; beyond it the binning box is far too
@@ -121,6 +121,24 @@ struct StepArgs {
   double* ch_out;
   DevFlags* flags;
   double* diag_part; // [blocks][8] when sampling
+
+  // bonded DEM (src/CabanaBondedDEM_Force.hpp:52-348), slot-indexed
+  int max_b;
+  const int* nbond;      // bond slots in use per particle
+  int* bj;               // [max_b][ld] partner slot, -1 once broken
+  const double* bl;      // [max_b][ld] rest length
+  double* bh;            // [9 max_b][ld] ft, mn, mt history per slot
+  const double2* bEG;    // E, G per particle (bond stiffness)
+  double beta_bond, bond_radius_factor, tensile_strength;
+  int unbonded_contacts; // 0: no pp contacts at all in a bonded run
+  // particle groups: vgroup[i] = g > 0 selects group g-1 of kind
+  //   DEM_GROUP_VELOCITY: u = vgv[g-1], omega = 0 after every half kick
+  //   DEM_GROUP_FIXED_FORCE: F = 0 after the force pass (torque kept)
+  const int* vgroup;
+  int n_vgroups;
+  int vgkind[DEM_MAX_GROUPS];
+  double vgv[DEM_MAX_GROUPS][3];
+  const double* fext;    // [n][3] constant external force, or null
 };
 
 struct BuildArgs {
@@ -155,7 +173,31 @@ struct BuildArgs {
   int* nn;
   int* nbr;
   DevFlags* flags;
+  // bonded DEM state (permuted with the particles when bonds_on)
+  int bonds_on, max_b;
+  const int* nbond_in; int* nbond_out;
+  const int* bj_in; int* bj_out;
+  const double* bl_in; double* bl_out;
+  const double* bh_in; double* bh_out;
+  const double2* bEG_in; double2* bEG_out;
+  const int* vgroup_in; int* vgroup_out;
+  const double* fext_in; double* fext_out;
 };
+
+// Bond set-up over the current neighbour rows: every j with r_ij < factor r_i.
+struct BondSetupArgs {
+  int n, ld, max_b;
+  double factor;
+  const double4* xr;
+  const int* nn;
+  const int* nbr;
+  int* nbond;
+  int* bj;
+  double* bl;
+  double* bh;
+  DevFlags* flags;
+};
+hipError_t launch_setup_bonds(const BondSetupArgs& a, hipStream_t s);
 
 // Launchers (defined in dem_build.hip / dem_force.hip); all enqueue on `s`.
 hipError_t launch_rebuild(const BuildArgs& a, hipStream_t s);
@@ -167,7 +209,8 @@ enum ForceKind {
   FK_PP_ACCUM = 1,     // F,T += pp (contact list updated)
   FK_PW_ACCUM = 2,     // F,T += pw
   FK_STEP_FUSED = 3,   // forces + stage3 + stage1 + stage2 of the next step
-  FK_STEP_LAST = 4     // forces + stage3, write F,T
+  FK_STEP_LAST = 4,    // forces + stage3, write F,T
+  FK_BOND_ACCUM = 5    // F,T += bond forces
 };
 hipError_t launch_force(ForceKind kind, const StepArgs& a, bool sample, hipStream_t s);
 hipError_t launch_zero_forces(const StepArgs& a, hipStream_t s);

@@ -33,7 +33,9 @@ extern "C" {
 #endif
 
 #define DEM_MAX_WALLS 8
-#define DEM_ABI_VERSION 1
+#define DEM_MAX_GROUPS 8
+#define DEM_MAX_BONDS 32
+#define DEM_ABI_VERSION 2
 
 enum dem_status {
   DEM_OK = 0,
@@ -48,6 +50,14 @@ enum dem_status {
 };
 
 enum dem_physics_mode { DEM_MODE_REFERENCE = 0, DEM_MODE_CORRECTED = 1 };
+
+/* Kinds of particle group (dem_set_groups). */
+enum dem_group_kind {
+  DEM_GROUP_VELOCITY = 0,   /* u = v_g and omega = 0 after every half kick (pulled faces) */
+  DEM_GROUP_FIXED_FORCE = 1 /* F = 0 after the force pass, torque kept
+                             * (applyBoundaryConditions,
+                             * examples/tst_01_BDEM_two_particles_tensile_3D.cpp:18-66) */
+};
 
 typedef struct dem_ctx dem_ctx;
 
@@ -101,6 +111,39 @@ typedef struct dem_walls_view {
   const double *E, *nu, *G; /* [n] */
 } dem_walls_view;
 
+/* Bond parameters of a bonded system.  The reference hard-codes them in
+ * computeBondedForce (src/CabanaBondedDEM_Force.hpp:186-188: beta 0.1,
+ * bond radius factor 1) and in ParticlesBondedDEM's MaxBonds template
+ * parameter (src/CabanaBondedDEM_Particles.hpp:29); breakage and contacts
+ * between unbonded pairs are extensions. */
+typedef struct dem_bond_config {
+  int max_bonds;           /* bond slots per particle (MaxBonds), <= DEM_MAX_BONDS */
+  int unbonded_contacts;   /* 1: Hertz contacts between overlapping unbonded pairs;
+                            * 0: no pp contacts in the bonded system (the reference) */
+  double beta_bond;        /* non-viscous damping coefficient (0.1) */
+  double radius_factor;    /* bond radius = min(r_i, r_j) * radius_factor (1.0) */
+  double tensile_strength; /* > 0: a bond breaks once E_i (r - L0) / (r_i + r_j)
+                            * exceeds it; <= 0: unbreakable (the reference) */
+} dem_bond_config;
+
+/* Host view of the bond state in the reference layout (bond fields of
+ * src/CabanaBondedDEM_Particles.hpp:40-47,187-201), all rows [n][max_bonds].  Slot k
+ * of particle i holds partner idx (original id; -1 = empty or broken), its
+ * rest length and the history of that bond.  The reference indexes the
+ * history and rest length by the partner id j instead of the slot k
+ * (src/CabanaBondedDEM_Force.hpp:207,231-233); this library uses the slot.
+ * NULL pointers are skipped. */
+typedef struct dem_bonds_view {
+  int n;
+  int max_bonds;
+  int *idx;                      /* bond_idx */
+  int *count;                    /* [n] total_no_bonds */
+  double *init_len;              /* bond_init_len */
+  double *ft_x, *ft_y, *ft_z;    /* tangential force history */
+  double *mn_x, *mn_y, *mn_z;    /* normal (torsion) moment history */
+  double *mt_x, *mt_y, *mt_z;    /* tangential (bending) moment history */
+} dem_bonds_view;
+
 /* Run-average counters (extension; the reference reports none). */
 typedef struct dem_counters {
   int64_t steps;          /* steps taken by this context */
@@ -110,6 +153,7 @@ typedef struct dem_counters {
   double mean_neighbors;  /* n-bar at the last sampled step */
   double mean_contacts;   /* c-bar (tracked pp contacts) at the last sampled step */
   double mean_wall_contacts; /* w-bar at the last sampled step */
+  int64_t bonds_broken;   /* bonds broken so far (tensile_strength > 0) */
 } dem_counters;
 
 /* ---- lifecycle ---------------------------------------------------------- */
@@ -149,8 +193,9 @@ int dem_build_neighbors(dem_ctx* ctx);
 int dem_get_neighbors(dem_ctx* ctx, int* counts, int* ids, int max_n);
 
 /* ---- force stages ---------------------------------------------------------- */
-/* Replaces: Force::makeForceTorqueZeroOnParticle (src/CabanaDEM_Force.hpp:39-47);
- * F starts at m*gravity. */
+/* Replaces: Force::makeForceTorqueZeroOnParticle (src/CabanaDEM_Force.hpp:39-47)
+ * and resetForcesAndTorques (src/CabanaBondedDEM_Force.hpp:15-50);
+ * F starts at m*gravity plus the external force, if one is set. */
 int dem_zero_forces(dem_ctx* ctx);
 /* Replaces: Force::update_tangential_contacts (src/CabanaDEM_Force.hpp:343-445). */
 int dem_update_tangential(dem_ctx* ctx);
@@ -183,6 +228,39 @@ int dem_get_diag_history(dem_ctx* ctx, double* rows, int cap, int* nrows);
 int dem_diagnostics(dem_ctx* ctx, double out[5]);
 int dem_get_counters(dem_ctx* ctx, dem_counters* out);
 int dem_synchronize(dem_ctx* ctx);
+
+/* ---- bonded DEM ---------------------------------------------------------- */
+/* Turns the context into a bonded system (allocates max_bonds slots per
+ * particle; no bonds yet).  Replaces: instantiating ParticlesBondedDEM
+ * (src/CabanaBondedDEM_Particles.hpp:29-83).  May be called again to change
+ * the scalar parameters; max_bonds must then stay the same. */
+int dem_enable_bonds(dem_ctx* ctx, const dem_bond_config* cfg);
+/* Bonds every particle i to each neighbour j with |x_i - x_j| < factor r_i,
+ * rest length |x_i - x_j|, zero history, over the context's neighbour list
+ * (rebuilt first if stale).  factor * max r must not exceed the cutoff.
+ * Replaces: ParticlesBondedDEM::setupBonds (src/CabanaBondedDEM_Particles.hpp:85-171;
+ * the reference hard-codes factor 3 and a 4 r_0 search radius). */
+int dem_setup_bonds(dem_ctx* ctx, double factor);
+/* Bond state in and out, in the reference layout and original ids. */
+int dem_upload_bonds(dem_ctx* ctx, const dem_bonds_view* v);
+int dem_download_bonds(dem_ctx* ctx, dem_bonds_view* v);
+/* F, T += bond forces and moments; updates the bond histories and, with a
+ * tensile strength, breaks bonds.  Replaces: computeBondedForce
+ * (src/CabanaBondedDEM_Force.hpp:52-348).  In a bonded system dem_step,
+ * dem_compute_forces and dem_force_pp include the bonds (and, with
+ * unbonded_contacts, restrict pp contacts to unbonded pairs). */
+int dem_bond_forces(dem_ctx* ctx);
+/* Constant per-particle external force, added where the force pass resets F
+ * (F = m g + f_ext).  f = [n][3] in original order; NULL removes it.
+ * Replaces: applyTensileForce (examples/tst_01_BDEM_two_particles_tensile_3D.cpp:69-117). */
+int dem_set_external_force(dem_ctx* ctx, const double* f);
+/* Particle groups: group[i] in 0..ngroups (0 = none), original order;
+ * kind[g] = DEM_GROUP_*, vel[g][3] the prescribed velocity of a
+ * DEM_GROUP_VELOCITY group.  group = NULL removes all groups.
+ * Replaces: applyBoundaryConditions (examples/tst_01_...cpp:18-66) and the
+ * pulled end faces of a tensile test. */
+int dem_set_groups(dem_ctx* ctx, const int* group, int ngroups, const int* kind,
+                   const double* vel);
 
 /* ---- kernel timing (HIP events on the context's stream) ------------------ */
 /* Enables per-launch HIP-event timing of the kernels dem_step launches. */

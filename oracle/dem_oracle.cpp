@@ -226,8 +226,40 @@ void oracle_update_tangential(OracleParticles* p) {
 // tracking by neighbour id.  Accumulates into F_i (and, in corrected mode,
 // T_i).  Returns 1 if any particle needed more than max_cnts tracked
 // contacts (the reference writes out of bounds there; the oracle refuses).
+static bool bonded_to(const OracleBonds* b, int i, int j) {
+  if (!b) return false;
+  for (int k = 0; k < b->count[i]; ++k)
+    if (b->idx[(long)i * b->max_bonds + k] == j) return true;
+  return false;
+}
+
+static int force_pp_impl(OracleParticles* p, const int* counts, const int* ids, int max_n,
+                         const OracleParams* prm, const OracleBonds* skip);
+
 int oracle_force_pp(OracleParticles* p, const int* counts, const int* ids,
                     int max_n, const OracleParams* prm) {
+  return force_pp_impl(p, counts, ids, max_n, prm, nullptr);
+}
+
+int oracle_force_pp_unbonded(OracleParticles* p, const int* counts, const int* ids, int max_n,
+                             const OracleParams* prm, const OracleBonds* b) {
+  return force_pp_impl(p, counts, ids, max_n, prm, b);
+}
+
+void oracle_velocity_groups(OracleParticles* p, const int* group, const double* vel, int ngroups) {
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < p->n; ++i) {
+    const int g = group[i];
+    if (g <= 0 || g > ngroups) continue;
+    for (int d = 0; d < 3; ++d) {
+      p->u[3 * i + d] = vel[3 * (g - 1) + d];
+      p->omega[3 * i + d] = 0.;
+    }
+  }
+}
+
+static int force_pp_impl(OracleParticles* p, const int* counts, const int* ids, int max_n,
+                         const OracleParams* prm, const OracleBonds* skip) {
   const int M = p->max_cnts;
   const bool ref = prm->mode == ORACLE_MODE_REFERENCE;
   const double dt = prm->dt;
@@ -274,6 +306,7 @@ int oracle_force_pp(OracleParticles* p, const int* counts, const int* ids,
       double vn0 = vdn * nx, vn1 = vdn * ny, vn2 = vdn * nz;   // :147-149
       double vt0 = v0 - vn0, vt1 = v1 - vn1, vt2 = v2 - vn2;   // :152-154
       if (!(overlap > 0.)) continue;                           // :162
+      if (bonded_to(skip, i, j)) continue;                     // bonded pairs: bond force only
       // :166-192 find j among the tracked ids (last match wins), else append
       int found_at = -1;
       for (int k = 0; k < p->ss_cnt[i]; ++k)
@@ -595,7 +628,7 @@ int oracle_setup_bonds(int n, const double* x, const double* rad, double factor,
 void oracle_bond_force(OracleParticles* p, OracleBonds* b, const OracleBondParams* bp,
                        double dt, int* n_broken) {
   const int MB = b->max_bonds;
-  const bool ref = true;
+  const bool ref = bp->mode == ORACLE_MODE_REFERENCE;
   const double* x = p->x;
   const double* u = p->u;
   const double* w = p->omega;
@@ -657,9 +690,12 @@ void oracle_bond_force(OracleParticles* p, OracleBonds* b, const OracleBondParam
       double mx = (vn0 >= 0.0) ? 1.0 : -1.0;                   // :216-218
       double my = (vn1 >= 0.0) ? 1.0 : -1.0;
       double mz = (vn2 >= 0.0) ? 1.0 : -1.0;
-      double fd0 = fn0 + beta_bond * std::fabs(fn0) * mx;      // :220-222
-      double fd1 = fn1 + beta_bond * std::fabs(fn1) * my;
-      double fd2 = fn2 + beta_bond * std::fabs(fn2) * mz;
+      // :220-222 add +beta|fn| sign(v_n), feeding energy in (quirk Q15);
+      // corrected mode subtracts it, as the tangential terms do (:258-260)
+      const double bs = ref ? beta_bond : -beta_bond;
+      double fd0 = fn0 + bs * std::fabs(fn0) * mx;
+      double fd1 = fn1 + bs * std::fabs(fn1) * my;
+      double fd2 = fn2 + bs * std::fabs(fn2) * mz;
       double* ft = b->ft + 3 * s;                              // :231-233
       double f0 = ft[0], f1 = ft[1], f2 = ft[2];
       double fdp = f0 * n0 + f1 * n1 + f2 * n2;                // :235-239

@@ -110,6 +110,7 @@ typedef struct OracleBondParams {
   double radius_factor;   /* 1.0 in the reference */
   double tensile_strength;/* <= 0: unbreakable (the reference has no breakage) */
   int contacts;           /* 1: Hertz contacts between unbonded overlapping pairs */
+  int mode;               /* ORACLE_MODE_* (Q1 in the bond relative velocity) */
 } OracleBondParams;
 
 int oracle_setup_bonds(int n, const double *x, const double *rad, double factor,
@@ -117,6 +118,11 @@ int oracle_setup_bonds(int n, const double *x, const double *rad, double factor,
                        OracleBonds *b);
 void oracle_bond_force(OracleParticles *p, OracleBonds *b,
                        const OracleBondParams *bp, double dt, int *n_broken);
+/* oracle_force_pp restricted to pairs that are not joined by an intact bond. */
+int oracle_force_pp_unbonded(OracleParticles *p, const int *counts, const int *ids,
+                             int max_n, const OracleParams *prm, const OracleBonds *b);
+/* Velocity-prescribed particles: group[i] > 0 sets u_i = vel[group-1], omega_i = 0. */
+void oracle_velocity_groups(OracleParticles *p, const int *group, const double *vel, int ngroups);
 
 #ifdef __cplusplus
 }

@@ -58,7 +58,7 @@ class OracleBonds(C.Structure):
 
 class OracleBondParams(C.Structure):
     _fields_ = [("beta_bond", C.c_double), ("radius_factor", C.c_double),
-                ("tensile_strength", C.c_double), ("contacts", C.c_int)]
+                ("tensile_strength", C.c_double), ("contacts", C.c_int), ("mode", C.c_int)]
 
 
 def build():
@@ -94,6 +94,10 @@ def lib():
         L.oracle_setup_bonds.restype = C.c_int
         L.oracle_bond_force.argtypes = [C.POINTER(OracleParticles), C.POINTER(OracleBonds),
                                         C.POINTER(OracleBondParams), C.c_double, _ip]
+        L.oracle_force_pp_unbonded.argtypes = [C.POINTER(OracleParticles), _ip, _ip, C.c_int,
+                                               C.POINTER(OracleParams), C.POINTER(OracleBonds)]
+        L.oracle_force_pp_unbonded.restype = C.c_int
+        L.oracle_velocity_groups.argtypes = [C.POINTER(OracleParticles), _ip, _dp, C.c_int]
         _L = L
     return _L
 
@@ -229,3 +233,73 @@ def run(st, w, prm, *, nsteps, rc, lo, hi, rebuild_every=1, max_n=64, diag_every
     if rc_ != 0:
         raise RuntimeError(f"oracle_run failed with {rc_}")
     return diag[: nd.value], secs.value
+
+
+class Bonds:
+    """Bond state in the reference layout: idx/init_len [n][max_bonds], ft/mn/mt
+    [n][max_bonds][3], count [n] (src/CabanaBondedDEM_Particles.hpp:40-47,187-201)."""
+
+    def __init__(self, n, max_bonds):
+        self.n, self.max_bonds = n, max_bonds
+        self.idx = -np.ones((n, max_bonds), dtype=np.int32)
+        self.init_len = np.zeros((n, max_bonds))
+        self.ft = np.zeros((n, max_bonds, 3))
+        self.mn = np.zeros((n, max_bonds, 3))
+        self.mt = np.zeros((n, max_bonds, 3))
+        self.count = np.zeros(n, dtype=np.int32)
+
+    def copy(self):
+        c = Bonds(self.n, self.max_bonds)
+        for f in ("idx", "init_len", "ft", "mn", "mt", "count"):
+            setattr(c, f, getattr(self, f).copy())
+        return c
+
+    def struct(self):
+        b = OracleBonds()
+        b.max_bonds = self.max_bonds
+        b.idx = _p(self.idx, C.c_int)
+        b.init_len = _p(self.init_len)
+        b.ft, b.mn, b.mt = _p(self.ft), _p(self.mn), _p(self.mt)
+        b.count = _p(self.count, C.c_int)
+        b._keep = self
+        return b
+
+
+def bond_params(beta=0.1, radius_factor=1.0, strength=0.0, contacts=1, mode=0):
+    bp = OracleBondParams()
+    bp.beta_bond, bp.radius_factor, bp.tensile_strength = beta, radius_factor, strength
+    bp.contacts, bp.mode = contacts, mode
+    return bp
+
+
+def setup_bonds(st, factor, rc, lo, hi, max_bonds):
+    b = Bonds(st.n, max_bonds)
+    x = np.ascontiguousarray(st.x, dtype=np.float64)
+    rad = np.ascontiguousarray(st.rad, dtype=np.float64)
+    lo3, hi3 = np.asarray(lo, dtype=np.float64), np.asarray(hi, dtype=np.float64)
+    rc_ = lib().oracle_setup_bonds(st.n, _p(x), _p(rad), factor, rc, _p(lo3), _p(hi3),
+                                   C.byref(b.struct()))
+    if rc_ != 0:
+        raise RuntimeError(f"oracle_setup_bonds failed with {rc_}")
+    return b
+
+
+def bond_force(st, bonds, bp, dt):
+    nb = C.c_int(0)
+    lib().oracle_bond_force(C.byref(particles(st)), C.byref(bonds.struct()), C.byref(bp), dt,
+                            C.byref(nb))
+    return nb.value
+
+
+def force_pp_unbonded(st, counts, ids, prm, bonds):
+    counts = np.ascontiguousarray(counts, dtype=np.int32)
+    ids = np.ascontiguousarray(ids, dtype=np.int32)
+    return lib().oracle_force_pp_unbonded(C.byref(particles(st)), _p(counts, C.c_int),
+                                          _p(ids, C.c_int), ids.shape[1], C.byref(prm),
+                                          C.byref(bonds.struct()))
+
+
+def velocity_groups(st, group, vel):
+    group = np.ascontiguousarray(group, dtype=np.int32)
+    vel = np.ascontiguousarray(np.asarray(vel, dtype=np.float64).reshape(-1, 3))
+    lib().oracle_velocity_groups(C.byref(particles(st)), _p(group, C.c_int), _p(vel), vel.shape[0])

@@ -152,3 +152,42 @@ def box_walls(lo, hi, E, nu, G):
     x = [(lo[0], 0, 0), (hi[0], 0, 0), (0, lo[1], 0), (0, hi[1], 0), (0, 0, lo[2]), (0, 0, hi[2])]
     nrm = [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)]
     return Walls(x, nrm, E, nu, G)
+
+
+def tst01_bonded(max_bonds: int = 27):
+    """Two bonded spheres of examples/tst_01_BDEM_two_particles_tensile_3D.cpp:128-232:
+    r = 0.5, rho = 2000, E = 1e4, G = 1e3, nu = 0.25, at x = -r and +r,
+    binning box +-4 r, dt = 1e-4, 4 s; external force -+1e3 N along x on
+    particles 0 and 1 (:287-291); particle 0's force zeroed (:57-59)."""
+    r = 0.5
+    st = ParticleState(2, max_cnts=6, max_walls=1)
+    for i in range(2):
+        _sphere(st, i, r, 2000., 1e4, 1e3, 0.25)
+        st.x[i] = (-r + i * 2. * r, 0., 0.)
+    fext = np.zeros((2, 3))
+    fext[0, 0], fext[1, 0] = -1e3, 1e3
+    return dict(st=st, r=r, dt=1e-4, final_time=4., lo=(-4. * r,) * 3, hi=(4. * r,) * 3,
+                fext=fext, group=np.array([1, 0], dtype=np.int32), kinds=[1], vel=None,
+                max_bonds=max_bonds, factor=3.0, cutoff=4. * r)
+
+
+def bonded_block(nx: int, ny: int, nz: int, *, spacing=1e-3, r=0.5e-3, E=1e9, nu=0.25, rho=2600.,
+                 jitter=0.0, seed=0, max_cnts=8):
+    """Simple-cubic block of touching spheres (the bonded tensile configuration's
+    shape: spacing 2r, bonds to the six face neighbours at factor 2.1)."""
+    n = nx * ny * nz
+    st = ParticleState(n, max_cnts=max_cnts, max_walls=1)
+    rng = np.random.default_rng(seed)
+    G = E / (2. * (1. + nu))
+    k = 0
+    for iz in range(nz):
+        for iy in range(ny):
+            for ix in range(nx):
+                _sphere(st, k, r, rho, E, G, nu)
+                st.x[k] = (ix * spacing, iy * spacing, iz * spacing)
+                k += 1
+    if jitter > 0:
+        st.x += rng.uniform(-jitter, jitter, size=st.x.shape)
+    lo = np.array([-2. * spacing] * 3)
+    hi = np.array([(nx + 1) * spacing, (ny + 1) * spacing, (nz + 1) * spacing])
+    return st, lo, hi
