@@ -62,7 +62,10 @@ def build_oracle() -> Path:
     return ORACLE_DIR / "_build" / "libdem_oracle.so"
 
 
-EXAMPLES = {"dem_validation": "examples/dem_validation.cpp"}
+EXAMPLES = {
+    "dem_validation": "examples/dem_validation.cpp",
+    "bonded_tensile": "examples/bonded_tensile.cpp",
+}
 
 
 def build_examples() -> list:

@@ -12,4 +12,9 @@
 #include "CabanaDEM_Integrator.hpp"
 #include "CabanaDEM_Solver.hpp"
 
+#include "CabanaBondedDEM_Particles.hpp"
+#include "CabanaBondedDEM_Force.hpp"
+#include "CabanaBondedDEM_Solver.hpp"
+#include "CabanaBondedDEM_Integrator.hpp"
+
 #endif

@@ -138,6 +138,11 @@ struct DeviceBinding {
   std::function<int()> max_walls;
   std::function<int()> count;
   Mirror* mirror = nullptr;
+  // Optional state beyond the particle view (bonded containers): run on a
+  // new context before its first upload, and beside every upload/download.
+  std::function<void()> after_create;
+  std::function<void()> upload_extra;
+  std::function<void()> download_extra;
 
   DeviceBinding() {
     cfg.max_contacts = 16;
@@ -160,11 +165,13 @@ struct DeviceBinding {
     dem_particles_view v{};
     fill_view(v);
     check(dem_download_particles(ctx, &v), ctx);
+    if (download_extra) download_extra();
   }
   void upload() {
     dem_particles_view v{};
     fill_view(v);
     check(dem_upload_particles(ctx, &v), ctx);
+    if (upload_extra) upload_extra();
   }
 
   static bool structural_equal(const dem_config& a, const dem_config& b) {
@@ -196,6 +203,7 @@ struct DeviceBinding {
       check(dem_create(&want, &ctx), nullptr);
       cfg = want;
       ++generation;
+      if (after_create) after_create();
       if (mirror) mirror->host_touched = true;
     } else {
       check(dem_set_params(ctx, &want), ctx);

@@ -31,7 +31,7 @@ def run_driver(case, json_name, out_dir, extra=None):
 
 def state_arrays(res):
     ps = res["particles"]
-    return {k: np.array([p[k] for p in ps]) for k in ("x", "u", "omega", "force")}
+    return {k: np.array([p[k] for p in ps]) for k in ("x", "u", "omega", "force") if k in ps[0]}
 
 
 def compare_to_oracle(res, st, tol=1e-9):
@@ -88,3 +88,34 @@ def test_one_hits_two_matches_oracle(gpu_lib, tmp_path):
     g = state_arrays(res)
     p = (st.m[:, None] * g["u"]).sum(0)
     assert abs(p[0] - st.m[0] * inp["velocity_p"]) < 1e-12 * st.m[0] * inp["velocity_p"]
+
+
+BONDED_EXE = ROOT / "examples" / "_build" / "bonded_tensile"
+
+
+@pytest.mark.parametrize("mode,final_time", [("staged", 0.5), ("solver", 4.0)])
+def test_bonded_tensile_dropin_matches_oracle(gpu_lib, tmp_path, mode, final_time):
+    """examples/tst_01_BDEM_two_particles_tensile_3D.cpp through the drop-in
+    bonded API (examples/bonded_tensile.cpp): the reference's staged loop with
+    host-side force edits, and the fused solver with the same physics set once."""
+    import oracle_py as O
+    from bonded_driver import bonded_run
+
+    assert BONDED_EXE.exists(), "build() compiles examples/_build/bonded_tensile"
+    cmd = [str(BONDED_EXE), mode, repr(final_time), str(tmp_path)]
+    print("driver:", " ".join(cmd), flush=True)
+    p = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+    res = json.loads(p.stdout.strip().splitlines()[-1])
+    sc = S.tst01_bonded()
+    st = sc["st"]
+    nsteps = int(final_time / sc["dt"]) + 1
+    assert res["num_steps"] + 1 == nsteps
+    ob = O.setup_bonds(st, sc["factor"], sc["cutoff"], sc["lo"], sc["hi"], sc["max_bonds"])
+    bonded_run(st, ob, nsteps=nsteps, prm=O.params(dt=sc["dt"]), bp=O.bond_params(beta=0.1),
+               dt=sc["dt"], fext=sc["fext"], group=sc["group"], kinds=sc["kinds"])
+    g = state_arrays(res)
+    assert normwise(g["x"], st.x) <= 1e-10
+    assert normwise(g["u"], st.u) <= 1e-10
+    assert [q["bonds"] for q in res["particles"]] == [1, 1]
+    assert (tmp_path / "particles_0.csv").exists()
