@@ -78,21 +78,216 @@ def cpu_baseline(dims, steps_cadence, steps_faithful):
     }
 
 
+def cpu_baseline_bonded(dims, steps, rebuild_every):
+    """The CPU oracle's bonded step loop (OpenMP kernels driven from Python)
+    on the same bonded block, from its set-up state."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle_py as O
+    from oracle_bonded import bonded_run
+    from cabanadem_amd import workloads as W
+
+    wl = W.bonded_block(dims=dims)
+    st = wl.state
+    b = O.setup_bonds(st, wl.bond_factor, wl.cutoff, wl.lo, wl.hi, wl.max_bonds)
+    prm = O.params(dt=wl.dt, cor_pp=wl.cor, cor_pw=wl.cor, fr_pp=wl.friction, fr_pw=wl.friction)
+    bp = O.bond_params(beta=0.1, strength=wl.tensile_strength)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    t0 = time.perf_counter()
+    bonded_run(st, b, nsteps=steps, prm=prm, bp=bp, dt=wl.dt, group=wl.group, kinds=[0, 0],
+               vel=wl.group_vel, contacts=True, rc=wl.cutoff, lo=wl.lo, hi=wl.hi,
+               rebuild_every=rebuild_every)
+    t = time.perf_counter() - t0
+    return {
+        "value": st.n * steps / t, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+        "sample": f"{st.n} bonded spheres from the set-up state, {steps} steps, neighbour list "
+                  f"rebuilt every {rebuild_every} steps ({t:.1f} s)",
+    }
+
+
+def run_bonded(args):
+    """Bonded tensile block: particle-steps/s of the fused bonded step."""
+    from cabanadem_amd import workloads as W
+
+    dims = tuple(int(v) for v in (args.dims or "100,100,100").split(","))
+    t0 = time.perf_counter()
+    wl = W.bonded_block(dims=dims)
+    n = wl.state.n
+    sysm = wl.system(mode=args.mode)
+    bonds0 = int(sysm.download_bonds().count.sum())
+    log(f"[bench] {n} bonded spheres, {bonds0} bonds, set up in {time.perf_counter() - t0:.1f} s")
+    sysm.set_diag_every(1000)
+    if args.warmup > 0:
+        sysm.step(args.warmup)
+    nrows0 = sysm.diag_history().shape[0]
+    sysm.set_timing(True)
+    sysm.sync()
+    t0 = time.perf_counter()
+    sysm.step(args.steps)
+    sysm.sync()
+    elapsed = time.perf_counter() - t0
+    ms, launches = sysm.timing()
+    sysm.set_timing(False)
+    diag = sysm.diag_history()[nrows0:]
+    nbar, cbar = float(diag[:, 6].mean()), float(diag[:, 7].mean())
+    broken = int(sysm.counters().bonds_broken)
+    bbar = (bonds0 - broken) / n
+    value = n * args.steps / elapsed
+    k_ms = ms[0] / max(launches[0], 1)
+    b_alg = W.bytes_bonded_passes(bbar, nbar, cbar)
+    achieved = b_alg * n / (k_ms * 1e-3) / 1e9
+    out = {
+        "metric": "FP64 bonded-DEM particle-steps/s (1M-sphere tensile block)",
+        "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (simple-cubic lattice of touching spheres)",
+        "config": {"workload": "bonded DEM: %d spheres (%dx%dx%d), face bonds, x faces pulled at "
+                               "+-0.01 m/s, contacts between unbonded pairs" % (n, *dims),
+                   "spheres": n, "dt": wl.dt, "cutoff": wl.cutoff, "rebuild_every": wl.rebuild_every,
+                   "physics_mode": "reference" if args.mode == 0 else "corrected",
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "fused bonded step kernel (bonds + contacts + stage3 + next stage1/2)",
+                     "algorithmic_bytes_per_particle": b_alg, "kernel_ms": k_ms,
+                     "launches": launches[0]},
+        "bonds": {"initial": bonds0, "broken": broken, "mean_intact_per_particle": bbar},
+        "contacts": {"mean_neighbors": nbar, "mean_contacts": cbar},
+        "step_model": {"rebuild_ms": ms[1] / max(launches[1], 1), "rebuilds": launches[1]},
+    }
+    sysm.close()
+    del sysm
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_bonded(dims, 200, wl.rebuild_every)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline_ensemble(wl, nsys_sample, steps):
+    """The CPU oracle on the first systems of the ensemble, one after the
+    other (each system is an independent run)."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle_py as O
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    work, secs = 0, 0.0
+    for k in range(nsys_sample):
+        sd, st = wl.systems[k], wl.states[k].copy()
+        prm = O.params(dt=wl.dt, cor_pp=sd["cor_pp"], cor_pw=sd["cor_pw"], fr_pp=sd["friction_pp"],
+                       fr_pw=sd["friction_pw"], gravity=wl.gravity, mode=0)
+        _, t = O.run(st, wl.walls[k], prm, nsteps=steps, rc=wl.cutoff, lo=sd["lo"], hi=sd["hi"],
+                     rebuild_every=wl.rebuild_every, max_n=64, threads=threads)
+        work += sd["n"] * steps
+        secs += t
+    n = sum(wl.systems[k]["n"] for k in range(nsys_sample))
+    return {"value": work / secs, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"the first {nsys_sample} systems ({n} spheres), {steps} steps each, one "
+                      f"system after the other, rebuild every {wl.rebuild_every} ({secs:.1f} s)"}
+
+
+def run_ensemble(args):
+    """Batched ensemble: 256 independent systems, one fused launch per step."""
+    from cabanadem_amd import workloads as W
+
+    t0 = time.perf_counter()
+    wl = W.ensemble()
+    n = wl.n
+    sysm = wl.system(mode=args.mode)
+    log(f"[bench] ensemble of {len(wl.systems)} systems, {n} spheres, built in "
+        f"{time.perf_counter() - t0:.1f} s")
+    if args.warmup > 0:
+        sysm.step(args.warmup)
+    t0 = time.perf_counter()
+    tune = sysm.autotune(trial_steps=80)
+    log(f"[bench] autotune: rebuild every {tune.rebuild_every}, grid {tune.grid_blocks}, graph "
+        f"{tune.use_graph}, {tune.ms_per_step:.4f} ms/step ({tune.n_trials} trials, "
+        f"{time.perf_counter() - t0:.1f} s)")
+    # kernel duration and contact statistics: a short timed window, eager
+    use_graph = bool(tune.use_graph)
+    sysm.set_graph(False)
+    sysm.set_diag_every(50)
+    sysm.set_timing(True)
+    sysm.step(200)
+    ms, launches = sysm.timing()
+    sysm.set_timing(False)
+    diag = sysm.diag_history()
+    nbar, cbar, wbar = (float(diag[:, c].mean()) for c in (6, 7, 8))
+    sysm.set_diag_every(0)
+    sysm.set_graph(use_graph)
+    sysm.sync()
+    t0 = time.perf_counter()
+    sysm.step(args.steps)
+    sysm.sync()
+    elapsed = time.perf_counter() - t0
+    value = n * args.steps / elapsed
+    k_ms = ms[0] / max(launches[0], 1)
+    b_alg = W.bytes_reference_passes(nbar, cbar, wbar)
+    achieved = b_alg * n / (k_ms * 1e-3) / 1e9
+    out = {
+        "metric": "FP64 DEM particle-steps/s, batched ensemble of 256 systems",
+        "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (per-system jittered-lattice packings, seeded)",
+        "config": {"workload": "ensemble: %d wall-bounded systems, %d spheres in total (1k-64k "
+                               "log-uniform), per-system restitution and friction, gravity"
+                               % (len(wl.systems), n),
+                   "spheres": n, "systems": len(wl.systems), "dt": wl.dt,
+                   "autotuned": {"rebuild_every": tune.rebuild_every,
+                                 "grid_blocks": tune.grid_blocks, "graph": tune.use_graph,
+                                 "trials": tune.trials()},
+                   "physics_mode": "reference" if args.mode == 0 else "corrected",
+                   "parallelism": "single GPU, one launch per step for all systems"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "fused step kernel over all systems",
+                     "algorithmic_bytes_per_particle": b_alg, "kernel_ms": k_ms,
+                     "launches": launches[0]},
+        "contacts": {"mean_neighbors": nbar, "mean_contacts": cbar, "mean_wall_contacts": wbar,
+                     "contact_pairs_per_s": cbar * value},
+    }
+    sysm.close()
+    del sysm
+    out["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline_ensemble(wl, 64, 500)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10000)
-    ap.add_argument("--warmup", type=int, default=2000)
-    ap.add_argument("--dims", type=str, default="200,200,100")
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--dims", type=str, default=None)
+    ap.add_argument("--workload", choices=("plain_large", "bonded_block", "ensemble"),
+                    default="plain_large")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=str, default="200,40")
-    ap.add_argument("--mode", type=int, default=0, help="0 reference arithmetic, 1 corrected")
+    ap.add_argument("--mode", type=int, default=None,
+                    help="0 reference arithmetic, 1 corrected (default 0; bonded_block: 1)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(the design for sharding is described in DESIGN.md).", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "bonded_block":
+        args.steps = 20000 if args.steps is None else args.steps
+        args.warmup = 0 if args.warmup is None else args.warmup
+        # The reference's normal bond damping adds energy (+beta |f_n| sign(v_n)),
+        # which shakes the pulled block apart within a few thousand steps; the
+        # tensile workload is run with the corrected sign unless asked otherwise.
+        args.mode = 1 if args.mode is None else args.mode
+        return run_bonded(args)
+    args.mode = 0 if args.mode is None else args.mode
+    if args.workload == "ensemble":
+        args.steps = 5000 if args.steps is None else args.steps
+        args.warmup = 0 if args.warmup is None else args.warmup
+        return run_ensemble(args)
+    args.steps = 10000 if args.steps is None else args.steps
+    args.warmup = 2000 if args.warmup is None else args.warmup
+    args.dims = args.dims or "200,200,100"
 
     from cabanadem_amd import abi
     from cabanadem_amd import workloads as W

@@ -78,6 +78,34 @@ class DemCounters(C.Structure):
     ]
 
 
+class DemSystemDesc(C.Structure):
+    _fields_ = [
+        ("n", C.c_int), ("n_walls", C.c_int),
+        ("cor_pp", C.c_double), ("cor_pw", C.c_double),
+        ("friction_pp", C.c_double), ("friction_pw", C.c_double),
+        ("lo", C.c_double * 3), ("hi", C.c_double * 3),
+    ]
+
+
+DEM_TUNE_MAX_TRIALS = 64
+
+
+class DemTuneResult(C.Structure):
+    _fields_ = [
+        ("rebuild_every", C.c_int), ("grid_blocks", C.c_int), ("use_graph", C.c_int),
+        ("n_trials", C.c_int), ("ms_per_step", C.c_double),
+        ("trial_rebuild_every", C.c_int * DEM_TUNE_MAX_TRIALS),
+        ("trial_grid_blocks", C.c_int * DEM_TUNE_MAX_TRIALS),
+        ("trial_graph", C.c_int * DEM_TUNE_MAX_TRIALS),
+        ("trial_ms_per_step", C.c_double * DEM_TUNE_MAX_TRIALS),
+    ]
+
+    def trials(self):
+        return [dict(rebuild_every=self.trial_rebuild_every[k], grid_blocks=self.trial_grid_blocks[k],
+                     graph=self.trial_graph[k], ms_per_step=self.trial_ms_per_step[k])
+                for k in range(self.n_trials)]
+
+
 class DemBondConfig(C.Structure):
     _fields_ = [
         ("max_bonds", C.c_int), ("unbonded_contacts", C.c_int),
@@ -124,6 +152,16 @@ SYMBOLS = {
     "dem_synchronize": (C.c_int, [C.c_void_p]),
     "dem_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "dem_get_timing": (C.c_int, [C.c_void_p, _dp, C.POINTER(C.c_int64)]),
+    "dem_create_ensemble": (C.c_int, [C.POINTER(DemConfig), C.c_int, C.POINTER(DemSystemDesc),
+                                      C.POINTER(C.c_void_p)]),
+    "dem_get_systems": (C.c_int, [C.c_void_p, _ip, _ip]),
+    "dem_upload_system_walls": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(DemWallsView)]),
+    "dem_set_system_params": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double,
+                                        C.c_double]),
+    "dem_system_diagnostics": (C.c_int, [C.c_void_p, _dp]),
+    "dem_set_launch": (C.c_int, [C.c_void_p, C.c_int]),
+    "dem_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
+    "dem_autotune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(DemTuneResult)]),
     "dem_enable_bonds": (C.c_int, [C.c_void_p, C.POINTER(DemBondConfig)]),
     "dem_setup_bonds": (C.c_int, [C.c_void_p, C.c_double]),
     "dem_upload_bonds": (C.c_int, [C.c_void_p, C.POINTER(DemBondsView)]),
@@ -288,7 +326,10 @@ class System:
 
     def __init__(self, n, *, dt, cutoff, lo, hi, n_walls=0, max_neighbors=0, max_contacts=16,
                  cor_pp=1.0, cor_pw=1.0, friction_pp=0.0, friction_pw=0.0, gravity=(0., 0., 0.),
-                 skin=0.0, rebuild_every=1, physics_mode=MODE_REFERENCE):
+                 skin=0.0, rebuild_every=1, physics_mode=MODE_REFERENCE, systems=None):
+        """systems: None for one system, or a list of dicts (n, n_walls, cor_pp,
+        cor_pw, friction_pp, friction_pw, lo, hi) for an ensemble, whose
+        particles are concatenated in system order."""
         self._lib = lib()
         cfg = DemConfig()
         cfg.n, cfg.n_walls, cfg.max_neighbors, cfg.max_contacts = n, n_walls, max_neighbors, max_contacts
@@ -301,8 +342,24 @@ class System:
         cfg.rebuild_every, cfg.physics_mode, cfg.block_size = rebuild_every, physics_mode, 0
         self.cfg = cfg
         self.n = n
+        self.n_systems = 1
+        self.offsets = np.array([0, n])
         h = C.c_void_p()
-        rc = self._lib.dem_create(C.byref(cfg), C.byref(h))
+        if systems is None:
+            rc = self._lib.dem_create(C.byref(cfg), C.byref(h))
+        else:
+            arr = (DemSystemDesc * len(systems))()
+            for k, sd in enumerate(systems):
+                arr[k].n, arr[k].n_walls = sd["n"], sd.get("n_walls", 0)
+                arr[k].cor_pp, arr[k].cor_pw = sd["cor_pp"], sd.get("cor_pw", sd["cor_pp"])
+                arr[k].friction_pp = sd["friction_pp"]
+                arr[k].friction_pw = sd.get("friction_pw", sd["friction_pp"])
+                arr[k].lo[:] = list(sd["lo"])
+                arr[k].hi[:] = list(sd["hi"])
+            rc = self._lib.dem_create_ensemble(C.byref(cfg), len(systems), arr, C.byref(h))
+            self.n_systems = len(systems)
+            self.offsets = np.concatenate([[0], np.cumsum([sd["n"] for sd in systems])])
+            self.n = int(self.offsets[-1])
         if rc != 0:
             raise DemError(rc, self._lib.dem_last_error(None).decode())
         self._h = h
@@ -375,6 +432,39 @@ class System:
         c = DemCounters()
         self._check(self._lib.dem_get_counters(self._h, C.byref(c)))
         return c
+
+    # ---- ensembles ----
+    def upload_system_walls(self, system: int, w: Walls):
+        self._check(self._lib.dem_upload_system_walls(self._h, system, C.byref(w.view())))
+
+    def set_system_params(self, system, cor_pp, cor_pw, friction_pp, friction_pw):
+        self._check(self._lib.dem_set_system_params(self._h, system, cor_pp, cor_pw, friction_pp,
+                                                    friction_pw))
+
+    def system_diagnostics(self) -> np.ndarray:
+        out = np.zeros((self.n_systems, 5))
+        self._check(self._lib.dem_system_diagnostics(self._h, _ptr(out)))
+        return out
+
+    def set_launch(self, grid_blocks: int):
+        self._check(self._lib.dem_set_launch(self._h, grid_blocks))
+
+    def set_params(self, **changes):
+        """Non-structural dem_config fields (dt, cor_pp, ..., rebuild_every, physics_mode)."""
+        for k, v in changes.items():
+            if k in ("gravity", "lo", "hi"):
+                getattr(self.cfg, k)[:] = list(v)
+            else:
+                setattr(self.cfg, k, v)
+        self._check(self._lib.dem_set_params(self._h, C.byref(self.cfg)))
+
+    def set_graph(self, on: bool):
+        self._check(self._lib.dem_set_graph(self._h, 1 if on else 0))
+
+    def autotune(self, trial_steps: int = 100) -> DemTuneResult:
+        r = DemTuneResult()
+        self._check(self._lib.dem_autotune(self._h, trial_steps, C.byref(r)))
+        return r
 
     # ---- bonded DEM, groups, external force ----
     def enable_bonds(self, max_bonds, *, beta_bond=0.1, radius_factor=1.0, tensile_strength=0.0,

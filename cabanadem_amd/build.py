@@ -48,6 +48,19 @@ def build_library(extra_flags=None) -> Path:
     return LIB
 
 
+GNN_LIB = LIB.parent / "libcabanadem_gnn.so"
+
+
+def build_gnn_library(extra_flags=None) -> Path:
+    """The surrogate's kernels (gnn.hip) as their own shared library."""
+    LIB.parent.mkdir(parents=True, exist_ok=True)
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    tmp = GNN_LIB.with_suffix(".so.tmp")
+    _run([hipcc] + HIPCC_FLAGS + list(extra_flags or []) + [CSRC / "gnn.hip", "-o", tmp], cwd=CSRC)
+    os.replace(tmp, GNN_LIB)
+    return GNN_LIB
+
+
 def build_copy_probe() -> Path:
     """Streaming-copy probe (known bytes) used to calibrate the HBM counters."""
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -86,6 +99,7 @@ def build_examples() -> list:
 def build_all():
     build_oracle()
     build_library()
+    build_gnn_library()
     build_examples()
     build_copy_probe()
 
@@ -94,6 +108,8 @@ if __name__ == "__main__":
     what = sys.argv[1] if len(sys.argv) > 1 else "all"
     if what == "lib":
         build_library()
+    elif what == "gnn":
+        build_gnn_library()
     elif what == "oracle":
         build_oracle()
     elif what == "examples":

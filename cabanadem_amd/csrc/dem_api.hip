@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <chrono>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -51,10 +52,33 @@ struct dem_ctx {
   dem_config cfg{};
   std::string err;
   hipStream_t stream = nullptr;
-  int n = 0, ld = 0, nw = 0, max_n = 0, max_c = 0;
-  Grid grid{};
+  int n = 0, ld = 0, max_n = 0, max_c = 0;
   long ncell = 0;
   double rc2 = 0.;
+
+  // systems: one, or an ensemble of independent systems in one context.
+  // System s owns slots [sys_off[s], sys_off[s+1]) and tiles [tile0[s], tile0[s+1]).
+  int nsys = 1;
+  std::vector<int> sys_off, tile0;
+  std::vector<Grid> grids;
+  std::vector<long> kbase;
+  std::vector<SysParams> h_sys;
+  int ntiles = 0;
+  SysParams* d_sys = nullptr;
+  int4* d_tiles = nullptr;   // ensembles only
+  int* d_tile0 = nullptr;
+  int* d_sysid = nullptr;    // ensembles only
+  Grid* d_grids = nullptr;   // ensembles only
+  long* d_kbase = nullptr;   // ensembles only
+  double* d_sysdiag = nullptr; // [nsys][8]
+  int grid_blocks = 0;       // force-kernel workgroups; 0 = one per tile
+
+  // captured step cycles (dem_set_graph): valid while the launch arguments
+  // (generation gen) and the buffer parity are those of the capture
+  bool use_graph = false;
+  bool capturing = false;
+  hipGraphExec_t gexec = nullptr;
+  int gen = 0, g_gen = -1, g_parity = -1, g_K = -1;
 
   // double-buffered groups: P (xr, uw), S (mat, mi, I, id, ft, wall), C (contacts)
   double4* xr[2] = {nullptr, nullptr};
@@ -100,7 +124,6 @@ struct dem_ctx {
   double* diag_rows = nullptr;
   int diag_rows_cap = 0;
 
-  WallConst walls[DEM_MAX_WALLS]{};
   bool list_valid = false;
   bool built_once = false;
   int64_t step = 0;
@@ -132,6 +155,9 @@ struct dem_ctx {
     hipFree(nn); hipFree(nbr); hipFree(xb); hipFree(key); hipFree(rank); hipFree(key_sorted);
     hipFree(cell_count); hipFree(cell_start); hipFree(perm); hipFree(inv); hipFree(scan_tmp);
     hipFree(flags); hipFree(diag_part); hipFree(diag_rows);
+    hipFree(d_sys); hipFree(d_tiles); hipFree(d_tile0); hipFree(d_sysid); hipFree(d_grids);
+    hipFree(d_kbase); hipFree(d_sysdiag);
+    if (gexec) hipGraphExecDestroy(gexec);
     for (auto e : ev_pool) hipEventDestroy(e);
     if (stream) hipStreamDestroy(stream);
   }
@@ -147,17 +173,15 @@ struct dem_ctx {
 
   StepArgs step_args() const {
     StepArgs a{};
-    a.n = n; a.ld = ld; a.n_walls = nw; a.max_n = max_n; a.max_c = max_c;
+    a.n = n; a.ld = ld; a.max_n = max_n; a.max_c = max_c;
     a.physics_mode = cfg.physics_mode;
     a.dt = cfg.dt;
     a.hdt = 0.5 * cfg.dt; // src/CabanaDEM_Integrator.hpp:25
     for (int d = 0; d < 3; ++d) a.g[d] = cfg.gravity[d];
     a.gravity_on = (cfg.gravity[0] != 0. || cfg.gravity[1] != 0. || cfg.gravity[2] != 0.) ? 1 : 0;
-    a.c_eta_pp = eta_prefactor(cfg.cor_pp);
-    a.c_eta_pw = eta_prefactor(cfg.cor_pw);
-    a.fr_pp = cfg.friction_pp;
-    a.fr_pw = cfg.friction_pw;
-    for (int w = 0; w < DEM_MAX_WALLS; ++w) a.walls[w] = walls[w];
+    a.sys = d_sys;
+    a.tiles = nsys > 1 ? d_tiles : nullptr;
+    a.ntiles = ntiles;
     a.xr_in = xr[cP]; a.xr_out = xr[cP ^ 1];
     a.uw_in = uw[cP]; a.uw_out = uw[cP ^ 1];
     a.mat = mat[cS]; a.mI = mI[cS]; a.ft = ft[cS];
@@ -192,7 +216,8 @@ struct dem_ctx {
     b.keep_xb = cfg.skin > 0. ? 1 : 0;
     b.permute_ft = permute_ft ? 1 : 0;
     b.n = n; b.ld = ld; b.max_n = max_n; b.max_c = max_c;
-    b.grid = grid; b.ncell = ncell; b.rc2 = rc2;
+    b.grid = grids[0]; b.ncell = ncell; b.rc2 = rc2;
+    if (nsys > 1) { b.sysid = d_sysid; b.grids = d_grids; b.kbase = d_kbase; }
     double sh = 0.5 * cfg.skin;
     b.skin_half_sq = sh * sh;
     b.check_skin = (check_skin && cfg.skin > 0.) ? 1 : 0;
@@ -233,14 +258,14 @@ struct dem_ctx {
     return ev_pool[ev_used++];
   }
   void mark_begin(int slot) {
-    if (!timing) return;
+    if (!timing || capturing) return;
     hipEvent_t e = next_event();
     if (!e) return;
     hipEventRecord(e, stream);
     ev_marks.push_back({slot, ev_used - 1});
   }
   void mark_end() {
-    if (!timing || ev_marks.empty()) return;
+    if (!timing || capturing || ev_marks.empty()) return;
     hipEvent_t e = next_event();
     if (!e) return;
     hipEventRecord(e, stream);
@@ -296,6 +321,14 @@ struct dem_ctx {
     return r;
   }
 
+  // Materials and walls of every system to the device (synchronous).
+  int upload_sys() {
+    hipError_t e = hipStreamSynchronize(stream);
+    if (e == hipSuccess)
+      e = hipMemcpy(d_sys, h_sys.data(), h_sys.size() * sizeof(SysParams), hipMemcpyHostToDevice);
+    return e == hipSuccess ? DEM_OK : hip_fail(e, "upload system parameters");
+  }
+
   // original id of each sorted slot (host copy of the current order)
   int sorted_ids(std::vector<int>* id) {
     id->resize(n);
@@ -313,6 +346,7 @@ struct dem_ctx {
   }
 
   int alloc_nbr(int cap) {
+    ++gen; // the rows move: captured launches are stale
     hipFree(nbr);
     nbr = nullptr;
     hipError_t e = hipSuccess;
@@ -439,26 +473,103 @@ const char* dem_last_error(const dem_ctx* ctx) {
   return copy.c_str();
 }
 
-int dem_create(const dem_config* cfg, dem_ctx** out) {
+}  // extern "C" (reopened below)
+
+namespace {
+
+// Materials of one system into its parameter block.
+void set_materials(SysParams& P, double cor_pp, double cor_pw, double fr_pp, double fr_pw) {
+  P.c_eta_pp = eta_prefactor(cor_pp);
+  P.c_eta_pw = eta_prefactor(cor_pw);
+  P.fr_pp = fr_pp;
+  P.fr_pw = fr_pw;
+}
+
+// Walls of one system: src/CabanaDEM_Force.hpp:562 and :613 per-wall factors.
+int set_walls(SysParams& P, const dem_walls_view* w, std::string* msg) {
+  if (!w || w->n < 0 || w->n > DEM_MAX_WALLS) { *msg = "wall count must be in [0, 8]"; return DEM_ERR_ARG; }
+  if (w->n > 0 && (!w->E || !w->nu || !w->G)) { *msg = "walls need E, nu and G"; return DEM_ERR_ARG; }
+  P.n_walls = w->n;
+  for (int j = 0; j < w->n; ++j) {
+    WallConst& W = P.walls[j];
+    for (int d = 0; d < 3; ++d) {
+      W.x[d] = w->x ? w->x[3 * j + d] : 0.;
+      W.n[d] = w->normal ? w->normal[3 * j + d] : 0.;
+    }
+    W.a1 = (1. - (w->nu[j] * w->nu[j])) / w->E[j];
+    W.a2 = (2. - w->nu[j]) / w->G[j];
+  }
+  return DEM_OK;
+}
+
+// One system's configuration: the common fields plus its own count,
+// materials and binning box.
+dem_config system_config(const dem_config& common, const dem_system_desc& d) {
+  dem_config c = common;
+  c.n = d.n;
+  c.n_walls = d.n_walls;
+  c.cor_pp = d.cor_pp; c.cor_pw = d.cor_pw;
+  c.friction_pp = d.friction_pp; c.friction_pw = d.friction_pw;
+  for (int k = 0; k < 3; ++k) { c.lo[k] = d.lo[k]; c.hi[k] = d.hi[k]; }
+  return c;
+}
+
+int create_ctx(const dem_config* common, int nsys, const dem_system_desc* sd, dem_ctx** out) {
   if (!out) { set_global_error("null output pointer"); return DEM_ERR_ARG; }
   *out = nullptr;
+  if (!common || nsys < 1 || !sd) { set_global_error("need a configuration and at least one system"); return DEM_ERR_ARG; }
   std::string msg;
-  int rc = validate(cfg, &msg);
-  if (rc) { set_global_error(msg); return rc; }
+  long total = 0;
+  for (int s = 0; s < nsys; ++s) {
+    dem_config cs = system_config(*common, sd[s]);
+    int rc = validate(&cs, &msg);
+    if (rc) { set_global_error(nsys > 1 ? "system " + std::to_string(s) + ": " + msg : msg); return rc; }
+    total += sd[s].n;
+  }
+  if (total > (1l << 30)) { set_global_error("more than 2^30 particles in one context"); return DEM_ERR_ARG; }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
     set_global_error("no HIP device available");
     return DEM_ERR_HIP;
   }
   dem_ctx* c = new dem_ctx();
-  c->cfg = *cfg;
+  c->cfg = system_config(*common, sd[0]);
+  c->cfg.n = (int)total;
   if (c->cfg.rebuild_every <= 0) c->cfg.rebuild_every = 1;
-  c->n = cfg->n;
-  c->ld = (cfg->n + 63) / 64 * 64;
-  c->nw = cfg->n_walls;
-  c->max_c = cfg->max_contacts;
-  c->rc2 = cfg->cutoff * cfg->cutoff; // Cabana: rsqr = delta * delta
-  c->grid = make_grid(*cfg, &c->ncell);
+  c->n = (int)total;
+  c->ld = (c->n + 63) / 64 * 64;
+  c->max_c = common->max_contacts;
+  c->rc2 = common->cutoff * common->cutoff; // Cabana: rsqr = delta * delta
+  // systems: slot ranges, tiles, grids and key ranges, materials
+  c->nsys = nsys;
+  c->sys_off.assign(nsys + 1, 0);
+  c->tile0.assign(nsys + 1, 0);
+  c->grids.resize(nsys);
+  c->kbase.assign(nsys + 1, 0);
+  c->h_sys.assign(nsys, SysParams{});
+  std::vector<int4> tiles;
+  std::vector<int> sysid;
+  for (int s = 0; s < nsys; ++s) {
+    dem_config cs = system_config(*common, sd[s]);
+    long nc = 0;
+    c->grids[s] = make_grid(cs, &nc);
+    c->kbase[s + 1] = c->kbase[s] + nc;
+    c->sys_off[s + 1] = c->sys_off[s] + sd[s].n;
+    const int nt = (sd[s].n + kBlock - 1) / kBlock;
+    c->tile0[s + 1] = c->tile0[s] + nt;
+    for (int t = 0; t < nt; ++t)
+      tiles.push_back(make_int4(s, c->sys_off[s] + t * kBlock, c->sys_off[s + 1], 0));
+    if (nsys > 1) sysid.insert(sysid.end(), (size_t)sd[s].n, s);
+    set_materials(c->h_sys[s], sd[s].cor_pp, sd[s].cor_pw, sd[s].friction_pp, sd[s].friction_pw);
+    c->h_sys[s].n_walls = 0;
+  }
+  c->ncell = c->kbase[nsys];
+  c->ntiles = c->tile0[nsys];
+  if (c->ncell > (1l << 30)) {
+    delete c;
+    set_global_error("the binning grids need more than 2^30 cells in total");
+    return DEM_ERR_ARG;
+  }
   hipError_t e = hipStreamCreate(&c->stream);
   if (e != hipSuccess) { int r = c->hip_fail(e, "hipStreamCreate"); delete c; return r; }
   const size_t n = (size_t)c->n, ld = (size_t)c->ld;
@@ -486,9 +597,19 @@ int dem_create(const dem_config* cfg, dem_ctx** out) {
   c->inv = dalloc<int>(n, &e);
   c->scan_tmp = dalloc<int>((size_t)(c->ncell / 4096 + 2), &e);
   c->flags = dalloc<DevFlags>(1, &e);
-  c->diag_part = dalloc<double>((size_t)force_blocks(c->n) * 8, &e);
+  c->diag_part = dalloc<double>((size_t)c->ntiles * 8, &e);
+  c->d_sys = dalloc<SysParams>((size_t)nsys, &e);
+  c->d_tile0 = dalloc<int>((size_t)nsys + 1, &e);
+  c->d_sysdiag = dalloc<double>((size_t)nsys * 8, &e);
+  if (nsys > 1) {
+    c->d_tiles = dalloc<int4>(tiles.size(), &e);
+    c->d_sysid = dalloc<int>(n, &e);
+    c->d_grids = dalloc<Grid>((size_t)nsys, &e);
+    c->d_kbase = dalloc<long>((size_t)nsys, &e);
+  }
   if (e != hipSuccess) { int r = c->hip_fail(e, "hipMalloc"); delete c; return r; }
-  int cap = cfg->max_neighbors > 0 ? cfg->max_neighbors : 48;
+  int rc;
+  int cap = common->max_neighbors > 0 ? common->max_neighbors : 48;
   if ((rc = c->alloc_nbr(cap))) { delete c; return rc; }
   // zero state: identity order, no contacts, no wall history
   std::vector<int> ids(n);
@@ -500,11 +621,39 @@ int dem_create(const dem_config* cfg, dem_ctx** out) {
   if (e == hipSuccess) e = hipMemset(c->xr[0], 0, n * sizeof(double4));
   if (e == hipSuccess) e = hipMemset(c->nn, 0, n * sizeof(int));
   if (e == hipSuccess) e = hipMemset(c->flags, 0, sizeof(DevFlags));
+  if (e == hipSuccess) e = hipMemcpy(c->d_tile0, c->tile0.data(), (nsys + 1) * sizeof(int), hipMemcpyHostToDevice);
+  if (e == hipSuccess && nsys > 1) {
+    e = hipMemcpy(c->d_tiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(c->d_sysid, sysid.data(), n * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(c->d_grids, c->grids.data(), nsys * sizeof(Grid), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(c->d_kbase, c->kbase.data(), nsys * sizeof(long), hipMemcpyHostToDevice);
+  }
   if (e != hipSuccess) { int r = c->hip_fail(e, "initial fill"); delete c; return r; }
+  if ((rc = c->upload_sys())) { delete c; return rc; }
   c->h_m.assign(n, 1.); c->h_rad.assign(n, 1.); c->h_E.assign(n, 1.);
   c->h_nu.assign(n, 0.); c->h_G.assign(n, 1.); c->h_I.assign(n, 1.);
   *out = c;
   return DEM_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int dem_create(const dem_config* cfg, dem_ctx** out) {
+  if (!cfg) { set_global_error("null config"); if (out) *out = nullptr; return DEM_ERR_ARG; }
+  dem_system_desc d{};
+  d.n = cfg->n;
+  d.n_walls = cfg->n_walls;
+  d.cor_pp = cfg->cor_pp; d.cor_pw = cfg->cor_pw;
+  d.friction_pp = cfg->friction_pp; d.friction_pw = cfg->friction_pw;
+  for (int k = 0; k < 3; ++k) { d.lo[k] = cfg->lo[k]; d.hi[k] = cfg->hi[k]; }
+  return create_ctx(cfg, 1, &d, out);
+}
+
+int dem_create_ensemble(const dem_config* common, int n_systems, const dem_system_desc* systems,
+                        dem_ctx** out) {
+  return create_ctx(common, n_systems, systems, out);
 }
 
 void dem_destroy(dem_ctx* ctx) {
@@ -515,36 +664,92 @@ void dem_destroy(dem_ctx* ctx) {
 
 int dem_set_params(dem_ctx* c, const dem_config* cfg) {
   if (int r = check_ctx(c)) return r;
+  dem_config want = *cfg;
+  if (c->nsys > 1) {
+    // ensembles: only the common fields change; count, box and materials
+    // stay per system (dem_set_system_params)
+    want.n = c->cfg.n;
+    for (int d = 0; d < 3; ++d) { want.lo[d] = c->cfg.lo[d]; want.hi[d] = c->cfg.hi[d]; }
+    want.cor_pp = c->cfg.cor_pp; want.cor_pw = c->cfg.cor_pw;
+    want.friction_pp = c->cfg.friction_pp; want.friction_pw = c->cfg.friction_pw;
+  }
   std::string msg;
-  if (int r = validate(cfg, &msg)) return c->fail(r, msg);
+  if (int r = validate(&want, &msg)) return c->fail(r, msg);
   const dem_config& o = c->cfg;
-  bool same = cfg->n == o.n && cfg->max_contacts == o.max_contacts && cfg->cutoff == o.cutoff;
-  for (int d = 0; d < 3; ++d) same = same && cfg->lo[d] == o.lo[d] && cfg->hi[d] == o.hi[d];
+  bool same = want.n == o.n && want.max_contacts == o.max_contacts && want.cutoff == o.cutoff;
+  for (int d = 0; d < 3; ++d) same = same && want.lo[d] == o.lo[d] && want.hi[d] == o.hi[d];
   if (!same) return c->fail(DEM_ERR_ARG, "dem_set_params cannot change n, max_contacts, cutoff or the box");
   hipStreamSynchronize(c->stream);
   dem_config keep = c->cfg;
-  c->cfg = *cfg;
+  c->cfg = want;
   c->cfg.max_neighbors = keep.max_neighbors;
   c->cfg.n_walls = keep.n_walls;
   if (c->cfg.rebuild_every <= 0) c->cfg.rebuild_every = 1;
+  ++c->gen;
+  if (c->nsys == 1) {
+    set_materials(c->h_sys[0], want.cor_pp, want.cor_pw, want.friction_pp, want.friction_pw);
+    return c->upload_sys();
+  }
   return DEM_OK;
+}
+
+int dem_set_system_params(dem_ctx* c, int system, double cor_pp, double cor_pw,
+                          double friction_pp, double friction_pw) {
+  if (int r = check_ctx(c)) return r;
+  if (system < 0 || system >= c->nsys) return c->fail(DEM_ERR_ARG, "system index out of range");
+  if (!(cor_pp > 0. && cor_pp <= 1.) || !(cor_pw > 0. && cor_pw <= 1.))
+    return c->fail(DEM_ERR_ARG, "restitution coefficients must be in (0, 1]");
+  set_materials(c->h_sys[system], cor_pp, cor_pw, friction_pp, friction_pw);
+  if (c->nsys == 1) {
+    c->cfg.cor_pp = cor_pp; c->cfg.cor_pw = cor_pw;
+    c->cfg.friction_pp = friction_pp; c->cfg.friction_pw = friction_pw;
+  }
+  return c->upload_sys();
 }
 
 int dem_upload_walls(dem_ctx* c, const dem_walls_view* w) {
   if (int r = check_ctx(c)) return r;
-  if (!w || w->n < 0 || w->n > DEM_MAX_WALLS) return c->fail(DEM_ERR_ARG, "wall count must be in [0, 8]");
-  c->nw = w->n;
-  c->cfg.n_walls = w->n;
-  for (int j = 0; j < c->nw; ++j) {
-    WallConst& W = c->walls[j];
-    for (int d = 0; d < 3; ++d) {
-      W.x[d] = w->x ? w->x[3 * j + d] : 0.;
-      W.n[d] = w->normal ? w->normal[3 * j + d] : 0.;
-    }
-    // src/CabanaDEM_Force.hpp:562 and :613 per-wall factors
-    W.a1 = (1. - (w->nu[j] * w->nu[j])) / w->E[j];
-    W.a2 = (2. - w->nu[j]) / w->G[j];
-  }
+  if (c->nsys > 1) return c->fail(DEM_ERR_STATE, "an ensemble takes walls per system (dem_upload_system_walls)");
+  return dem_upload_system_walls(c, 0, w);
+}
+
+int dem_upload_system_walls(dem_ctx* c, int system, const dem_walls_view* w) {
+  if (int r = check_ctx(c)) return r;
+  if (system < 0 || system >= c->nsys) return c->fail(DEM_ERR_ARG, "system index out of range");
+  std::string msg;
+  if (int r = set_walls(c->h_sys[system], w, &msg)) return c->fail(r, msg);
+  if (c->nsys == 1) c->cfg.n_walls = w->n;
+  return c->upload_sys();
+}
+
+int dem_get_systems(dem_ctx* c, int* n_systems, int* offsets) {
+  if (int r = check_ctx(c)) return r;
+  if (n_systems) *n_systems = c->nsys;
+  if (offsets) std::copy(c->sys_off.begin(), c->sys_off.end(), offsets);
+  return DEM_OK;
+}
+
+int dem_system_diagnostics(dem_ctx* c, double* out) {
+  if (int r = check_ctx(c)) return r;
+  if (!out) return c->fail(DEM_ERR_ARG, "null output");
+  StepArgs a = c->step_args();
+  hipError_t e = launch_diag_current(a, c->uw[c->cP], c->stream);
+  if (e == hipSuccess) e = launch_diag_systems(c->diag_part, c->d_tile0, c->nsys, c->d_sysdiag, c->stream);
+  std::vector<double> h((size_t)c->nsys * 8);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(h.data(), c->d_sysdiag, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return c->hip_fail(e, "system diagnostics");
+  for (int s = 0; s < c->nsys; ++s)
+    for (int k = 0; k < 5; ++k) out[5 * s + k] = h[8 * (size_t)s + k];
+  return DEM_OK;
+}
+
+int dem_set_launch(dem_ctx* c, int grid_blocks) {
+  if (int r = check_ctx(c)) return r;
+  if (grid_blocks < 0) return c->fail(DEM_ERR_ARG, "grid_blocks must be >= 0");
+  c->grid_blocks = grid_blocks;
+  ++c->gen;
   return DEM_OK;
 }
 
@@ -790,7 +995,7 @@ int dem_update_tangential(dem_ctx* c) {
 static int force_call(dem_ctx* c, ForceKind kind, bool needs_list) {
   if (int r = check_ctx(c)) return r;
   if (needs_list && !c->list_valid) return c->fail(DEM_ERR_STATE, "neighbour list not built for the current positions");
-  hipError_t e = launch_force(kind, c->step_args(), false, c->stream);
+  hipError_t e = launch_force(kind, c->step_args(), false, c->grid_blocks, c->stream);
   if (e != hipSuccess) return c->hip_fail(e, "force launch");
   if (kind != FK_PW_ACCUM && kind != FK_BOND_ACCUM) c->cC ^= 1;
   return c->check_and_clear_flags();
@@ -817,11 +1022,85 @@ int dem_set_diag_every(dem_ctx* c, int every) {
   return DEM_OK;
 }
 
+// Enqueues nsteps full steps (the first kick, then fused steps, the last one
+// ending with stage3 and F, T written) without synchronising.  Sampled
+// diagnostic rows go to diag_rows[*row...].
+static int enqueue_steps(dem_ctx* c, int nsteps, int* row) {
+  const int K = c->cfg.rebuild_every;
+  // stage1 + stage2 of the first step (later ones are fused into the force kernel)
+  StepArgs a0 = c->step_args();
+  hipError_t e = launch_kick(a0, c->xr[c->cP], c->uw[c->cP], c->ft[c->cS], 1, 1, c->stream);
+  if (e != hipSuccess) return c->hip_fail(e, "stage1/2 launch");
+  for (int s = 0; s < nsteps; ++s) {
+    const int64_t gstep = c->step;
+    if (!c->list_valid || gstep % K == 0) {
+      int r = c->rebuild(!c->built_once, c->list_valid, false);
+      if (r) return r;
+    }
+    const bool sample = c->diag_every > 0 && gstep % c->diag_every == 0;
+    const bool last = (s == nsteps - 1);
+    StepArgs a = c->step_args();
+    c->mark_begin(0);
+    e = launch_force(last ? FK_STEP_LAST : FK_STEP_FUSED, a, sample, c->grid_blocks, c->stream);
+    c->mark_end();
+    if (e != hipSuccess) return c->hip_fail(e, "step launch");
+    c->cP ^= 1;
+    c->cC ^= 1;
+    if (sample) {
+      e = launch_diag_final(c->diag_part, c->ntiles, c->diag_rows + 9 * (*row), (double)gstep, c->stream);
+      if (e != hipSuccess) return c->hip_fail(e, "diag launch");
+      ++*row;
+    }
+    c->step++;
+  }
+  return DEM_OK;
+}
+
+// One cycle of 2K steps starting at a multiple of K, from a captured graph:
+// captured (and run) on first use or when the launch arguments changed,
+// replayed afterwards.  A cycle flips every double buffer an even number of
+// times, so the captured pointers stay valid from one replay to the next.
+static int graph_cycle(dem_ctx* c) {
+  const int K = c->cfg.rebuild_every;
+  const int parity = c->cP | (c->cS << 1) | (c->cC << 2);
+  const bool valid = c->gexec && c->g_gen == c->gen && c->g_parity == parity && c->g_K == K;
+  if (valid) {
+    hipError_t e = hipGraphLaunch(c->gexec, c->stream);
+    if (e != hipSuccess) return c->hip_fail(e, "graph launch");
+    c->step += 2 * K;
+    c->rebuilds += 2;
+    return DEM_OK;
+  }
+  if (c->gexec) { hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+  hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return c->hip_fail(e, "begin capture");
+  c->capturing = true;
+  int row = 0;
+  const int64_t step0 = c->step, reb0 = c->rebuilds;
+  int r = enqueue_steps(c, 2 * K, &row);
+  c->capturing = false;
+  hipGraph_t g = nullptr;
+  e = hipStreamEndCapture(c->stream, &g);
+  if (r) { if (g) hipGraphDestroy(g); return r; }
+  if (e != hipSuccess) return c->hip_fail(e, "end capture");
+  e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (e != hipSuccess) { c->gexec = nullptr; return c->hip_fail(e, "graph instantiate"); }
+  // the capture advanced the host-side counters; the device runs it now
+  e = hipGraphLaunch(c->gexec, c->stream);
+  if (e != hipSuccess) return c->hip_fail(e, "graph launch");
+  c->g_gen = c->gen;
+  c->g_parity = c->cP | (c->cS << 1) | (c->cC << 2);
+  c->g_K = K;
+  if (c->g_parity != parity || c->step != step0 + 2 * K || c->rebuilds != reb0 + 2)
+    return c->fail(DEM_ERR_STATE, "captured cycle did not return to its starting buffers");
+  return DEM_OK;
+}
+
 int dem_step(dem_ctx* c, int nsteps) {
   if (int r = check_ctx(c)) return r;
   if (nsteps <= 0) return DEM_OK;
   const int K = c->cfg.rebuild_every;
-  const int nb = force_blocks(c->n);
   // diagnostic rows of this call
   int rows = 0;
   if (c->diag_every > 0)
@@ -835,32 +1114,38 @@ int dem_step(dem_ctx* c, int nsteps) {
     if (e != hipSuccess) return c->hip_fail(e, "hipMalloc diag rows");
     c->diag_rows_cap = rows;
   }
-  // stage1 + stage2 of the first step (later ones are fused into the force kernel)
-  StepArgs a0 = c->step_args();
-  hipError_t e = launch_kick(a0, c->xr[c->cP], c->uw[c->cP], c->ft[c->cS], 1, 1, c->stream);
-  if (e != hipSuccess) return c->hip_fail(e, "stage1/2 launch");
   int row = 0;
-  for (int s = 0; s < nsteps; ++s) {
-    const int64_t gstep = c->step;
-    if (!c->list_valid || gstep % K == 0) {
-      int r = c->rebuild(!c->built_once, c->list_valid, false);
-      if (r) return r;
+  hipError_t e;
+  const bool graphs = c->use_graph && c->diag_every == 0 && !c->timing && K < (1 << 20);
+  if (!graphs) {
+    if (int r = enqueue_steps(c, nsteps, &row)) return r;
+  } else {
+    // eager up to a multiple of K (and through the first build), whole
+    // cycles of 2K steps from the graph, the rest eager
+    int left = nsteps;
+    int head = (int)((K - c->step % K) % K);
+    if (head == 0 && (!c->built_once || !c->list_valid)) head = K; // build eagerly first
+    head = std::min(head, left);
+    if (head > 0) {
+      if (int r = enqueue_steps(c, head, &row)) return r;
+      if (!c->built_once) {
+        // the first build sizes the neighbour rows; settle it before capturing
+        e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) return c->hip_fail(e, "step sync");
+      }
+      left -= head;
     }
-    const bool sample = c->diag_every > 0 && gstep % c->diag_every == 0;
-    const bool last = (s == nsteps - 1);
-    StepArgs a = c->step_args();
-    c->mark_begin(0);
-    e = launch_force(last ? FK_STEP_LAST : FK_STEP_FUSED, a, sample, c->stream);
-    c->mark_end();
-    if (e != hipSuccess) return c->hip_fail(e, "step launch");
-    c->cP ^= 1;
-    c->cC ^= 1;
-    if (sample) {
-      e = launch_diag_final(c->diag_part, nb, c->diag_rows + 9 * row, (double)gstep, c->stream);
-      if (e != hipSuccess) return c->hip_fail(e, "diag launch");
-      ++row;
+    if (c->step % K != 0 && left > 0) {
+      // the head could not reach a multiple of K (first build): finish eagerly
+      if (int r = enqueue_steps(c, left, &row)) return r;
+      left = 0;
     }
-    c->step++;
+    while (left >= 2 * K) {
+      if (int r = graph_cycle(c)) return r;
+      left -= 2 * K;
+    }
+    if (left > 0)
+      if (int r = enqueue_steps(c, left, &row)) return r;
   }
   if (row > 0) {
     std::vector<double> h((size_t)row * 9);
@@ -879,6 +1164,179 @@ int dem_step(dem_ctx* c, int nsteps) {
   if (e != hipSuccess) return c->hip_fail(e, "step sync");
   c->collect_timing();
   return c->check_and_clear_flags();
+}
+
+int dem_set_graph(dem_ctx* c, int enable) {
+  if (int r = check_ctx(c)) return r;
+  c->use_graph = enable != 0;
+  return DEM_OK;
+}
+
+} // extern "C"
+
+namespace {
+
+// Exact copy of a context's live state: the current half of every double
+// buffer, the neighbour rows and the host-side counters.
+struct Snapshot {
+  std::vector<std::pair<void*, void*>> bufs; // (live buffer, copy)
+  std::vector<size_t> bytes;
+  int cP, cS, cC, max_n, max_nn_seen;
+  int64_t step, rebuilds;
+  bool list_valid, built_once;
+  size_t diag_rows;
+  double last_sample[3];
+  dem_config cfg;
+  int grid_blocks, diag_every;
+  bool use_graph, timing;
+
+  void add(void* p, size_t nbytes, hipError_t* e) {
+    if (!p || nbytes == 0) return;
+    void* q = nullptr;
+    if (*e == hipSuccess) *e = hipMalloc(&q, nbytes);
+    if (*e == hipSuccess) *e = hipMemcpy(q, p, nbytes, hipMemcpyDeviceToDevice);
+    bufs.push_back({p, q});
+    bytes.push_back(nbytes);
+  }
+  ~Snapshot() {
+    for (auto& b : bufs) hipFree(b.second);
+  }
+};
+
+// Buffers in the order restore() expects them; nbr is the last one.
+void snapshot_buffers(dem_ctx* c, Snapshot* s, hipError_t* e) {
+  const size_t n = c->n, ld = c->ld;
+  s->add(c->xr[c->cP], n * sizeof(double4), e);
+  s->add(c->uw[c->cP], 6 * n * sizeof(double), e);
+  s->add(c->ft[c->cS], 6 * n * sizeof(double), e);
+  s->add(c->mat[c->cS], n * sizeof(double4), e);
+  s->add(c->mI[c->cS], n * sizeof(double2), e);
+  s->add(c->id[c->cS], n * sizeof(int), e);
+  s->add(c->sw[c->cS], (size_t)DEM_MAX_WALLS * 3 * ld * sizeof(double), e);
+  s->add(c->cmeta[c->cC], n * sizeof(int), e);
+  s->add(c->cj[c->cC], (size_t)c->max_c * ld * sizeof(int), e);
+  s->add(c->ch[c->cC], (size_t)c->max_c * 3 * ld * sizeof(double), e);
+  s->add(c->nn, n * sizeof(int), e);
+  s->add(c->xb, 3 * n * sizeof(double), e);
+  s->add(c->flags, sizeof(DevFlags), e);
+  if (c->bonds_on) {
+    s->add(c->nbond[c->cS], n * sizeof(int), e);
+    s->add(c->bj[c->cS], (size_t)c->max_b * ld * sizeof(int), e);
+    s->add(c->bl[c->cS], (size_t)c->max_b * ld * sizeof(double), e);
+    s->add(c->bh[c->cS], (size_t)9 * c->max_b * ld * sizeof(double), e);
+    s->add(c->bEG[c->cS], n * sizeof(double2), e);
+  }
+  if (c->n_groups > 0) s->add(c->vgroup[c->cS], n * sizeof(int), e);
+  if (c->fext[c->cS]) s->add(c->fext[c->cS], 3 * n * sizeof(double), e);
+  s->add(c->nbr, (size_t)c->max_n * ld * sizeof(int), e);
+}
+
+int take_snapshot(dem_ctx* c, Snapshot* s) {
+  hipError_t e = hipStreamSynchronize(c->stream);
+  s->cP = c->cP; s->cS = c->cS; s->cC = c->cC;
+  s->max_n = c->max_n; s->max_nn_seen = c->max_nn_seen;
+  s->step = c->step; s->rebuilds = c->rebuilds;
+  s->list_valid = c->list_valid; s->built_once = c->built_once;
+  s->diag_rows = c->diag_hist.size();
+  std::copy(c->last_sample, c->last_sample + 3, s->last_sample);
+  s->cfg = c->cfg; s->grid_blocks = c->grid_blocks; s->diag_every = c->diag_every;
+  s->use_graph = c->use_graph; s->timing = c->timing;
+  if (e == hipSuccess) snapshot_buffers(c, s, &e);
+  return e == hipSuccess ? DEM_OK : c->hip_fail(e, "autotune snapshot");
+}
+
+int restore_snapshot(dem_ctx* c, const Snapshot& s) {
+  hipError_t e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return c->hip_fail(e, "autotune restore");
+  if (c->max_n != s.max_n)
+    if (int r = c->alloc_nbr(s.max_n)) return r;
+  c->cP = s.cP; c->cS = s.cS; c->cC = s.cC;
+  // the live pointers of the snapshot are the current halves again; the
+  // neighbour rows may have been re-allocated, so take the current one
+  for (size_t k = 0; k < s.bufs.size(); ++k) {
+    void* dst = (k + 1 == s.bufs.size()) ? (void*)c->nbr : s.bufs[k].first;
+    if (e == hipSuccess) e = hipMemcpy(dst, s.bufs[k].second, s.bytes[k], hipMemcpyDeviceToDevice);
+  }
+  if (e != hipSuccess) return c->hip_fail(e, "autotune restore");
+  c->max_nn_seen = s.max_nn_seen;
+  c->step = s.step; c->rebuilds = s.rebuilds;
+  c->list_valid = s.list_valid; c->built_once = s.built_once;
+  c->diag_hist.resize(s.diag_rows);
+  std::copy(s.last_sample, s.last_sample + 3, c->last_sample);
+  c->cfg = s.cfg; c->grid_blocks = s.grid_blocks; c->diag_every = s.diag_every;
+  c->use_graph = s.use_graph; c->timing = s.timing;
+  ++c->gen;
+  c->err.clear();
+  return DEM_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int dem_autotune(dem_ctx* c, int trial_steps, dem_tune_result* out) {
+  if (int r = check_ctx(c)) return r;
+  if (!out) return c->fail(DEM_ERR_ARG, "null result");
+  if (trial_steps <= 0) return c->fail(DEM_ERR_ARG, "trial_steps must be positive");
+  *out = dem_tune_result{};
+  Snapshot snap;
+  if (int r = take_snapshot(c, &snap)) return r;
+  int cus = 256;
+  {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+      cus = prop.multiProcessorCount;
+  }
+  std::vector<int> Ks;
+  const int K0 = snap.cfg.rebuild_every;
+  if (snap.cfg.skin > 0.) Ks = {5, 10, 20, 40, 80};
+  if (std::find(Ks.begin(), Ks.end(), K0) == Ks.end()) Ks.push_back(K0);
+  const std::vector<int> grids = {0, 2 * cus, 4 * cus, 8 * cus};
+  double best = -1.;
+  int nt = 0;
+  for (int K : Ks)
+    for (int g : grids)
+      for (int graph = 0; graph < 2; ++graph) {
+        if (nt >= DEM_TUNE_MAX_TRIALS) break;
+        if (int r = restore_snapshot(c, snap)) return r;
+        c->cfg.rebuild_every = K;
+        c->grid_blocks = g;
+        c->use_graph = graph != 0;
+        c->diag_every = 0;
+        c->timing = false;
+        ++c->gen;
+        const int steps = ((std::max(trial_steps, 2 * K) + 2 * K - 1) / (2 * K)) * (2 * K);
+        double ms = -1.;
+        int r = dem_step(c, 2 * K); // warm-up: builds, captures
+        if (r == DEM_OK) {
+          auto t0 = std::chrono::steady_clock::now();
+          r = dem_step(c, steps);
+          auto t1 = std::chrono::steady_clock::now();
+          if (r == DEM_OK) ms = std::chrono::duration<double, std::milli>(t1 - t0).count() / steps;
+        }
+        if (r == DEM_ERR_HIP) return r;
+        out->trial_rebuild_every[nt] = K;
+        out->trial_grid_blocks[nt] = g;
+        out->trial_graph[nt] = graph;
+        out->trial_ms_per_step[nt] = ms;
+        if (ms > 0. && (best < 0. || ms < best)) {
+          best = ms;
+          out->rebuild_every = K;
+          out->grid_blocks = g;
+          out->use_graph = graph;
+        }
+        ++nt;
+      }
+  out->n_trials = nt;
+  out->ms_per_step = best;
+  if (int r = restore_snapshot(c, snap)) return r;
+  if (best < 0.) return c->fail(DEM_ERR_STATE, "every autotune candidate failed");
+  c->cfg.rebuild_every = out->rebuild_every;
+  c->grid_blocks = out->grid_blocks;
+  c->use_graph = out->use_graph != 0;
+  ++c->gen;
+  return DEM_OK;
 }
 
 int dem_get_diag_history(dem_ctx* c, double* rows, int cap, int* nrows) {
@@ -903,7 +1361,7 @@ int dem_diagnostics(dem_ctx* c, double out[5]) {
     if (ea != hipSuccess) return c->hip_fail(ea, "hipMalloc diag rows");
     c->diag_rows_cap = 1;
   }
-  e = launch_diag_final(c->diag_part, force_blocks(c->n), c->diag_rows, (double)c->step, c->stream);
+  e = launch_diag_final(c->diag_part, c->ntiles, c->diag_rows, (double)c->step, c->stream);
   double h[9];
   if (e == hipSuccess) e = hipMemcpyAsync(h, c->diag_rows, sizeof(h), hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -970,6 +1428,7 @@ int dem_enable_bonds(dem_ctx* c, const dem_bond_config* b) {
     if (b->max_bonds != c->max_b) return c->fail(DEM_ERR_ARG, "max_bonds cannot change once bonds are enabled");
     hipStreamSynchronize(c->stream);
     c->bcfg = *b;
+    ++c->gen;
     return DEM_OK;
   }
   std::vector<int> id;
@@ -994,6 +1453,7 @@ int dem_enable_bonds(dem_ctx* c, const dem_bond_config* b) {
   c->max_b = b->max_bonds;
   c->bcfg = *b;
   c->bonds_on = true;
+  ++c->gen;
   return c->upload_bEG(id);
 }
 
@@ -1099,6 +1559,7 @@ int dem_set_external_force(dem_ctx* c, const double* f) {
   if (e != hipSuccess) return c->hip_fail(e, "sync");
   if (!f) {
     for (int k = 0; k < 2; ++k) { hipFree(c->fext[k]); c->fext[k] = nullptr; }
+    ++c->gen;
     return DEM_OK;
   }
   std::vector<int> id;
@@ -1111,6 +1572,7 @@ int dem_set_external_force(dem_ctx* c, const double* f) {
   for (size_t p = 0; p < n; ++p)
     for (int d = 0; d < 3; ++d) h[3 * p + d] = f[3 * (size_t)id[p] + d];
   e = hipMemcpy(c->fext[c->cS], h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice);
+  ++c->gen;
   return e == hipSuccess ? DEM_OK : c->hip_fail(e, "upload external force");
 }
 
@@ -1121,6 +1583,7 @@ int dem_set_groups(dem_ctx* c, const int* group, int ngroups, const int* kind, c
   if (!group) {
     for (int k = 0; k < 2; ++k) { hipFree(c->vgroup[k]); c->vgroup[k] = nullptr; }
     c->n_groups = 0;
+    ++c->gen;
     return DEM_OK;
   }
   if (ngroups < 1 || ngroups > DEM_MAX_GROUPS || !kind)
@@ -1148,6 +1611,7 @@ int dem_set_groups(dem_ctx* c, const int* group, int ngroups, const int* kind, c
     for (int d = 0; d < 3; ++d) c->gvel[g][d] = (g < ngroups && vel) ? vel[3 * g + d] : 0.;
   }
   c->n_groups = ngroups;
+  ++c->gen;
   return DEM_OK;
 }
 

@@ -48,10 +48,14 @@ __global__ __launch_bounds__(kBlock) void k_cell_keys(BuildArgs a) {
   if (i >= a.n) return;
   const double4 p = a.xr_in[i];
   if (!(isfinite(p.x) && isfinite(p.y) && isfinite(p.z))) atomicOr(&a.flags->nonfinite, 1);
-  const int cx = cell_coord(p.x, a.grid, 0);
-  const int cy = cell_coord(p.y, a.grid, 1);
-  const int cz = cell_coord(p.z, a.grid, 2);
-  const int key = (int)cell_key(a.grid, cx, cy, cz);
+  // ensembles: the slot's system gives the grid and the first key
+  const int sys = a.sysid ? a.sysid[i] : 0;
+  const Grid& g = a.sysid ? a.grids[sys] : a.grid;
+  const long base = a.sysid ? a.kbase[sys] : 0;
+  const int cx = cell_coord(p.x, g, 0);
+  const int cy = cell_coord(p.y, g, 1);
+  const int cz = cell_coord(p.z, g, 2);
+  const int key = (int)(base + cell_key(g, cx, cy, cz));
   a.key[i] = key;
   a.rank[i] = atomicAdd(&a.cell_count[key], 1);
   if (a.check_skin) {
@@ -244,9 +248,11 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(BuildArgs a) {
   const int p = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   int cnt = 0;
   if (p < a.n && !a.flags->cell_overflow) {
-    const Grid& g = a.grid;
+    const int sys = a.sysid ? a.sysid[p] : 0;
+    const Grid& g = a.sysid ? a.grids[sys] : a.grid;
+    const long base = a.sysid ? a.kbase[sys] : 0;
     const int nx = g.nc[0], ny = g.nc[1], nz = g.nc[2];
-    int key = a.key_sorted[p];
+    int key = (int)(a.key_sorted[p] - base);
     const int cx = key % nx;
     key /= nx;
     const int ly = key % kTileRows;
@@ -265,7 +271,7 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(BuildArgs a) {
         rs[q] = INT_MAX;
         re[q] = INT_MAX;
       } else {
-        const long k0 = cell_key(g, xlo, y, z);
+        const long k0 = base + cell_key(g, xlo, y, z);
         rs[q] = a.cell_start[k0];
         re[q] = a.cell_start[k0 + (xhi - xlo) + 1];
       }

@@ -52,7 +52,8 @@ __device__ inline void store6(double* p, const double* v) {
 template <bool REF>
 __device__ inline void pp_contact(const double4& xi, const double* ui, const double4& mi,
                                   const double4& xj, const double* uj, const double4& mj,
-                                  double* h, double* F, double* T, const StepArgs& a) {
+                                  double* h, double* F, double* T, const StepArgs& a,
+                                  const SysParams& P) {
   const double p0 = xi.x - xj.x, p1 = xi.y - xj.y, p2 = xi.z - xj.z; // :94-96
   const double r2 = p0 * p0 + p1 * p1 + p2 * p2;                      // :99
   const double rij = sqrt(r2);                                        // :101
@@ -86,10 +87,10 @@ __device__ inline void pp_contact(const double4& xi, const double* ui, const dou
   const double sRo = sqrt(R_eff * overlap);
   const double S_n = 2. * E_eff * sRo;                              // :211
   const double m_eff = 1. / (mi.w + mj.w);                          // :212-214
-  const double eta_n = a.c_eta_pp * sqrt(S_n * m_eff);              // :215
+  const double eta_n = P.c_eta_pp * sqrt(S_n * m_eff);              // :215
   const double G_eff = 1. / (mi.y + mj.y);                          // :218-220
   const double kt = 8. * G_eff * sRo;                               // :222
-  const double eta_t = REF ? 0. : a.c_eta_pp * sqrt(kt * m_eff);    // :224 (5/6 == 0: Q2)
+  const double eta_t = REF ? 0. : P.c_eta_pp * sqrt(kt * m_eff);    // :224 (5/6 == 0: Q2)
   const double fn = kn * (overlap * sqrt(overlap));                 // :227
   const double fn0 = fn * nx - eta_n * vn0;                         // :228-230
   const double fn1 = fn * ny - eta_n * vn1;
@@ -114,7 +115,7 @@ __device__ inline void pp_contact(const double4& xi, const double* ui, const dou
     const double ftm = sqrt(fs0 * fs0 + fs1 * fs1 + fs2 * fs2);     // :290
     const double fnm = sqrt(fn0 * fn0 + fn1 * fn1 + fn2 * fn2);     // :291
     double ftms = ftm;                                              // :293-296
-    if (ftms > a.fr_pp * fnm) ftms = a.fr_pp * fnm;
+    if (ftms > P.fr_pp * fnm) ftms = P.fr_pp * fnm;
     tf0 = -ftms * ti0; tf1 = -ftms * ti1; tf2 = -ftms * ti2;        // :299-301
     const double m0 = -tf0 / kt, m1 = -tf1 / kt, m2 = -tf2 / kt;    // :303-305
     if (REF) {                                                      // :307-319 (Q3)
@@ -140,7 +141,7 @@ __device__ inline void pp_contact(const double4& xi, const double* ui, const dou
 template <bool REF>
 __device__ inline void pw_contact(const double4& xi, const double* ui, const double4& mi,
                                   const WallConst& W, double overlap, double* sw,
-                                  double* F, double* T, const StepArgs& a) {
+                                  double* F, double* T, const StepArgs& a, const SysParams& P) {
   const double nx = W.n[0], ny = W.n[1], nz = W.n[2];               // :509-511
   const double a_i = xi.w - overlap / 2.;                           // :517
   double v0, v1, v2;
@@ -162,7 +163,7 @@ __device__ inline void pw_contact(const double4& xi, const double* ui, const dou
   const double sRo = sqrt(R_eff * overlap);
   const double S_n = 2. * E_eff * sRo;                              // :576
   const double m_eff = 1. / (mi.w + 0.);                            // :577-579
-  const double eta_n = a.c_eta_pw * sqrt(S_n * m_eff);              // :580
+  const double eta_n = P.c_eta_pw * sqrt(S_n * m_eff);              // :580
   const double fn = kn * (overlap * sqrt(overlap));                 // :583
   const double fn0 = fn * nx - eta_n * vn0;                         // :584-586
   const double fn1 = fn * ny - eta_n * vn1;
@@ -175,7 +176,7 @@ __device__ inline void pw_contact(const double4& xi, const double* ui, const dou
     sw[0] += vt0 * a.dt; sw[1] += vt1 * a.dt; sw[2] += vt2 * a.dt;  // :607-609
     const double G_eff = 1. / (mi.y + W.a2);                        // :612-614
     const double kt = 8. * G_eff * sRo;                             // :616
-    const double eta_t = REF ? 0. : a.c_eta_pw * sqrt(kt * m_eff);  // :618 (Q2)
+    const double eta_t = REF ? 0. : P.c_eta_pw * sqrt(kt * m_eff);  // :618 (Q2)
     const double fs0 = -kt * sw[0] - eta_t * vt0;                   // :620-622
     const double fs1 = -kt * sw[1] - eta_t * vt1;
     const double fs2 = -kt * sw[2] - eta_t * vt2;
@@ -184,11 +185,21 @@ __device__ inline void pw_contact(const double4& xi, const double* ui, const dou
     double ti0 = 0., ti1 = 0., ti2 = 0.;
     if (ftm > 1e-12) { ti0 = fs0 / ftm; ti1 = fs1 / ftm; ti2 = fs2 / ftm; } // :630-634
     const double fnm = sqrt(fn0 * fn0 + fn1 * fn1 + fn2 * fn2);     // :636
-    const double lim = a.fr_pw * fnm;                               // :638 std::min(lim, ftm)
+    const double lim = P.fr_pw * fnm;                               // :638 std::min(lim, ftm)
     const double ftms = (ftm < lim) ? ftm : lim;
     ft0 = ftms * ti0; ft1 = ftms * ti1; ft2 = ftms * ti2;           // :641-643
-    ft0 += eta_t * vt0; ft1 += eta_t * vt1; ft2 += eta_t * vt2;     // :646-648
-    sw[0] = -ft0 / kt; sw[1] = -ft1 / kt; sw[2] = -ft2 / kt;        // :651-653
+    if (REF) {
+      // :646-648 add +eta_t v_t to the applied force (along the slip, quirk
+      // Q16; inert here since eta_t = 0), then reset the spring to it
+      ft0 += eta_t * vt0; ft1 += eta_t * vt1; ft2 += eta_t * vt2;
+      sw[0] = -ft0 / kt; sw[1] = -ft1 / kt; sw[2] = -ft2 / kt;      // :651-653
+    } else {
+      // the limited force already holds the damping; the spring keeps the
+      // elastic part of it
+      sw[0] = -(ft0 + eta_t * vt0) / kt;
+      sw[1] = -(ft1 + eta_t * vt1) / kt;
+      sw[2] = -(ft2 + eta_t * vt2) / kt;
+    }
   }
   F[0] += fn0 + ft0;                                                // :657-659
   F[1] += fn1 + ft1;
@@ -360,14 +371,19 @@ __device__ inline void block_reduce8(const double* v, double* part, int blk) {
 
 // KICK: 0 = forces only (write F,T); 1 = + stage3 (write state, F,T);
 //       2 = + stage3 + stage1 + stage2 of the next step (write state).
+// One tile: kBlock consecutive slots of one system.
 template <bool REF, bool PP, bool PW, int KICK, bool ACCUM, bool DIAG, bool BOND>
-__global__ __launch_bounds__(kBlock, DEM_FORCE_MIN_WAVES) void k_force(StepArgs a) {
-  extern __shared__ int s_cj[]; // [max_c][kBlock] contacts found in phase 1
+__device__ __forceinline__ void force_tile(const StepArgs& a, const int tile, int* s_cj) {
   const int tid = threadIdx.x;
-  const int blk = xcd_block(blockIdx.x, gridDim.x);
-  const int i = blk * kBlock + tid;
+  int sys = 0, first = tile * kBlock, end = a.n;
+  if (a.tiles) {
+    const int4 t = a.tiles[tile];
+    sys = t.x; first = t.y; end = t.z;
+  }
+  const SysParams& P = a.sys[sys];
+  const int i = first + tid;
   double dg[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
-  if (i < a.n) {
+  if (i < end) {
     const long ld = a.ld;
     const double4 xi = a.xr_in[i];
     double ui[6];
@@ -471,7 +487,7 @@ __global__ __launch_bounds__(kBlock, DEM_FORCE_MIN_WAVES) void k_force(StepArgs 
         double uj[6];
         load6(a.uw_in + 6l * j, uj);
         const double4 mj = a.mat[j];
-        pp_contact<REF>(xi, ui, mti, xj, uj, mj, h, F, T, a);
+        pp_contact<REF>(xi, ui, mti, xj, uj, mj, h, F, T, a, P);
         a.cj_out[m * ld + i] = j;
         a.ch_out[(3 * m + 0) * ld + i] = h[0];
         a.ch_out[(3 * m + 1) * ld + i] = h[1];
@@ -482,8 +498,8 @@ __global__ __launch_bounds__(kBlock, DEM_FORCE_MIN_WAVES) void k_force(StepArgs 
     if (PW) {
       const int wm = wn;
       wn = 0;
-      for (int w = 0; w < a.n_walls; ++w) {
-        const WallConst& W = a.walls[w];
+      for (int w = 0; w < P.n_walls; ++w) {
+        const WallConst& W = P.walls[w];
         const double q0 = xi.x - W.x[0], q1 = xi.y - W.x[1], q2 = xi.z - W.x[2]; // :497-499
         const double tmp = q0 * W.n[0] + q1 * W.n[1] + q2 * W.n[2];              // :514
         const double overlap = xi.w - tmp;                                        // :515
@@ -495,7 +511,7 @@ __global__ __launch_bounds__(kBlock, DEM_FORCE_MIN_WAVES) void k_force(StepArgs 
             sw[1] = a.sw[(3 * w + 1) * ld + i];
             sw[2] = a.sw[(3 * w + 2) * ld + i];
           }
-          pw_contact<REF>(xi, ui, mti, W, overlap, sw, F, T, a);
+          pw_contact<REF>(xi, ui, mti, W, overlap, sw, F, T, a, P);
           a.sw[(3 * w + 0) * ld + i] = sw[0];
           a.sw[(3 * w + 1) * ld + i] = sw[1];
           a.sw[(3 * w + 2) * ld + i] = sw[2];
@@ -556,7 +572,19 @@ __global__ __launch_bounds__(kBlock, DEM_FORCE_MIN_WAVES) void k_force(StepArgs 
     dg[6] = (double)nc;
     dg[7] = (double)nwc;
   }
-  if (DIAG) block_reduce8(dg, a.diag_part, blk);
+  if (DIAG) block_reduce8(dg, a.diag_part, tile);
+}
+
+// The fused force / step kernel.  A grid of one workgroup per tile, or a
+// persistent grid that strides over the tiles; either way tile v is mapped
+// XCD-aware (contiguous tiles per XCD when the grid is a multiple of 8).
+template <bool REF, bool PP, bool PW, int KICK, bool ACCUM, bool DIAG, bool BOND>
+__global__ __launch_bounds__(kBlock, DEM_FORCE_MIN_WAVES) void k_force(StepArgs a) {
+  extern __shared__ int s_cj[]; // [max_c][kBlock] contacts found in phase 1
+  for (int v = blockIdx.x; v < a.ntiles; v += gridDim.x) {
+    force_tile<REF, PP, PW, KICK, ACCUM, DIAG, BOND>(a, xcd_block(v, a.ntiles), s_cj);
+    if (DIAG) __syncthreads(); // the reduction's LDS is reused by the next tile
+  }
 }
 
 // F = m g + f_ext, T = 0  (src/CabanaDEM_Force.hpp:39-47 plus the body and
@@ -622,10 +650,15 @@ __global__ __launch_bounds__(kBlock) void k_kick(StepArgs a, double4* xr, double
 }
 
 __global__ __launch_bounds__(kBlock) void k_diag_current(StepArgs a, const double* uw) {
-  const int blk = blockIdx.x;
-  const int i = blk * kBlock + threadIdx.x;
+  const int tile = blockIdx.x;
+  int first = tile * kBlock, end = a.n;
+  if (a.tiles) {
+    const int4 t = a.tiles[tile];
+    first = t.y; end = t.z;
+  }
+  const int i = first + threadIdx.x;
   double dg[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
-  if (i < a.n) {
+  if (i < end) {
     double s[6];
     load6(uw + 6l * i, s);
     const double m = a.mI[i].x, I = a.mI[i].y;
@@ -636,7 +669,17 @@ __global__ __launch_bounds__(kBlock) void k_diag_current(StepArgs a, const doubl
     dg[6] = (double)meta_count(a.cmeta_in[i]);
     dg[7] = (double)__popc(meta_walls(a.cmeta_in[i]));
   }
-  block_reduce8(dg, a.diag_part, blk);
+  block_reduce8(dg, a.diag_part, tile);
+}
+
+// One workgroup per system: its tiles' partials summed in tile order.
+__global__ __launch_bounds__(kBlock) void k_diag_systems(const double* part, const int* tile0,
+                                                         double* out) {
+  const int s = blockIdx.x;
+  double v[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
+  for (int b = tile0[s] + threadIdx.x; b < tile0[s + 1]; b += kBlock)
+    for (int k = 0; k < 8; ++k) v[k] += part[(long)b * 8 + k];
+  block_reduce8(v, out, s);
 }
 
 // Sum block partials in block order; row = (step, 8 sums).
@@ -659,9 +702,11 @@ __global__ __launch_bounds__(kBlock) void k_diag_final(const double* part, int n
 inline unsigned nblk(long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 template <bool REF, bool BOND>
-hipError_t launch_force_mode(ForceKind kind, const StepArgs& a, bool sample, hipStream_t s) {
+hipError_t launch_force_mode(ForceKind kind, const StepArgs& a, bool sample, int grid_blocks,
+                             hipStream_t s) {
   const size_t lds = (size_t)(a.max_c > 0 ? a.max_c : 1) * kBlock * sizeof(int);
-  const dim3 g(nblk(a.n)), b(kBlock);
+  const int nb = (grid_blocks > 0 && grid_blocks < a.ntiles) ? grid_blocks : a.ntiles;
+  const dim3 g(nb), b(kBlock);
   switch (kind) {
     case FK_PP_PW_FORCES:
       if (sample) hipLaunchKernelGGL((k_force<REF, true, true, 0, false, true, BOND>), g, b, lds, s, a);
@@ -692,12 +737,13 @@ hipError_t launch_force_mode(ForceKind kind, const StepArgs& a, bool sample, hip
 
 int force_blocks(int n) { return (int)nblk(n); }
 
-hipError_t launch_force(ForceKind kind, const StepArgs& a, bool sample, hipStream_t s) {
+hipError_t launch_force(ForceKind kind, const StepArgs& a, bool sample, int grid_blocks,
+                        hipStream_t s) {
   const bool ref = a.physics_mode == DEM_MODE_REFERENCE;
-  if (a.nbond) return ref ? launch_force_mode<true, true>(kind, a, sample, s)
-                          : launch_force_mode<false, true>(kind, a, sample, s);
-  return ref ? launch_force_mode<true, false>(kind, a, sample, s)
-             : launch_force_mode<false, false>(kind, a, sample, s);
+  if (a.nbond) return ref ? launch_force_mode<true, true>(kind, a, sample, grid_blocks, s)
+                          : launch_force_mode<false, true>(kind, a, sample, grid_blocks, s);
+  return ref ? launch_force_mode<true, false>(kind, a, sample, grid_blocks, s)
+             : launch_force_mode<false, false>(kind, a, sample, grid_blocks, s);
 }
 
 hipError_t launch_zero_forces(const StepArgs& a, hipStream_t s) {
@@ -717,7 +763,13 @@ hipError_t launch_kick(const StepArgs& a, double4* xr, double* uw, const double*
 }
 
 hipError_t launch_diag_current(const StepArgs& a, const double* uw, hipStream_t s) {
-  hipLaunchKernelGGL(k_diag_current, dim3(nblk(a.n)), dim3(kBlock), 0, s, a, uw);
+  hipLaunchKernelGGL(k_diag_current, dim3(a.ntiles), dim3(kBlock), 0, s, a, uw);
+  return hipGetLastError();
+}
+
+hipError_t launch_diag_systems(const double* part, const int* tile0, int nsys, double* out,
+                               hipStream_t s) {
+  hipLaunchKernelGGL(k_diag_systems, dim3(nsys), dim3(kBlock), 0, s, part, tile0, out);
   return hipGetLastError();
 }
 

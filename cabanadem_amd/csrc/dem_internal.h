@@ -62,6 +62,17 @@ struct Grid {
   int nty;    // y-tiles
 };
 
+// Per-system constants of the contact model.  A context holds one system
+// or an ensemble of independent systems; each system has its own materials,
+// walls and binning box, and its particles occupy one contiguous slot range
+// that the re-sort never leaves (cell keys are system-major).
+struct SysParams {
+  double c_eta_pp, c_eta_pw;   // (-2 sqrt(5/6)) * beta of the restitution coefficient
+  double fr_pp, fr_pw;
+  int n_walls, pad;
+  WallConst walls[DEM_MAX_WALLS];
+};
+
 // Cell key of cell (cx, cy, cz): tile-major, then z, then y inside the
 // tile, then x.  Keys of x-adjacent cells of one row are consecutive.
 __host__ __device__ inline long cell_key(const Grid& g, int cx, int cy, int cz) {
@@ -94,14 +105,16 @@ __host__ __device__ inline int make_meta(int count, int walls) { return count | 
 
 // Pointers and scalars of one launch, resolved for the current buffers.
 struct StepArgs {
-  int n, ld, n_walls, max_n, max_c;
+  int n, ld, max_n, max_c;
   int physics_mode;
   double dt, hdt;
   double g[3];
   int gravity_on;
-  double c_eta_pp, c_eta_pw;   // (-2 sqrt(5/6)) * beta
-  double fr_pp, fr_pw;
-  WallConst walls[DEM_MAX_WALLS];
+  const SysParams* sys;  // [n_systems]
+  // Tiles of kBlock slots, each inside one system: (system, first, end, -).
+  // Null for a single system (tile t = slots [t kBlock, min((t+1) kBlock, n))).
+  const int4* tiles;
+  int ntiles;
 
   const double4* xr_in;
   double4* xr_out;
@@ -143,8 +156,12 @@ struct StepArgs {
 
 struct BuildArgs {
   int n, ld, max_n, max_c;
-  Grid grid;
-  long ncell;        // number of cell keys (padded to whole tiles)
+  Grid grid;         // the single system's grid (sysid == null)
+  // ensembles: system of each slot (static), each system's grid and first key
+  const int* sysid;
+  const Grid* grids;
+  const long* kbase;
+  long ncell;        // number of cell keys (padded to whole tiles, all systems)
   double rc2;
   double skin_half_sq;
   int check_skin;    // compare with the positions of the last build
@@ -212,14 +229,21 @@ enum ForceKind {
   FK_STEP_LAST = 4,    // forces + stage3, write F,T
   FK_BOND_ACCUM = 5    // F,T += bond forces
 };
-hipError_t launch_force(ForceKind kind, const StepArgs& a, bool sample, hipStream_t s);
+// grid_blocks: workgroups of the launch; 0 = one per tile, otherwise a
+// persistent grid that strides over the tiles.
+hipError_t launch_force(ForceKind kind, const StepArgs& a, bool sample, int grid_blocks,
+                        hipStream_t s);
 hipError_t launch_zero_forces(const StepArgs& a, hipStream_t s);
 hipError_t launch_update_tangential(const StepArgs& a, hipStream_t s);
 hipError_t launch_kick(const StepArgs& a, double4* xr, double* uw, const double* ft,
                        int do_kick, int do_drift, hipStream_t s);
+// Per-tile partial sums of the current state (one partial row per tile).
 hipError_t launch_diag_current(const StepArgs& a, const double* uw, hipStream_t s);
 hipError_t launch_diag_final(const double* part, int nblocks, double* out_row,
                              double step, hipStream_t s);
+// Per-system sums of the tile partials: out[s][8] over tiles [tile0[s], tile0[s+1]).
+hipError_t launch_diag_systems(const double* part, const int* tile0, int nsys, double* out,
+                               hipStream_t s);
 int force_blocks(int n);
 
 } // namespace demhip

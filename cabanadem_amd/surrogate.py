@@ -1,0 +1,641 @@
+"""Learned surrogate of the DEM: a message-passing network over the DEM
+neighbour graph that predicts each particle's mean acceleration over a
+macro-step of 100 DEM steps (no counterpart in the reference).
+
+Every arithmetic step of training and inference runs in the HIP kernels of
+libcabanadem_gnn.so (include/cabanadem_gnn.h): fused two-layer MLPs on the
+BF16 MFMA units with FP32 accumulation, their backward passes and weight
+gradients, the graph aggregation and its adjoints, Adam, the features and
+the macro-step integration.  This module lays out the parameters and
+activations (device memory from torch, which is only the allocator here),
+strings the kernels into the network, and runs the DEM (libcabanadem_hip.so)
+as data generator, judge and fallback.
+
+Network (encode-process-decode):
+  node encoder   x_i (32: 5 velocities, radius, 4 wall distances)   -> h_i (128)
+  edge encoder   x_ij (32: (x_i - x_j)/R, |x_i - x_j|/R)           -> e_ij (128)
+  10 layers      e_ij += MLP_e([e_ij, h_i, h_j]);  h_i += MLP_n([h_i, sum_j e_ij])
+  decoder        h_i -> a_i (3), the acceleration minus gravity over the
+                 macro-step, in units of a_scale
+Each MLP is Linear(K, 128) - ReLU - Linear(128, out).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+import time
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+from . import abi
+
+GNN_LIB_PATH = Path(os.environ.get("CABANADEM_GNN_LIB") or
+                    Path(__file__).resolve().parent / "lib" / "libcabanadem_gnn.so")
+LAT = 128
+FEAT = 32
+HIST = 5
+GNN_ABI_VERSION = 1
+
+_vp = C.c_void_p
+_ip = C.POINTER(C.c_int)
+_fp = C.POINTER(C.c_float)
+_dp = C.POINTER(C.c_double)
+
+
+class GnnMlpIo(C.Structure):
+    _fields_ = [("M", C.c_int), ("nseg", C.c_int), ("src", _vp * 3), ("idx", _vp * 3),
+                ("width", C.c_int * 3)]
+
+
+class GnnMlpWeights(C.Structure):
+    _fields_ = [("K", C.c_int), ("nout", C.c_int), ("nout_pad", C.c_int),
+                ("W1t", _vp), ("W1", _vp), ("b1", _vp), ("W2t", _vp), ("W2", _vp), ("b2", _vp)]
+
+
+_SIG = {
+    "gnn_abi_version": (C.c_int, []),
+    "gnn_mlp_forward": (C.c_int, [C.POINTER(GnnMlpIo), C.POINTER(GnnMlpWeights), _vp, _vp, _vp, _vp,
+                                  C.c_int, _vp]),
+    "gnn_mlp_backward": (C.c_int, [C.POINTER(GnnMlpIo), C.POINTER(GnnMlpWeights), _vp, _vp, _vp,
+                                   C.POINTER(_vp), C.c_int, C.c_int, _vp]),
+    "gnn_weight_grad": (C.c_int, [C.POINTER(GnnMlpIo), _vp, _vp, C.c_int, C.c_int, _vp, _vp, C.c_int,
+                                  _vp, _vp, C.c_int, _vp]),
+    "gnn_segment_sum": (C.c_int, [_vp, _vp, C.c_int, _vp, _vp]),
+    "gnn_edge_grad_to_nodes": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp, _vp]),
+    "gnn_broadcast_add": (C.c_int, [_vp, _vp, C.c_long, _vp, _vp]),
+    "gnn_adam": (C.c_int, [_vp, _vp, _vp, _vp, C.c_long, C.c_float, C.c_float, C.c_float, C.c_float,
+                           C.c_float, _vp]),
+    "gnn_cast_weights": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp]),
+    "gnn_node_features": (C.c_int, [C.c_int, _vp, _vp, C.c_int, _vp, C.c_float, C.c_float, C.c_float,
+                                    _vp, C.c_int, C.c_float, _vp, _vp]),
+    "gnn_edge_features": (C.c_int, [C.c_long, _vp, _vp, _vp, C.c_double, _vp, _vp]),
+    "gnn_integrate": (C.c_int, [C.c_int, _vp, C.c_float, _vp, C.c_double, _vp, _vp, C.c_int, _vp]),
+    "gnn_mse": (C.c_int, [_vp, _vp, C.c_long, _vp, _vp, _vp]),
+}
+
+_GLIB = None
+
+
+def glib():
+    global _GLIB
+    if _GLIB is None:
+        if not GNN_LIB_PATH.exists():
+            raise RuntimeError(f"{GNN_LIB_PATH} is missing: build it with "
+                               "`python -m cabanadem_amd.build gnn` (there is no fallback)")
+        L = C.CDLL(str(GNN_LIB_PATH))
+        for name, (res, args) in _SIG.items():
+            f = getattr(L, name)
+            f.restype, f.argtypes = res, args
+        if L.gnn_abi_version() != GNN_ABI_VERSION:
+            raise RuntimeError("libcabanadem_gnn.so ABI mismatch: rebuild it")
+        _GLIB = L
+    return _GLIB
+
+
+def _torch():
+    import torch  # device memory and the stream only
+    return torch
+
+
+def ptr(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def _ok(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"{what} failed ({rc})")
+
+
+def stream():
+    return C.c_void_p(_torch().cuda.current_stream().cuda_stream)
+
+
+# ---------------------------------------------------------------------------
+# graph
+# ---------------------------------------------------------------------------
+
+@dataclass
+class Graph:
+    """Receiver-major edge list on the device: the edges of node i are
+    [row[i], row[i+1]); edge e is send[e] -> recv[e]; rev[e] is the edge
+    recv[e] -> send[e]."""
+    N: int
+    E: int
+    row: object
+    send: object
+    recv: object
+    rev: object
+
+    @staticmethod
+    def from_rows(counts, ids, device="cuda"):
+        torch = _torch()
+        counts = np.asarray(counts, dtype=np.int64)
+        N = counts.shape[0]
+        row = np.zeros(N + 1, dtype=np.int64)
+        np.cumsum(counts, out=row[1:])
+        mask = np.arange(ids.shape[1])[None, :] < counts[:, None]
+        send = ids[mask].astype(np.int64)
+        recv = np.repeat(np.arange(N, dtype=np.int64), counts)
+        key = recv * N + send            # ascending: rows ascending, ids sorted in a row
+        rev = np.searchsorted(key, send * N + recv)
+        if len(key) and not np.array_equal(key[np.minimum(rev, len(key) - 1)], send * N + recv):
+            raise ValueError("the neighbour graph is not symmetric")
+        t = lambda a: torch.from_numpy(a.astype(np.int32)).to(device)
+        return Graph(N, int(row[-1]), t(row), t(send), t(recv), t(rev))
+
+
+class GraphBuilder:
+    """Radius graph of a particle set through the DEM library's neighbour
+    build (the same cell-binned list the DEM uses)."""
+
+    def __init__(self, n, radius, lo, hi, max_n=64):
+        self.n, self.radius, self.max_n = n, radius, max_n
+        self.sys = abi.System(n, dt=1.0, cutoff=radius, lo=lo, hi=hi, max_contacts=1)
+        self.st = abi.ParticleState(n, max_cnts=1, max_walls=1)
+        self.st.rad[:] = radius / 4.
+
+    def build(self, x_host):
+        self.st.x[:] = x_host
+        self.sys.upload(self.st, fields=("x", "rad", "m", "I", "E", "G"))
+        self.sys.build_neighbors()
+        counts, ids = self.sys.neighbors(self.max_n)
+        return Graph.from_rows(counts, ids)
+
+
+# ---------------------------------------------------------------------------
+# parameters
+# ---------------------------------------------------------------------------
+
+class MLP:
+    """Slots of one two-layer MLP in the flat FP32 parameter vector:
+    W1 [K][128], b1 [128], W2 [128][NP], b2 [NP] (NP = 16 or 128)."""
+
+    def __init__(self, K, nout, off):
+        self.K, self.nout = K, nout
+        self.NP = 128 if nout > 16 else 16
+        self.o_W1 = off
+        self.o_b1 = self.o_W1 + K * LAT
+        self.o_W2 = self.o_b1 + LAT
+        self.o_b2 = self.o_W2 + LAT * self.NP
+        self.end = self.o_b2 + self.NP
+
+    def alloc(self, dev):
+        torch = _torch()
+        bf = torch.bfloat16
+        self.W1 = torch.zeros((self.K, LAT), dtype=bf, device=dev)
+        self.W1t = torch.zeros((LAT, self.K), dtype=bf, device=dev)
+        self.W2 = torch.zeros((LAT, self.NP), dtype=bf, device=dev)
+        self.W2t = torch.zeros((self.NP, LAT), dtype=bf, device=dev)
+
+    def views(self, flat):
+        return (flat[self.o_W1:self.o_b1].view(self.K, LAT), flat[self.o_b1:self.o_W2],
+                flat[self.o_W2:self.o_b2].view(LAT, self.NP), flat[self.o_b2:self.end])
+
+    def weights(self, flat):
+        w = GnnMlpWeights()
+        w.K, w.nout, w.nout_pad = self.K, self.nout, self.NP
+        w.W1t, w.W1, w.W2t, w.W2 = ptr(self.W1t), ptr(self.W1), ptr(self.W2t), ptr(self.W2)
+        w.b1 = C.c_void_p(flat.data_ptr() + 4 * self.o_b1)
+        w.b2 = C.c_void_p(flat.data_ptr() + 4 * self.o_b2)
+        return w
+
+
+def io(M, *segs):
+    """segs: (bf16 tensor [rows][width], index tensor or None)."""
+    s = GnnMlpIo()
+    s.M, s.nseg = M, len(segs)
+    for k, (t, idx) in enumerate(segs):
+        s.src[k] = t.data_ptr()
+        s.idx[k] = None if idx is None else idx.data_ptr()
+        s.width[k] = t.shape[1]
+    return s
+
+
+class Network:
+    def __init__(self, n_layers=10, seed=0, device="cuda", residual_scale=0.1):
+        torch = _torch()
+        self.L = n_layers
+        self.dev = device
+        off = 0
+        self.enc_n = MLP(FEAT, LAT, off); off = self.enc_n.end
+        self.enc_e = MLP(FEAT, LAT, off); off = self.enc_e.end
+        self.edge, self.node = [], []
+        for _ in range(n_layers):
+            m = MLP(3 * LAT, LAT, off); off = m.end; self.edge.append(m)
+            m = MLP(2 * LAT, LAT, off); off = m.end; self.node.append(m)
+        self.dec = MLP(LAT, 3, off); off = self.dec.end
+        self.mlps = [self.enc_n, self.enc_e] + self.edge + self.node + [self.dec]
+        self.n_params = off
+        rng = np.random.default_rng(seed)
+        p = np.zeros(off, dtype=np.float32)
+        real_in = {id(self.enc_n): 3 * HIST + 1 + 4, id(self.enc_e): 4}
+        for m in self.mlps:
+            fan1 = real_in.get(id(m), m.K)
+            b1 = math.sqrt(6. / fan1)
+            W1 = np.zeros((m.K, LAT), dtype=np.float32)
+            W1[:fan1] = rng.uniform(-b1, b1, size=(fan1, LAT))
+            scale = residual_scale if m in self.edge or m in self.node else 1.0
+            b2 = scale * math.sqrt(6. / LAT) / (1. if m is not self.dec else 4.)
+            W2 = np.zeros((LAT, m.NP), dtype=np.float32)
+            W2[:, :m.nout] = rng.uniform(-b2, b2, size=(LAT, m.nout))
+            p[m.o_W1:m.o_b1] = W1.ravel()
+            p[m.o_W2:m.o_b2] = W2.ravel()
+        self.flat = torch.from_numpy(p).to(device)
+        self.grad = torch.zeros_like(self.flat)
+        self.m = torch.zeros_like(self.flat)
+        self.v = torch.zeros_like(self.flat)
+        self.t = 0
+        for m in self.mlps:
+            m.alloc(device)
+        self.cast()
+        self.cap_N = self.cap_E = 0
+        self.part = self.bpart = None
+        self.nchunks = 64
+
+    # -- parameters --------------------------------------------------------
+    def cast(self):
+        L, s = glib(), stream()
+        for m in self.mlps:
+            base = self.flat.data_ptr()
+            _ok(L.gnn_cast_weights(C.c_void_p(base + 4 * m.o_W1), m.K, LAT, LAT, ptr(m.W1), ptr(m.W1t), s),
+                "cast W1")
+            _ok(L.gnn_cast_weights(C.c_void_p(base + 4 * m.o_W2), LAT, m.nout, m.NP, ptr(m.W2), ptr(m.W2t),
+                                   s), "cast W2")
+
+    def adam(self, lr, b1=0.9, b2=0.999, eps=1e-8, wd=0.0):
+        self.t += 1
+        lr_t = lr * math.sqrt(1. - b2 ** self.t) / (1. - b1 ** self.t)
+        _ok(glib().gnn_adam(ptr(self.flat), ptr(self.grad), ptr(self.m), ptr(self.v), self.n_params,
+                            lr_t, b1, b2, eps, wd, stream()), "adam")
+        self.cast()
+
+    def state_dict(self):
+        return {"flat": self.flat.detach().cpu().numpy(), "L": self.L}
+
+    def load(self, flat):
+        torch = _torch()
+        self.flat.copy_(torch.from_numpy(np.asarray(flat, dtype=np.float32)))
+        self.cast()
+
+    # -- buffers -----------------------------------------------------------
+    def _ensure(self, N, E, train):
+        torch = _torch()
+        if N <= self.cap_N and E <= self.cap_E and (not train or self.part is not None):
+            return
+        N, E = max(N, self.cap_N), max(E, self.cap_E)
+        bf, f32, dev = torch.bfloat16, torch.float32, self.dev
+        self.cap_N, self.cap_E = N, E
+        L = self.L
+        z = lambda *s, dt=bf: torch.zeros(s, dtype=dt, device=dev)
+        self.h = [z(N, LAT) for _ in range(L + 1)]
+        self.e = [z(E, LAT) for _ in range(L + 1)]
+        self.agg = [z(N, LAT) for _ in range(L)]
+        self.hid_e = [z(E, LAT) for _ in range(L)]
+        self.hid_n = [z(N, LAT) for _ in range(L)]
+        self.hid_enc_n, self.hid_enc_e, self.hid_dec = z(N, LAT), z(E, LAT), z(N, LAT)
+        self.out = z(N, 3, dt=f32)
+        if train:
+            self.d_h = z(N, LAT, dt=f32)
+            self.d_agg = z(N, LAT, dt=f32)
+            self.d_e = z(E, LAT, dt=f32)
+            self.d_hi = z(E, LAT, dt=f32)
+            self.d_hj = z(E, LAT, dt=f32)
+            self.dH_e = z(E, LAT)
+            self.dH_n = z(N, LAT)
+            self.part = z(self.nchunks * 3 * LAT * LAT, dt=f32)
+            self.bpart = z(self.nchunks * LAT, dt=f32)
+
+    # -- forward -----------------------------------------------------------
+    def forward(self, g: Graph, xn, xe, train=False):
+        """xn bf16 [N][32], xe bf16 [E][32] -> self.out float [N][3]."""
+        self._ensure(g.N, g.E, train)
+        Lb, s, N, E = glib(), stream(), g.N, g.E
+        F = self.flat
+        fwd = lambda mlp, inp, res, hid, out, outf: _ok(
+            Lb.gnn_mlp_forward(C.byref(inp), C.byref(mlp.weights(F)), ptr(res), ptr(hid), ptr(out), ptr(outf),
+                               0, s), "mlp forward")
+        fwd(self.enc_n, io(N, (xn, None)), None, self.hid_enc_n, self.h[0], None)
+        if E > 0:
+            fwd(self.enc_e, io(E, (xe, None)), None, self.hid_enc_e, self.e[0], None)
+        for l in range(self.L):
+            h, e = self.h[l], self.e[l]
+            if E > 0:
+                fwd(self.edge[l], io(E, (e[:E], None), (h, g.recv), (h, g.send)), e, self.hid_e[l],
+                    self.e[l + 1], None)
+            _ok(Lb.gnn_segment_sum(ptr(self.e[l + 1]), ptr(g.row), N, ptr(self.agg[l]), s), "segment sum")
+            fwd(self.node[l], io(N, (h[:N], None), (self.agg[l], None)), h, self.hid_n[l], self.h[l + 1], None)
+        fwd(self.dec, io(N, (self.h[self.L][:N], None)), None, self.hid_dec, None, self.out)
+        return self.out[:N]
+
+    # -- backward ----------------------------------------------------------
+    def _wgrad(self, mlp, inp, hid, dH, dY32, M, accumulate=0):
+        Lb, s = glib(), stream()
+        base = self.grad.data_ptr()
+        # W2, b2: X = hidden, G = dY (float [M][nout])
+        _ok(Lb.gnn_weight_grad(C.byref(io(M, (hid, None))), None, ptr(dY32), mlp.nout, mlp.NP,
+                               ptr(self.part), ptr(self.bpart), self.nchunks,
+                               C.c_void_p(base + 4 * mlp.o_W2), C.c_void_p(base + 4 * mlp.o_b2),
+                               accumulate, s), "wgrad W2")
+        # W1, b1: X = the MLP input, G = dH (bf16 [M][128])
+        _ok(Lb.gnn_weight_grad(C.byref(inp), ptr(dH), None, LAT, LAT, ptr(self.part), ptr(self.bpart),
+                               self.nchunks, C.c_void_p(base + 4 * mlp.o_W1),
+                               C.c_void_p(base + 4 * mlp.o_b1), accumulate, s), "wgrad W1")
+
+    def backward(self, g: Graph, xn, xe, d_out):
+        """d_out float [N][3]: gradient of the loss at the decoder output.
+        Fills self.grad (overwritten)."""
+        torch = _torch()
+        Lb, s, N, E = glib(), stream(), g.N, g.E
+        F = self.flat
+        bwd = lambda mlp, inp, dY, hid, dH, dX, acc, M: _ok(
+            Lb.gnn_mlp_backward(C.byref(inp), C.byref(mlp.weights(F)), ptr(dY), ptr(hid), ptr(dH),
+                                (C.c_void_p * 3)(*[None if d is None else d.data_ptr() for d in dX]),
+                                acc, 0, s), "mlp backward")
+        d_h = self.d_h[:N]
+        # decoder
+        inp = io(N, (self.h[self.L][:N], None))
+        dH = self.dH_n[:N]
+        bwd(self.dec, inp, d_out, self.hid_dec, dH, [d_h, None, None], 0, N)
+        self._wgrad(self.dec, inp, self.hid_dec, dH, d_out, N)
+        d_e = self.d_e[:E]
+        d_e.zero_()
+        for l in reversed(range(self.L)):
+            h, e = self.h[l], self.e[l]
+            # node MLP: h_{l+1} = h_l + MLP_n([h_l, agg_l])
+            inp = io(N, (h[:N], None), (self.agg[l], None))
+            self._wgrad_w2_only(self.node[l], self.hid_n[l], d_h, N)
+            bwd(self.node[l], inp, d_h, self.hid_n[l], self.dH_n[:N], [d_h, self.d_agg[:N], None], 1, N)
+            self._wgrad_w1_only(self.node[l], inp, self.dH_n[:N])
+            if E > 0:
+                # aggregation: d_e(l+1) += d_agg[recv]
+                _ok(Lb.gnn_broadcast_add(ptr(self.d_agg), ptr(g.recv), E, ptr(d_e), s), "broadcast")
+                # edge MLP: e_{l+1} = e_l + MLP_e([e_l, h_l[recv], h_l[send]])
+                inp = io(E, (e[:E], None), (h, g.recv), (h, g.send))
+                self._wgrad_w2_only(self.edge[l], self.hid_e[l], d_e, E)
+                bwd(self.edge[l], inp, d_e, self.hid_e[l], self.dH_e[:E],
+                    [d_e, self.d_hi[:E], self.d_hj[:E]], 1, E)
+                self._wgrad_w1_only(self.edge[l], inp, self.dH_e[:E])
+                _ok(Lb.gnn_edge_grad_to_nodes(ptr(self.d_hi), ptr(self.d_hj), ptr(g.row), ptr(g.rev), N,
+                                              ptr(d_h), s), "edge grads to nodes")
+        # encoders (their inputs are data: no input gradient)
+        if E > 0:
+            inp = io(E, (xe, None))
+            self._wgrad_w2_only(self.enc_e, self.hid_enc_e, d_e, E)
+            bwd(self.enc_e, inp, d_e, self.hid_enc_e, self.dH_e[:E], [None, None, None], 0, E)
+            self._wgrad_w1_only(self.enc_e, inp, self.dH_e[:E])
+        else:
+            self._zero_grad(self.enc_e)
+            for l in range(self.L):
+                self._zero_grad(self.edge[l])
+        inp = io(N, (xn, None))
+        self._wgrad_w2_only(self.enc_n, self.hid_enc_n, d_h, N)
+        bwd(self.enc_n, inp, d_h, self.hid_enc_n, self.dH_n[:N], [None, None, None], 0, N)
+        self._wgrad_w1_only(self.enc_n, inp, self.dH_n[:N])
+
+    def _zero_grad(self, mlp):
+        self.grad[mlp.o_W1:mlp.end].zero_()
+
+    def _wgrad_w2_only(self, mlp, hid, dY32, M):
+        base = self.grad.data_ptr()
+        _ok(glib().gnn_weight_grad(C.byref(io(M, (hid[:M], None))), None, ptr(dY32), mlp.nout, mlp.NP,
+                                   ptr(self.part), ptr(self.bpart), self.nchunks,
+                                   C.c_void_p(base + 4 * mlp.o_W2), C.c_void_p(base + 4 * mlp.o_b2), 0,
+                                   stream()), "wgrad W2")
+
+    def _wgrad_w1_only(self, mlp, inp, dH):
+        base = self.grad.data_ptr()
+        _ok(glib().gnn_weight_grad(C.byref(inp), ptr(dH), None, LAT, LAT, ptr(self.part), ptr(self.bpart),
+                                   self.nchunks, C.c_void_p(base + 4 * mlp.o_W1),
+                                   C.c_void_p(base + 4 * mlp.o_b1), 0, stream()), "wgrad W1")
+
+
+def mse(pred, target):
+    """Mean squared error and its gradient (float [N][3] each), on the device."""
+    torch = _torch()
+    n = pred.numel()
+    d = torch.empty_like(pred)
+    part = torch.empty((n + 255) // 256, dtype=torch.float32, device=pred.device)
+    _ok(glib().gnn_mse(ptr(pred), ptr(target), n, ptr(d), ptr(part), stream()), "mse")
+    return float(part.cpu().numpy().astype(np.float64).sum()) / n, d
+
+
+# ---------------------------------------------------------------------------
+# features, data, training, judged roll-out
+# ---------------------------------------------------------------------------
+
+MACRO = 100  # DEM steps per macro-step
+
+
+@dataclass
+class Scales:
+    v: float        # velocity scale of the node features
+    a: float        # scale of the predicted acceleration
+    r_mean: float
+    r_std: float
+    radius: float   # graph radius
+
+
+def features(g: Graph, x_dev, vhist_dev, rad_dev, planes_dev, n_planes, sc: Scales, xn, xe):
+    """Node and edge features on the device (bf16 [N][32], [E][32])."""
+    L, s = glib(), stream()
+    _ok(L.gnn_node_features(g.N, ptr(x_dev), ptr(vhist_dev), HIST, ptr(rad_dev), sc.v, sc.r_mean,
+                            sc.r_std, ptr(planes_dev), n_planes, sc.radius, ptr(xn), s), "node features")
+    _ok(L.gnn_edge_features(g.E, ptr(x_dev), ptr(g.recv), ptr(g.send), sc.radius, ptr(xe), s),
+        "edge features")
+
+
+@dataclass
+class Trajectory:
+    x: np.ndarray        # [T][N][3] float64 (host, for the graph build)
+    x_dev: object        # [T][N][3] float64 (device)
+    v_dev: object        # [T][N][3] float32 (device)
+    a_host: np.ndarray   # [T-1][N][3] float32: (v_{t+1} - v_t)/Dt - g
+    rad: np.ndarray
+    wl: object
+
+
+def generate(wl, n_frames, log=None):
+    """Runs the DEM from the workload's state and keeps every MACRO-th step."""
+    torch = _torch()
+    sysm = wl.system()
+    st = abi.ParticleState(wl.state.n, max_cnts=wl.max_contacts, max_walls=6)
+    xs = np.empty((n_frames, wl.state.n, 3))
+    vs = np.empty((n_frames, wl.state.n, 3), dtype=np.float32)
+    t0 = time.perf_counter()
+    for f in range(n_frames):
+        sysm.step(MACRO)
+        sysm.download(st, fields=("x", "u"))
+        xs[f] = st.x
+        vs[f] = st.u
+    sysm.close()
+    if log:
+        log(f"[surrogate] {n_frames} frames of {wl.state.n} spheres in {time.perf_counter() - t0:.1f} s")
+    dt = wl.dt * MACRO
+    g = np.asarray(wl.gravity, dtype=np.float32)
+    a = (vs[1:] - vs[:-1]) / np.float32(dt) - g
+    return Trajectory(xs, torch.from_numpy(xs).cuda(), torch.from_numpy(vs).cuda(), a.astype(np.float32),
+                      wl.state.rad.copy(), wl)
+
+
+def fit_scales(trajs, radius):
+    v = np.concatenate([t.v_dev[::10].cpu().numpy().ravel() for t in trajs])
+    a = np.concatenate([t.a_host[::10].ravel() for t in trajs])
+    r = np.concatenate([t.rad for t in trajs])
+    return Scales(v=float(max(v.std(), 1e-3)), a=float(max(a.std(), 1e-3)), r_mean=float(r.mean()),
+                  r_std=float(max(r.std(), 1e-6)), radius=radius)
+
+
+class Trainer:
+    def __init__(self, net: Network, trajs, scales: Scales, seed=0):
+        torch = _torch()
+        self.net, self.trajs, self.sc = net, trajs, scales
+        self.rng = np.random.default_rng(seed)
+        self.builders = []
+        self.planes = []
+        for t in trajs:
+            wl = t.wl
+            self.builders.append(GraphBuilder(wl.state.n, scales.radius, wl.lo, wl.hi))
+            self.planes.append(torch.tensor(np.asarray(wl.extra["planes"], dtype=np.float32).ravel(),
+                                            device="cuda"))
+        self.rad = [torch.from_numpy(t.rad).cuda() for t in trajs]
+        self.targets = [torch.from_numpy(t.a_host / np.float32(scales.a)).cuda() for t in trajs]
+
+    def sample(self, k, t):
+        torch = _torch()
+        tr = self.trajs[k]
+        g = self.builders[k].build(tr.x[t])
+        vh = tr.v_dev[t - HIST + 1:t + 1].contiguous()
+        xn = torch.empty((g.N, FEAT), dtype=torch.bfloat16, device="cuda")
+        xe = torch.empty((max(g.E, 1), FEAT), dtype=torch.bfloat16, device="cuda")
+        features(g, tr.x_dev[t], vh, self.rad[k], self.planes[k], len(tr.wl.extra["planes"]), self.sc, xn, xe)
+        return g, xn, xe, self.targets[k][t]
+
+    def step(self, lr, k=None, t=None):
+        if k is None:
+            k = int(self.rng.integers(len(self.trajs)))
+        T = self.trajs[k].x.shape[0]
+        if t is None:
+            t = int(self.rng.integers(HIST - 1, T - 1))
+        g, xn, xe, target = self.sample(k, t)
+        out = self.net.forward(g, xn, xe[:g.E], train=True)
+        loss, d = mse(out, target)
+        self.net.backward(g, xn, xe[:g.E], d)
+        self.net.adam(lr)
+        return loss
+
+    def evaluate(self, frames):
+        """Mean one-macro-step MSE of the network and of the no-network
+        baseline (a = gravity only) over (trajectory, frame) pairs."""
+        torch = _torch()
+        ln = lb = 0.
+        for k, t in frames:
+            g, xn, xe, target = self.sample(k, t)
+            out = self.net.forward(g, xn, xe[:g.E])
+            ln += mse(out, target)[0]
+            lb += mse(torch.zeros_like(target), target)[0]
+        return ln / len(frames), lb / len(frames)
+
+
+def dem_state_to(sysm, x, v, n):
+    """Resets a DEM context to positions x and velocities v (host arrays),
+    with spins and contact histories cleared."""
+    st = abi.ParticleState(n, max_cnts=sysm.cfg.max_contacts, max_walls=6)
+    st.x[:] = x
+    st.u[:] = v
+    sysm.upload(st, fields=("x", "u", "omega", "force", "torque", "sw_x", "sw_y", "sw_z", "ss_x", "ss_y",
+                            "ss_z", "ss_idx", "ss_cnt"))
+
+
+class Rollout:
+    """The surrogate advancing a DEM workload by macro-steps, with the FP64
+    DEM as judge and fallback: every `check_every` macro-steps the DEM
+    re-simulates the same macro-step from the same state; the mean position
+    error |x_surrogate - x_DEM| / r_mean is recorded and, above `tol`, the
+    run continues from the DEM state."""
+
+    def __init__(self, net: Network, wl, scales: Scales, check_every=10, tol=0.02):
+        torch = _torch()
+        self.net, self.wl, self.sc = net, wl, scales
+        self.check_every, self.tol = check_every, tol
+        n = wl.state.n
+        self.n = n
+        self.dem = wl.system()
+        self.builder = GraphBuilder(n, scales.radius, wl.lo, wl.hi)
+        self.planes = torch.tensor(np.asarray(wl.extra["planes"], dtype=np.float32).ravel(), device="cuda")
+        self.n_planes = len(wl.extra["planes"])
+        self.rad = torch.from_numpy(wl.state.rad.copy()).cuda()
+        self.r_mean = float(wl.state.rad.mean())
+        self.g_dev = torch.tensor(np.asarray(wl.gravity, dtype=np.float64), device="cuda")
+        self.Dt = wl.dt * MACRO
+        self.xn = torch.empty((n, FEAT), dtype=torch.bfloat16, device="cuda")
+        self.xe = None
+        self.st = abi.ParticleState(n, max_cnts=wl.max_contacts, max_walls=6)
+
+    def warm_up(self):
+        """HIST macro-steps of DEM to fill the velocity history."""
+        torch = _torch()
+        vh = []
+        for _ in range(HIST):
+            self.dem.step(MACRO)
+            self.dem.download(self.st, fields=("x", "u"))
+            vh.append(self.st.u.astype(np.float32))
+        self.x = torch.from_numpy(self.st.x.copy()).cuda()
+        self.vhist = torch.from_numpy(np.stack(vh)).cuda().contiguous()
+
+    def surrogate_step(self):
+        torch = _torch()
+        g = self.builder.build(self.x.cpu().numpy())
+        if self.xe is None or self.xe.shape[0] < max(g.E, 1):
+            self.xe = torch.empty((max(g.E, 1) * 5 // 4, FEAT), dtype=torch.bfloat16, device="cuda")
+        features(g, self.x, self.vhist, self.rad, self.planes, self.n_planes, self.sc, self.xn, self.xe)
+        out = self.net.forward(g, self.xn, self.xe[:g.E])
+        _ok(glib().gnn_integrate(self.n, ptr(out), self.sc.a, ptr(self.g_dev), self.Dt, ptr(self.x),
+                                 ptr(self.vhist), HIST, stream()), "integrate")
+
+    def dem_step_from(self, x, v):
+        dem_state_to(self.dem, x, v, self.n)
+        self.dem.step(MACRO)
+        self.dem.download(self.st, fields=("x", "u"))
+        return self.st.x.copy(), self.st.u.astype(np.float32)
+
+    def run(self, n_macro):
+        torch = _torch()
+        self.warm_up()
+        errs, fallbacks = [], 0
+        t_sur = t_dem = 0.
+        n_sur = n_dem = 0
+        for m in range(n_macro):
+            if m % self.check_every == 0:
+                x0 = self.x.cpu().numpy().copy()
+                v0 = self.vhist[-1].cpu().numpy().copy()
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                self.surrogate_step()
+                torch.cuda.synchronize()
+                t_sur += time.perf_counter() - t0
+                n_sur += 1
+                t0 = time.perf_counter()
+                xd, vd = self.dem_step_from(x0, v0)
+                t_dem += time.perf_counter() - t0
+                n_dem += 1
+                err = float(np.linalg.norm(self.x.cpu().numpy() - xd, axis=1).mean()) / self.r_mean
+                errs.append(err)
+                if err > self.tol:
+                    fallbacks += 1
+                    self.x.copy_(torch.from_numpy(xd))
+                    self.vhist[-1].copy_(torch.from_numpy(vd))
+            else:
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                self.surrogate_step()
+                torch.cuda.synchronize()
+                t_sur += time.perf_counter() - t0
+                n_sur += 1
+        return {"macro_steps": n_macro, "checks": len(errs), "fallbacks": fallbacks,
+                "mean_position_error_over_r": float(np.mean(errs)) if errs else None,
+                "max_position_error_over_r": float(np.max(errs)) if errs else None,
+                "surrogate_ms_per_macro": 1e3 * t_sur / max(n_sur, 1),
+                "dem_ms_per_macro": 1e3 * t_dem / max(n_dem, 1)}

@@ -84,6 +84,17 @@ typedef struct dem_config {
   int reserved;
 } dem_config;
 
+/* One system of an ensemble (dem_create_ensemble): its particle count,
+ * wall count, contact materials and binning box.  The other fields of
+ * dem_config (dt, cutoff, skin, capacities, gravity, rebuild cadence,
+ * physics mode) are common to the ensemble. */
+typedef struct dem_system_desc {
+  int n;
+  int n_walls;
+  double cor_pp, cor_pw, friction_pp, friction_pw;
+  double lo[3], hi[3];
+} dem_system_desc;
+
 /* Host view of the particle state in the reference layout
  * (fields of src/CabanaDEM_Particles.hpp:348-370).  NULL pointers are
  * skipped on upload/download.  max_cnts / max_walls give the row length of
@@ -261,6 +272,57 @@ int dem_set_external_force(dem_ctx* ctx, const double* f);
  * pulled end faces of a tensile test. */
 int dem_set_groups(dem_ctx* ctx, const int* group, int ngroups, const int* kind,
                    const double* vel);
+
+/* ---- ensembles (extension; no counterpart in the reference) --------------- */
+/* One context holding n_systems independent systems, advanced together: one
+ * fused kernel launch per step covers every system, and one neighbour
+ * rebuild re-sorts all of them (cell keys are system-major, so a system's
+ * particles never leave its slot range and never see another system).
+ * Particle arrays in the particle-view calls are the systems' particles
+ * concatenated in system order (dem_get_systems gives the offsets).  Each
+ * system evolves exactly as it would alone in its own context. */
+int dem_create_ensemble(const dem_config* common, int n_systems, const dem_system_desc* systems,
+                        dem_ctx** out);
+/* offsets: [n_systems + 1] first particle of each system, then the total. */
+int dem_get_systems(dem_ctx* ctx, int* n_systems, int* offsets);
+int dem_upload_system_walls(dem_ctx* ctx, int system, const dem_walls_view* w);
+int dem_set_system_params(dem_ctx* ctx, int system, double cor_pp, double cor_pw,
+                          double friction_pp, double friction_pw);
+/* out[s][5] = KE_trans, KE_rot, px, py, pz of each system, current state. */
+int dem_system_diagnostics(dem_ctx* ctx, double* out);
+/* Force-kernel launch shape: grid_blocks = 0 launches one workgroup per
+ * tile of 256 particles; > 0 launches a persistent grid of that many
+ * workgroups that stride over the tiles.  Results do not depend on it. */
+int dem_set_launch(dem_ctx* ctx, int grid_blocks);
+
+/* Captured step sequences: with enable set (and no diagnostics sampling or
+ * kernel timing on), dem_step runs whole cycles of 2 rebuild_every steps
+ * that start at a multiple of rebuild_every from one hipGraph, captured on
+ * first use and replayed after; the steps outside whole cycles run as
+ * usual.  Results are identical either way. */
+int dem_set_graph(dem_ctx* ctx, int enable);
+
+/* Autotuner: times trial_steps steps of every candidate configuration from
+ * the current state -- neighbour-rebuild cadence (5, 10, 20, 40, 80 and the
+ * configured one, when a skin is set; a candidate that breaks the skin
+ * check is rejected), force-kernel launch shape (one workgroup per tile, or
+ * persistent grids of 2, 4 and 8 workgroups per compute unit) and captured
+ * graphs on or off -- restores the state exactly, and applies the fastest.
+ * Rebuild cadence changes the summation order of the contact sums after a
+ * re-sort, so runs with different cadences agree to rounding, not bitwise. */
+#define DEM_TUNE_MAX_TRIALS 64
+typedef struct dem_tune_result {
+  int rebuild_every;   /* chosen configuration */
+  int grid_blocks;
+  int use_graph;
+  int n_trials;
+  double ms_per_step;  /* of the chosen configuration */
+  int trial_rebuild_every[DEM_TUNE_MAX_TRIALS];
+  int trial_grid_blocks[DEM_TUNE_MAX_TRIALS];
+  int trial_graph[DEM_TUNE_MAX_TRIALS];
+  double trial_ms_per_step[DEM_TUNE_MAX_TRIALS]; /* < 0: rejected */
+} dem_tune_result;
+int dem_autotune(dem_ctx* ctx, int trial_steps, dem_tune_result* out);
 
 /* ---- kernel timing (HIP events on the context's stream) ------------------ */
 /* Enables per-launch HIP-event timing of the kernels dem_step launches. */

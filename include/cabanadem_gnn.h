@@ -1,0 +1,112 @@
+/*
+ * cabanadem_gnn.h -- C ABI of the learned-surrogate kernels
+ * (libcabanadem_gnn.so, built from cabanadem_amd/csrc/gnn.hip).
+ *
+ * No counterpart in the reference: the surrogate is an addition on top of
+ * the DEM path.  A message-passing network over the DEM neighbour graph
+ * predicts per-particle accelerations; these entry points are its building
+ * blocks (fused two-layer MLPs on the BF16 MFMA units with FP32 accumulation,
+ * their backward passes, weight gradients, the graph aggregation, Adam).
+ * The host side that strings them into a network, trains it and runs it
+ * against the DEM is cabanadem_amd/surrogate.py.
+ *
+ * Conventions: all pointers are device pointers; BF16 arrays are bf16
+ * (2 bytes, round-to-nearest-even from FP32), FP32 arrays are float.
+ * Feature widths are multiples of 32 (pad with zeros); the latent width is
+ * 128.  Calls enqueue on `stream` (a hipStream_t, NULL = default stream) and
+ * return 0, -1 on a bad argument, -2 on a launch failure.
+ */
+#ifndef CABANADEM_GNN_H
+#define CABANADEM_GNN_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GNN_ABI_VERSION 1
+#define GNN_LATENT 128
+
+/* Input rows of an MLP: the concatenation of up to three BF16 sources;
+ * source q contributes width[q] features of row idx[q][m] (or row m when
+ * idx[q] is NULL). */
+typedef struct gnn_mlp_io {
+  int M;               /* rows */
+  int nseg;            /* sources, 1..3 */
+  const void* src[3];  /* bf16 [rows_q][width_q] */
+  const int* idx[3];   /* [M] row of source q for MLP row m, or NULL */
+  int width[3];        /* multiples of 32 */
+} gnn_mlp_io;
+
+/* y = W2 relu(W1 x + b1) + b2, x of width K, hidden 128, output nout
+ * (padded to nout_pad = 16 or 128 in the BF16 copies).  W1t/W2t are the
+ * transposed copies used by the forward pass, W1/W2 the row-major copies
+ * used by the backward pass. */
+typedef struct gnn_mlp_weights {
+  int K, nout, nout_pad;
+  const void* W1t;     /* bf16 [128][K] */
+  const void* W1;      /* bf16 [K][128] */
+  const float* b1;     /* [128] */
+  const void* W2t;     /* bf16 [nout_pad][128] */
+  const void* W2;      /* bf16 [128][nout_pad] */
+  const float* b2;     /* [nout_pad] */
+} gnn_mlp_weights;
+
+int gnn_abi_version(void);
+
+/* out = residual + MLP(x); hidden (bf16 [M][128], optional) keeps relu(W1 x + b1)
+ * for the backward pass; out (bf16 [M][nout]) and/or out_f32 (float [M][nout]).
+ * grid_cap > 0 limits the workgroups (persistent tiles of 128 rows). */
+int gnn_mlp_forward(const gnn_mlp_io* io, const gnn_mlp_weights* w, const void* residual,
+                    void* hidden, void* out, float* out_f32, int grid_cap, void* stream);
+/* dH (bf16 [M][128]) = (dY W2^T) masked by hidden > 0; dX[q] (float
+ * [M][width_q], NULL to skip) = (dH W1^T) restricted to source q, stored
+ * or, with accumulate, added. */
+int gnn_mlp_backward(const gnn_mlp_io* io, const gnn_mlp_weights* w, const float* dY,
+                     const void* hidden, void* dH, float* const dX[3], int accumulate,
+                     int grid_cap, void* stream);
+/* dW (float [K][ncols_pad]) = X^T G and db (float [ncols_pad]) = column sums
+ * of G, with G bf16 or float [M][ncols]; part/bpart are scratch of
+ * nchunks*K*ncols_pad and nchunks*ncols_pad floats (the rows are split into
+ * nchunks chunks reduced in chunk order, so results are deterministic). */
+int gnn_weight_grad(const gnn_mlp_io* io, const void* G_bf16, const float* G_f32, int ncols,
+                    int ncols_pad, float* part, float* bpart, int nchunks, float* dW, float* db,
+                    int accumulate, void* stream);
+/* agg[i] (bf16 [N][128]) = sum of x[e] (bf16 [E][128]) over the edges
+ * e in [row[i], row[i+1]), summed in FP32. */
+int gnn_segment_sum(const void* x, const int* row, int N, void* agg, void* stream);
+/* dh[i] += sum over the edges e of i of d_recv[e] + d_send[rev[e]] (float
+ * [E][128] each, either may be NULL). */
+int gnn_edge_grad_to_nodes(const float* d_recv, const float* d_send, const int* row,
+                           const int* rev, int N, float* dh, void* stream);
+/* d_e[e] += d_agg[recv[e]] (float [E][128], float [N][128]). */
+int gnn_broadcast_add(const float* d_agg, const int* recv, long E, float* d_e, void* stream);
+/* Adam with decoupled bias correction folded into lr_t; wd = L2 factor. */
+int gnn_adam(float* p, const float* g, float* m, float* v, long n, float lr_t, float b1,
+             float b2, float eps, float wd, void* stream);
+/* FP32 master [K][NP] (columns >= N are zero) -> bf16 W [K][NP] and Wt [NP][K]. */
+int gnn_cast_weights(const float* p, int K, int N, int NP, void* W, void* Wt, void* stream);
+/* Node features (bf16 [N][32]): the H most recent velocities vhist
+ * (float [H][N][3], oldest first) / v_scale, (r - r_mean) / r_scale, and
+ * for each plane (float [n_planes][4] = unit normal, offset d) the distance
+ * n . x - d / dist_scale clipped to [0, 1]; zero padding.  3H + 1 + n_planes <= 32. */
+int gnn_node_features(int N, const double* x, const float* vhist, int H, const double* rad,
+                      float v_scale, float r_mean, float r_scale, const float* planes, int n_planes,
+                      float dist_scale, void* out, void* stream);
+/* Edge features (bf16 [E][32]) of edge send[e] -> recv[e]: (x_i - x_j) / radius
+ * and |x_i - x_j| / radius; zero padding. */
+int gnn_edge_features(long E, const double* x, const int* recv, const int* send, double radius,
+                      void* out, void* stream);
+/* Surrogate macro-step: v = v_last + (a a_scale + g) dt, x += v dt; the
+ * velocity history shifts by one.  a: float [N][3] (network output),
+ * gravity: double[3] on the device. */
+int gnn_integrate(int N, const float* a, float a_scale, const double* gravity, double dt, double* x,
+                  float* vhist, int H, void* stream);
+/* dY = 2 (y - t) / count, part[b] = partial sums of (y - t)^2 per block of 256. */
+int gnn_mse(const float* y, const float* t, long count, float* dY, float* part, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -465,8 +465,18 @@ void oracle_force_pw(OracleParticles* p, const OracleWalls* wl, const OraclePara
           double fnm = std::sqrt(fn0 * fn0 + fn1 * fn1 + fn2 * fn2); // :636
           double ftms = std::min(prm->fr_pw * fnm, ftm);       // :638
           ft0 = ftms * ti0; ft1 = ftms * ti1; ft2 = ftms * ti2; // :641-643
-          ft0 += eta_t * vt0; ft1 += eta_t * vt1; ft2 += eta_t * vt2; // :646-648
-          *swx = -ft0 / kt; *swy = -ft1 / kt; *swz = -ft2 / kt; // :651-653
+          if (ref) {
+            // :646-648 add +eta_t v_t (along the slip; quirk Q16, inert with
+            // eta_t = 0), then reset the spring to the applied force (:651-653)
+            ft0 += eta_t * vt0; ft1 += eta_t * vt1; ft2 += eta_t * vt2;
+            *swx = -ft0 / kt; *swy = -ft1 / kt; *swz = -ft2 / kt;
+          } else {
+            // corrected: the limited force holds the damping already; the
+            // spring keeps its elastic part
+            *swx = -(ft0 + eta_t * vt0) / kt;
+            *swy = -(ft1 + eta_t * vt1) / kt;
+            *swz = -(ft2 + eta_t * vt2) / kt;
+          }
         }
         p->force[3 * i + 0] += fn0 + ft0;                      // :657-659
         p->force[3 * i + 1] += fn1 + ft1;

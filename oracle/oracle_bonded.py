@@ -1,4 +1,7 @@
-"""Bonded-DEM step loop over the CPU oracle (test helper).
+"""Bonded-DEM step loop over the CPU oracle.
+
+TEST INFRASTRUCTURE ONLY (tests/, smoke and the benchmark's CPU baseline):
+the product path never imports it.
 
 Order of the reference's bonded driver
 (examples/tst_01_BDEM_two_particles_tensile_3D.cpp:297-309): stage1, stage2,
@@ -43,8 +46,9 @@ class _Bound:
 
 
 def bonded_step(st, bonds, *, prm, bp, dt, fext=None, vg=None, vel=None, ff=None,
-                contacts=False, rc=None, lo=None, hi=None, walls=None, bound=None):
-    """One step; returns the number of bonds broken in it."""
+                contacts=False, rc=None, lo=None, hi=None, walls=None, bound=None, nbrs=None):
+    """One step; returns the number of bonds broken in it.  nbrs: a neighbour
+    list (counts, ids) to reuse instead of building one for this step."""
     B = bound if bound is not None else _Bound(st, bonds, prm, bp)
     L = B.L
     L.oracle_stage1(B.p, dt)
@@ -57,7 +61,8 @@ def bonded_step(st, bonds, *, prm, bp, dt, fext=None, vg=None, vel=None, ff=None
     nb = O.C.c_int(0)
     L.oracle_bond_force(B.p, B.b, B.bp, dt, O.C.byref(nb))
     if contacts:
-        counts, ids = O.neighbors_cell(st.x, rc, lo, hi)
+        counts, ids = nbrs if nbrs is not None else O.neighbors_cell(st.x, rc, lo, hi)
+        B.last_nbrs = (counts, ids)
         L.oracle_update_tangential(B.p)
         ov = L.oracle_force_pp_unbonded(B.p, O._p(counts, O.C.c_int), O._p(ids, O.C.c_int),
                                         ids.shape[1], B.prm, B.b)
@@ -73,7 +78,8 @@ def bonded_step(st, bonds, *, prm, bp, dt, fext=None, vg=None, vel=None, ff=None
 
 
 def bonded_run(st, bonds, *, nsteps, prm, bp, dt, fext=None, group=None, kinds=None, vel=None,
-               contacts=False, rc=None, lo=None, hi=None, walls=None, record=None):
+               contacts=False, rc=None, lo=None, hi=None, walls=None, record=None,
+               rebuild_every=1):
     vg, ff = group_masks(group, kinds, st.x.shape[0])
     if vel is not None:
         vel = np.ascontiguousarray(np.asarray(vel, dtype=np.float64).reshape(-1, 3))
@@ -81,9 +87,15 @@ def bonded_run(st, bonds, *, nsteps, prm, bp, dt, fext=None, group=None, kinds=N
         fext = np.asarray(fext, dtype=np.float64)
     B = _Bound(st, bonds, prm, bp)
     broken = 0
+    nbrs = None
     for step in range(nsteps):
+        if contacts and rebuild_every > 1 and step % rebuild_every == 0:
+            nbrs = None
         broken += bonded_step(st, bonds, prm=prm, bp=bp, dt=dt, fext=fext, vg=vg, vel=vel, ff=ff,
-                              contacts=contacts, rc=rc, lo=lo, hi=hi, walls=walls, bound=B)
+                              contacts=contacts, rc=rc, lo=lo, hi=hi, walls=walls, bound=B,
+                              nbrs=nbrs)
+        if contacts and rebuild_every > 1 and nbrs is None:
+            nbrs = B.last_nbrs
         if record is not None:
             record(step, st, bonds)
     return broken

@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 import oracle_py as O
-from bonded_driver import GROUP_FIXED_FORCE, GROUP_VELOCITY, bonded_run
+from oracle_bonded import GROUP_FIXED_FORCE, GROUP_VELOCITY, bonded_run
 from cabanadem_amd import abi
 from gpu_helpers import make_system, normwise, oracle_params
 from scenarios import bonded_block, tst01_bonded

@@ -99,7 +99,7 @@ def test_bonded_tensile_dropin_matches_oracle(gpu_lib, tmp_path, mode, final_tim
     bonded API (examples/bonded_tensile.cpp): the reference's staged loop with
     host-side force edits, and the fused solver with the same physics set once."""
     import oracle_py as O
-    from bonded_driver import bonded_run
+    from oracle_bonded import bonded_run
 
     assert BONDED_EXE.exists(), "build() compiles examples/_build/bonded_tensile"
     cmd = [str(BONDED_EXE), mode, repr(final_time), str(tmp_path)]

@@ -25,7 +25,7 @@ import numpy as np
 import pytest
 
 import oracle_py as O
-from bonded_driver import GROUP_FIXED_FORCE, GROUP_VELOCITY, bonded_run
+from oracle_bonded import GROUP_FIXED_FORCE, GROUP_VELOCITY, bonded_run
 from cabanadem_amd.abi import ParticleState
 from scenarios import _sphere, bonded_block, tst01_bonded
 
