@@ -253,14 +253,125 @@ def run_ensemble(args):
     print(json.dumps(out), flush=True)
 
 
+def forward_flops(L, N, E):
+    """Multiply-add FLOPs of one network evaluation (2 per MAC)."""
+    lat = 128
+    enc = N * (32 * lat + lat * lat) + E * (32 * lat + lat * lat)
+    layers = L * (E * (3 * lat * lat + lat * lat) + N * (2 * lat * lat + lat * lat))
+    dec = N * (lat * lat + lat * 16)
+    return 2.0 * (enc + layers + dec)
+
+
+def run_surrogate(args):
+    """Learned surrogate on the 64k-sphere column collapse: training on DEM
+    trajectories, one-macro-step error against the no-network baseline, and
+    the judged roll-out's speed and error against the plain GPU DEM."""
+    import torch
+    from cabanadem_amd import surrogate as S
+    from cabanadem_amd import workloads as W
+
+    t0 = time.perf_counter()
+    shapes = [(20, 20, 20), (16, 20, 25), (25, 20, 16), (20, 16, 25)]
+    n_traj, frames = args.surrogate_traj, args.surrogate_frames
+    trajs = [S.generate(W.column(dims=shapes[k % len(shapes)], seed=100 + k), frames, log=log)
+             for k in range(n_traj)]
+    t_gen = time.perf_counter() - t0
+    radius = 2.5e-3
+    sc = S.fit_scales(trajs[:-1], radius)
+    net = S.Network(n_layers=10, seed=0)
+    tr = S.Trainer(net, trajs[:-1], sc, seed=0)
+    held = S.Trainer(net, trajs[-1:], sc, seed=1)
+    rng = np.random.default_rng(5)
+    eval_frames = [(0, int(t)) for t in rng.integers(S.HIST - 1, frames - 1, size=16)]
+    before, baseline = held.evaluate(eval_frames)
+    t0 = time.perf_counter()
+    losses = []
+    for it in range(args.surrogate_steps):
+        lr = 1e-3 * (0.1 ** (it / max(args.surrogate_steps, 1)))
+        losses.append(tr.step(lr))
+        if (it + 1) % 500 == 0:
+            log(f"[surrogate] step {it + 1}: loss {np.mean(losses[-500:]):.4f}")
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    after, _ = held.evaluate(eval_frames)
+    log(f"[surrogate] held-out one-step MSE (units of a_scale^2): untrained {before:.4f}, trained "
+        f"{after:.4f}, no network {baseline:.4f}; training {t_train:.1f} s")
+
+    # forward-pass MFMA rate on the 64k column's graph
+    wl = W.column(dims=(40, 40, 40), seed=7)
+    ro = S.Rollout(net, wl, sc, check_every=10, tol=0.02)
+    ro.warm_up()
+    g = ro.builder.build(ro.x.cpu().numpy())
+    xe = torch.empty((g.E, S.FEAT), dtype=torch.bfloat16, device="cuda")
+    S.features(g, ro.x, ro.vhist, ro.rad, ro.planes, ro.n_planes, sc, ro.xn, xe)
+    for _ in range(2):
+        net.forward(g, ro.xn, xe)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 10
+    e0.record()
+    for _ in range(reps):
+        net.forward(g, ro.xn, xe)
+    e1.record()
+    torch.cuda.synchronize()
+    fwd_ms = e0.elapsed_time(e1) / reps
+    flops = forward_flops(10, g.N, g.E)
+    mfma_tf = flops / (fwd_ms * 1e-3) / 1e12
+    ro.dem.close()
+
+    # judged roll-out against a DEM reference over the same horizon
+    n_macro = args.surrogate_macro
+    ro = S.Rollout(net, wl, sc, check_every=10, tol=0.02)
+    res = ro.run(n_macro)
+    x_sur = ro.x.cpu().numpy()
+    ref = wl.system()
+    tt = time.perf_counter()
+    ref.step(S.MACRO * (S.HIST + n_macro))
+    ref.sync()
+    dem_wall = time.perf_counter() - tt
+    st = ref.download(fields=("x",))
+    ref.close()
+    horizon_err = float(np.linalg.norm(x_sur - st.x, axis=1).mean() / wl.state.rad.mean())
+    dem_ms_macro = dem_wall * 1e3 / (S.HIST + n_macro)
+    sur_ms = res["surrogate_ms_per_macro"]
+    speed_no_checks = dem_ms_macro / sur_ms
+    k = ro.check_every
+    speed_checks = dem_ms_macro * k / (sur_ms * k + res["dem_ms_per_macro"])
+    out = {
+        "metric": "surrogate speedup at <=2% radius error (64k-sphere column collapse)",
+        "value": speed_checks, "unit": "x (simulated time per wall-second, surrogate with DEM checks "
+                                      "/ plain GPU DEM)",
+        "n_gpus": 1, "steps": n_macro, "warmup": S.HIST, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16 (f32 accumulate) network, f64 DEM",
+        "data": "synthetic: DEM column-collapse trajectories generated on the GPU",
+        "config": {"workload": "column collapse 40x40x40 = 64000 spheres, macro-step 100 DEM steps, "
+                               "DEM check every 10 macro-steps",
+                   "network": "10 message-passing layers, latent 128, radius 2.5 mm graph",
+                   "training": f"{n_traj - 1} trajectories x {frames} frames (8-10k spheres), "
+                               f"{args.surrogate_steps} Adam steps; 1 held-out trajectory",
+                   "parallelism": "single GPU"},
+        "speedup_without_checks": speed_no_checks,
+        "rollout": dict(res, horizon_mean_position_error_over_r=horizon_err,
+                        dem_ms_per_macro_reference_run=dem_ms_macro),
+        "one_step_mse_held_out": {"untrained": before, "trained": after, "no_network": baseline},
+        "mfma": {"forward_ms": fwd_ms, "nodes": g.N, "edges": g.E, "flops": flops,
+                 "achieved_tflops": mfma_tf, "peak_tflops": 2500.0, "frac": mfma_tf / 2500.0},
+        "times_s": {"data_generation": t_gen, "training": t_train},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--dims", type=str, default=None)
-    ap.add_argument("--workload", choices=("plain_large", "bonded_block", "ensemble"),
+    ap.add_argument("--workload", choices=("plain_large", "bonded_block", "ensemble", "surrogate"),
                     default="plain_large")
+    ap.add_argument("--surrogate-traj", type=int, default=5)
+    ap.add_argument("--surrogate-frames", type=int, default=300)
+    ap.add_argument("--surrogate-steps", type=int, default=3000)
+    ap.add_argument("--surrogate-macro", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=str, default="200,40")
     ap.add_argument("--mode", type=int, default=None,
@@ -281,6 +392,8 @@ def main():
         args.mode = 1 if args.mode is None else args.mode
         return run_bonded(args)
     args.mode = 0 if args.mode is None else args.mode
+    if args.workload == "surrogate":
+        return run_surrogate(args)
     if args.workload == "ensemble":
         args.steps = 5000 if args.steps is None else args.steps
         args.warmup = 0 if args.warmup is None else args.warmup

@@ -180,6 +180,7 @@ struct dem_ctx {
     for (int d = 0; d < 3; ++d) a.g[d] = cfg.gravity[d];
     a.gravity_on = (cfg.gravity[0] != 0. || cfg.gravity[1] != 0. || cfg.gravity[2] != 0.) ? 1 : 0;
     a.sys = d_sys;
+    a.sys0 = h_sys[0];
     a.tiles = nsys > 1 ? d_tiles : nullptr;
     a.ntiles = ntiles;
     a.xr_in = xr[cP]; a.xr_out = xr[cP ^ 1];
@@ -1420,6 +1421,7 @@ int dem_get_timing(dem_ctx* c, double ms[2], int64_t launches[2]) {
 int dem_enable_bonds(dem_ctx* c, const dem_bond_config* b) {
   if (int r = check_ctx(c)) return r;
   if (!b) return c->fail(DEM_ERR_ARG, "null bond config");
+  if (c->nsys > 1) return c->fail(DEM_ERR_STATE, "bonds are not available in an ensemble context");
   if (b->max_bonds <= 0 || b->max_bonds > DEM_MAX_BONDS)
     return c->fail(DEM_ERR_ARG, "max_bonds must be in [1, 32]");
   if (!(b->beta_bond >= 0.) || !(b->radius_factor > 0.))

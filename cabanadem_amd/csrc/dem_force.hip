@@ -371,16 +371,18 @@ __device__ inline void block_reduce8(const double* v, double* part, int blk) {
 
 // KICK: 0 = forces only (write F,T); 1 = + stage3 (write state, F,T);
 //       2 = + stage3 + stage1 + stage2 of the next step (write state).
-// One tile: kBlock consecutive slots of one system.
-template <bool REF, bool PP, bool PW, int KICK, bool ACCUM, bool DIAG, bool BOND>
+// One tile: kBlock consecutive slots of one system.  ENS: an ensemble (tile
+// table and per-system parameters from the device); otherwise one system
+// whose parameters travel in the kernel arguments.
+template <bool REF, bool PP, bool PW, int KICK, bool ACCUM, bool DIAG, bool BOND, bool ENS>
 __device__ __forceinline__ void force_tile(const StepArgs& a, const int tile, int* s_cj) {
   const int tid = threadIdx.x;
   int sys = 0, first = tile * kBlock, end = a.n;
-  if (a.tiles) {
+  if (ENS) {
     const int4 t = a.tiles[tile];
     sys = t.x; first = t.y; end = t.z;
   }
-  const SysParams& P = a.sys[sys];
+  const SysParams& P = ENS ? a.sys[sys] : a.sys0;
   const int i = first + tid;
   double dg[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
   if (i < end) {
@@ -578,11 +580,17 @@ __device__ __forceinline__ void force_tile(const StepArgs& a, const int tile, in
 // The fused force / step kernel.  A grid of one workgroup per tile, or a
 // persistent grid that strides over the tiles; either way tile v is mapped
 // XCD-aware (contiguous tiles per XCD when the grid is a multiple of 8).
-template <bool REF, bool PP, bool PW, int KICK, bool ACCUM, bool DIAG, bool BOND>
+template <bool REF, bool PP, bool PW, int KICK, bool ACCUM, bool DIAG, bool BOND, bool ENS>
 __global__ __launch_bounds__(kBlock, DEM_FORCE_MIN_WAVES) void k_force(StepArgs a) {
   extern __shared__ int s_cj[]; // [max_c][kBlock] contacts found in phase 1
+  if ((int)gridDim.x >= a.ntiles) {
+    // one workgroup per tile: no loop, so the compiler keeps the kernel
+    // arguments where they are used instead of live across a loop
+    force_tile<REF, PP, PW, KICK, ACCUM, DIAG, BOND, ENS>(a, xcd_block(blockIdx.x, a.ntiles), s_cj);
+    return;
+  }
   for (int v = blockIdx.x; v < a.ntiles; v += gridDim.x) {
-    force_tile<REF, PP, PW, KICK, ACCUM, DIAG, BOND>(a, xcd_block(v, a.ntiles), s_cj);
+    force_tile<REF, PP, PW, KICK, ACCUM, DIAG, BOND, ENS>(a, xcd_block(v, a.ntiles), s_cj);
     if (DIAG) __syncthreads(); // the reduction's LDS is reused by the next tile
   }
 }
@@ -701,7 +709,7 @@ __global__ __launch_bounds__(kBlock) void k_diag_final(const double* part, int n
 
 inline unsigned nblk(long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
-template <bool REF, bool BOND>
+template <bool REF, bool BOND, bool ENS>
 hipError_t launch_force_mode(ForceKind kind, const StepArgs& a, bool sample, int grid_blocks,
                              hipStream_t s) {
   const size_t lds = (size_t)(a.max_c > 0 ? a.max_c : 1) * kBlock * sizeof(int);
@@ -709,25 +717,28 @@ hipError_t launch_force_mode(ForceKind kind, const StepArgs& a, bool sample, int
   const dim3 g(nb), b(kBlock);
   switch (kind) {
     case FK_PP_PW_FORCES:
-      if (sample) hipLaunchKernelGGL((k_force<REF, true, true, 0, false, true, BOND>), g, b, lds, s, a);
-      else hipLaunchKernelGGL((k_force<REF, true, true, 0, false, false, BOND>), g, b, lds, s, a);
+      if (sample) hipLaunchKernelGGL((k_force<REF, true, true, 0, false, true, BOND, ENS>), g, b, lds, s, a);
+      else hipLaunchKernelGGL((k_force<REF, true, true, 0, false, false, BOND, ENS>), g, b, lds, s, a);
       break;
     case FK_PP_ACCUM:
-      hipLaunchKernelGGL((k_force<REF, true, false, 0, true, false, false>), g, b, lds, s, a);
+      hipLaunchKernelGGL((k_force<REF, true, false, 0, true, false, false, ENS>), g, b, lds, s, a);
       break;
     case FK_PW_ACCUM:
-      hipLaunchKernelGGL((k_force<REF, false, true, 0, true, false, false>), g, b, lds, s, a);
+      hipLaunchKernelGGL((k_force<REF, false, true, 0, true, false, false, ENS>), g, b, lds, s, a);
       break;
     case FK_BOND_ACCUM:
-      hipLaunchKernelGGL((k_force<REF, false, false, 0, true, false, true>), g, b, lds, s, a);
+      if constexpr (!ENS)
+        hipLaunchKernelGGL((k_force<REF, false, false, 0, true, false, true, false>), g, b, lds, s, a);
+      else
+        return hipErrorInvalidValue;
       break;
     case FK_STEP_FUSED:
-      if (sample) hipLaunchKernelGGL((k_force<REF, true, true, 2, false, true, BOND>), g, b, lds, s, a);
-      else hipLaunchKernelGGL((k_force<REF, true, true, 2, false, false, BOND>), g, b, lds, s, a);
+      if (sample) hipLaunchKernelGGL((k_force<REF, true, true, 2, false, true, BOND, ENS>), g, b, lds, s, a);
+      else hipLaunchKernelGGL((k_force<REF, true, true, 2, false, false, BOND, ENS>), g, b, lds, s, a);
       break;
     case FK_STEP_LAST:
-      if (sample) hipLaunchKernelGGL((k_force<REF, true, true, 1, false, true, BOND>), g, b, lds, s, a);
-      else hipLaunchKernelGGL((k_force<REF, true, true, 1, false, false, BOND>), g, b, lds, s, a);
+      if (sample) hipLaunchKernelGGL((k_force<REF, true, true, 1, false, true, BOND, ENS>), g, b, lds, s, a);
+      else hipLaunchKernelGGL((k_force<REF, true, true, 1, false, false, BOND, ENS>), g, b, lds, s, a);
       break;
   }
   return hipGetLastError();
@@ -740,10 +751,15 @@ int force_blocks(int n) { return (int)nblk(n); }
 hipError_t launch_force(ForceKind kind, const StepArgs& a, bool sample, int grid_blocks,
                         hipStream_t s) {
   const bool ref = a.physics_mode == DEM_MODE_REFERENCE;
-  if (a.nbond) return ref ? launch_force_mode<true, true>(kind, a, sample, grid_blocks, s)
-                          : launch_force_mode<false, true>(kind, a, sample, grid_blocks, s);
-  return ref ? launch_force_mode<true, false>(kind, a, sample, grid_blocks, s)
-             : launch_force_mode<false, false>(kind, a, sample, grid_blocks, s);
+  if (a.tiles) { // ensembles (no bonds)
+    if (a.nbond) return hipErrorInvalidValue;
+    return ref ? launch_force_mode<true, false, true>(kind, a, sample, grid_blocks, s)
+               : launch_force_mode<false, false, true>(kind, a, sample, grid_blocks, s);
+  }
+  if (a.nbond) return ref ? launch_force_mode<true, true, false>(kind, a, sample, grid_blocks, s)
+                          : launch_force_mode<false, true, false>(kind, a, sample, grid_blocks, s);
+  return ref ? launch_force_mode<true, false, false>(kind, a, sample, grid_blocks, s)
+             : launch_force_mode<false, false, false>(kind, a, sample, grid_blocks, s);
 }
 
 hipError_t launch_zero_forces(const StepArgs& a, hipStream_t s) {

@@ -110,7 +110,9 @@ struct StepArgs {
   double dt, hdt;
   double g[3];
   int gravity_on;
-  const SysParams* sys;  // [n_systems]
+  const SysParams* sys;  // [n_systems] (ensembles)
+  SysParams sys0;        // the single system's parameters, by value: kernel-argument
+                         // loads are scalar and can be repeated at will
   // Tiles of kBlock slots, each inside one system: (system, first, end, -).
   // Null for a single system (tile t = slots [t kBlock, min((t+1) kBlock, n))).
   const int4* tiles;

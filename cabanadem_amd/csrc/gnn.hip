@@ -188,7 +188,7 @@ struct BwdArgs {
   float* dX[3];      // [M][width_q] gradient of each input source, or null
   int width[3];
   int start[4];
-  int accumulate;    // dX += instead of =
+  int accumulate;    // bit q: dX[q] += instead of =
 };
 
 // dH = (dY W2^T) * [hid > 0];  dX = dH W1^T, split by source.
@@ -296,7 +296,7 @@ __global__ __launch_bounds__(kThreads) void k_mlp_bwd(BwdArgs a) {
             const int grow = r0 + rb * 16 + 4 * (lane >> 4) + r;
             if (grow >= a.M) continue;
             float* p = dst + (long)grow * wq + cc;
-            *p = a.accumulate ? *p + acc2[rb][cb][r] : acc2[rb][cb][r];
+            *p = ((a.accumulate >> q) & 1) ? *p + acc2[rb][cb][r] : acc2[rb][cb][r];
           }
         }
     }
@@ -332,7 +332,10 @@ __global__ __launch_bounds__(kThreads) void k_wgrad(WgradArgs a) {
   f32x4 acc[8];
 #pragma unroll
   for (int cb = 0; cb < 8; ++cb) acc[cb] = f32x4{0.f, 0.f, 0.f, 0.f};
-  float bsum = 0.f; // column sum of G for column threadIdx.x (kb == 0 blocks)
+  // column sums of G (bias gradient, kb == 0 blocks), in FP32 from the
+  // source values: thread t always loads the 8 columns starting at
+  // 8 (t mod NC/8), since the block size is a multiple of NC/8
+  float csum[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   for (int m0 = m_begin; m0 < m_end; m0 += 32) {
     // X tile: 32 rows x 64 k; thread t loads 8 k of row t / 8
     {
@@ -357,12 +360,10 @@ __global__ __launch_bounds__(kThreads) void k_wgrad(WgradArgs a) {
         if (m < m_end && nn + q < a.ncols_g)
           g = a.gb ? (float)a.gb[(long)m * a.ncols_g + nn + q] : a.gf[(long)m * a.ncols_g + nn + q];
         sg[nn + q][rr] = (bf16)g;
+        csum[q] += g;
       }
     }
     __syncthreads();
-    if (kb == 0 && threadIdx.x < a.NC) {
-      for (int rr = 0; rr < 32; ++rr) bsum += (float)sg[threadIdx.x][rr];
-    }
     const bf16x8 av = *reinterpret_cast<const bf16x8*>(&sx[w * 16 + lr][lk]);
 #pragma unroll
     for (int cb = 0; cb < 8; ++cb) {
@@ -382,7 +383,19 @@ __global__ __launch_bounds__(kThreads) void k_wgrad(WgradArgs a) {
       if (k < a.K) out[(long)k * a.NC + cb * 16 + lr] = acc[cb][r];
     }
   }
-  if (kb == 0 && threadIdx.x < a.NC) a.bpart[(long)ch * a.NC + threadIdx.x] = bsum;
+  if (kb == 0) {
+    // reduce the per-thread column sums in a fixed order
+    __shared__ float red[kThreads][8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) red[threadIdx.x][q] = csum[q];
+    __syncthreads();
+    if (threadIdx.x < a.NC) {
+      const int groups = a.NC / 8, grp = threadIdx.x / 8, q = threadIdx.x % 8;
+      float s = 0.f;
+      for (int t = grp; t < kThreads; t += groups) s += red[t][q];
+      a.bpart[(long)ch * a.NC + threadIdx.x] = s;
+    }
+  }
 }
 
 // out[i] = sum over chunks of part[c][i], in chunk order.

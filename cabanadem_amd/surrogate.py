@@ -222,9 +222,11 @@ class Network:
         off = 0
         self.enc_n = MLP(FEAT, LAT, off); off = self.enc_n.end
         self.enc_e = MLP(FEAT, LAT, off); off = self.enc_e.end
+        # parameter order: encoders, the L edge MLPs, the L node MLPs, decoder
         self.edge, self.node = [], []
         for _ in range(n_layers):
             m = MLP(3 * LAT, LAT, off); off = m.end; self.edge.append(m)
+        for _ in range(n_layers):
             m = MLP(2 * LAT, LAT, off); off = m.end; self.node.append(m)
         self.dec = MLP(LAT, 3, off); off = self.dec.end
         self.mlps = [self.enc_n, self.enc_e] + self.edge + self.node + [self.dec]
@@ -367,6 +369,7 @@ class Network:
             # node MLP: h_{l+1} = h_l + MLP_n([h_l, agg_l])
             inp = io(N, (h[:N], None), (self.agg[l], None))
             self._wgrad_w2_only(self.node[l], self.hid_n[l], d_h, N)
+            # accumulate bit 0 only: the residual source adds into d_h, d_agg is overwritten
             bwd(self.node[l], inp, d_h, self.hid_n[l], self.dH_n[:N], [d_h, self.d_agg[:N], None], 1, N)
             self._wgrad_w1_only(self.node[l], inp, self.dH_n[:N])
             if E > 0:

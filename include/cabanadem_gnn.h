@@ -61,8 +61,8 @@ int gnn_abi_version(void);
 int gnn_mlp_forward(const gnn_mlp_io* io, const gnn_mlp_weights* w, const void* residual,
                     void* hidden, void* out, float* out_f32, int grid_cap, void* stream);
 /* dH (bf16 [M][128]) = (dY W2^T) masked by hidden > 0; dX[q] (float
- * [M][width_q], NULL to skip) = (dH W1^T) restricted to source q, stored
- * or, with accumulate, added. */
+ * [M][width_q], NULL to skip) = (dH W1^T) restricted to source q, stored,
+ * or added when bit q of accumulate is set. */
 int gnn_mlp_backward(const gnn_mlp_io* io, const gnn_mlp_weights* w, const float* dY,
                      const void* hidden, void* dH, float* const dX[3], int accumulate,
                      int grid_cap, void* stream);
