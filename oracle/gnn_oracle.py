@@ -53,8 +53,11 @@ def unpack(flat, L):
 
 
 class Net:
-    def __init__(self, flat, L, quant=True):
-        self.L, self.quant = L, quant
+    def __init__(self, flat, L, quant=True, quant_grad=False):
+        """quant_grad: round the backward pass's GEMM operands to BF16 where
+        the GPU does (dY entering the hidden-layer product and the weight
+        gradient of the second layer, the hidden gradient dH)."""
+        self.L, self.quant, self.quant_grad = L, quant, quant_grad
         self.flat = np.asarray(flat, dtype=np.float64)
         q = bf16 if quant else (lambda a: np.asarray(a, dtype=np.float64))
         self.q = q
@@ -100,9 +103,12 @@ class Net:
     def _mlp_bwd(self, k, x, hid, dy, grads):
         W1, b1, W2, b2, NP = self.mlps[k]
         K, nout = W1.shape[0], W2.shape[1]
-        gW2 = hid.T @ dy
+        dyq = bf16(dy) if self.quant_grad else dy
+        gW2 = hid.T @ dyq
         gb2 = dy.sum(0)
-        dh = (dy @ W2.T) * (hid > 0)
+        dh = (dyq @ W2.T) * (hid > 0)
+        if self.quant_grad:
+            dh = bf16(dh)
         gW1 = x.T @ dh
         gb1 = dh.sum(0)
         dx = dh @ W1.T

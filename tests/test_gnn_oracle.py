@@ -89,3 +89,44 @@ def test_padding_carries_no_gradient():
     K, nout, NP, o = G.layout(L)[0][-1]
     o2 = o + K * G.LAT + G.LAT
     assert np.all(g[o2:o2 + G.LAT * NP].reshape(G.LAT, NP)[:, 3:] == 0.)  # padded outputs
+
+
+class _Fp32Accumulating(G.Net):
+    """The BF16 network with its products accumulated in FP32 (as the MFMA
+    units do) instead of FP64, rounding at the same points."""
+
+    def _mlp(self, k, x, res=None):
+        W1, b1, W2, b2, _ = self.mlps[k]
+        f = np.float32
+        pre = x.astype(f) @ W1.astype(f) + b1.astype(f)
+        h = self.q(np.maximum(pre, 0.).astype(np.float64))
+        y = h.astype(f) @ W2.astype(f) + b2.astype(f)
+        if res is not None:
+            y = y + res.astype(f)
+        return y.astype(np.float64), h
+
+
+def test_bf16_network_gradient_sensitivity_to_accumulation():
+    """Why the GPU's network gradients are compared to the oracle's at 1e-1:
+    with BF16-stored activations, only changing the forward's accumulation
+    from FP64 to FP32 moves the output by ~0.4 % and the gradients by ~5 %
+    (a few ReLU masks and BF16 roundings flip and the backward amplifies
+    them).  The GPU backward itself is checked tightly on its own forward."""
+    L = 3
+    row, send, recv, _ = small_graph(N=300, seed=5, radius=0.18)
+    N, E = len(row) - 1, len(send)
+    rng = np.random.default_rng(6)
+    xn = np.zeros((N, G.FEAT)); xn[:, :20] = rng.normal(size=(N, 20))
+    xe = np.zeros((E, G.FEAT)); xe[:, :4] = rng.normal(size=(E, 4))
+    flat = random_params(L, scale=0.08)
+    d_out = rng.normal(size=(N, 3))
+    a = G.Net(flat, L, quant=True)
+    b = _Fp32Accumulating(flat, L, quant=True)
+    ya = a.forward(row, send, recv, G.bf16(xn), G.bf16(xe))
+    yb = b.forward(row, send, recv, G.bf16(xn), G.bf16(xe))
+    fwd = np.linalg.norm(ya - yb) / np.linalg.norm(ya)
+    ga, gb = a.backward(d_out), b.backward(d_out)
+    grad = np.linalg.norm(ga - gb) / np.linalg.norm(ga)
+    assert 1e-4 < fwd < 2e-2
+    assert grad > 3 * fwd          # the gradients are the more sensitive
+    assert grad < 1e-1
