@@ -301,7 +301,7 @@ def run_surrogate(args):
     wl = W.column(dims=(40, 40, 40), seed=7)
     ro = S.Rollout(net, wl, sc, check_every=10, tol=0.02)
     ro.warm_up()
-    g = ro.builder.build(ro.x.cpu().numpy())
+    g = ro.builder.build(ro.x)
     xe = torch.empty((g.E, S.FEAT), dtype=torch.bfloat16, device="cuda")
     S.features(g, ro.x, ro.vhist, ro.rad, ro.planes, ro.n_planes, sc, ro.xn, xe)
     for _ in range(2):

@@ -47,6 +47,29 @@ __device__ inline bf16x8 ld8f(const float* p) {
   return r;
 }
 
+constexpr int kWPitch = kLat + 8; // LDS row pitch (bf16) of the weight and hidden tiles
+
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+
+// Rows e0 .. e0 + R - 1 (those below M) of a wave's LDS tile [R][kWPitch]
+// to dst [M][128] BF16, 16 bytes per lane: 4 whole rows per instruction.
+template <int R>
+__device__ inline void wave_store_tile(const bf16 (*tile)[kWPitch], bf16* dst, int e0, int M, int lane) {
+#pragma unroll
+  for (int it = 0; it < R / 4; ++it) {
+    const int r = it * 4 + (lane >> 4), c = (lane & 15) * 8;
+    if (e0 + r < M)
+      *reinterpret_cast<bf16x8*>(dst + (long)(e0 + r) * kLat + c) = *reinterpret_cast<const bf16x8*>(&tile[r][c]);
+  }
+}
+
+__device__ inline void wave_lds_sync() {
+  // a wave's own LDS tile: its writes before its later reads, and back
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 struct Segs {
   int nseg;
   const bf16* src[3];
@@ -110,15 +133,25 @@ __global__ __launch_bounds__(kThreads) void k_mlp_fwd(FwdArgs a) {
       const int r = r0 + rb * 16 + lr;
       rows[rb] = r < a.M ? r : a.M - 1; // clamped loads; stores are guarded
     }
-    for (int k0 = 0; k0 < a.K; k0 += 32) {
-      bf16x8 av[2];
+    // source by source; each gathered row's address is formed once per tile,
+    // so the k loop issues independent loads only
 #pragma unroll
-      for (int rb = 0; rb < 2; ++rb) av[rb] = seg_load(a.in, rows[rb], k0 + lk);
+    for (int q = 0; q < 3; ++q) {
+      if (q >= a.in.nseg) break;
+      const int wq = a.in.width[q];
+      const bf16* src = a.in.src[q];
+      const int* idx = a.in.idx[q];
+      const bf16* p0 = src + (long)(idx ? idx[rows[0]] : rows[0]) * wq + lk;
+      const bf16* p1 = src + (long)(idx ? idx[rows[1]] : rows[1]) * wq + lk;
+      const bf16* wt = a.W1t + (long)lr * a.K + a.in.start[q] + lk;
+      for (int kk = 0; kk < wq; kk += 32) {
+        const bf16x8 av0 = ld8(p0 + kk), av1 = ld8(p1 + kk);
 #pragma unroll
-      for (int cb = 0; cb < 8; ++cb) {
-        const bf16x8 bv = ld8(a.W1t + (long)(cb * 16 + lr) * a.K + k0 + lk);
-        acc[0][cb] = mfma(av[0], bv, acc[0][cb]);
-        acc[1][cb] = mfma(av[1], bv, acc[1][cb]);
+        for (int cb = 0; cb < 8; ++cb) {
+          const bf16x8 bv = ld8(wt + (long)cb * 16 * a.K + kk);
+          acc[0][cb] = mfma(av0, bv, acc[0][cb]);
+          acc[1][cb] = mfma(av1, bv, acc[1][cb]);
+        }
       }
     }
     // hidden = relu(acc + b1) -> LDS (and global when saved for backward)
@@ -133,13 +166,11 @@ __global__ __launch_bounds__(kThreads) void k_mlp_fwd(FwdArgs a) {
           const int lrow = rb * 16 + 4 * (lane >> 4) + r;
           float v = acc[rb][cb][r] + bias;
           v = v > 0.f ? v : 0.f;
-          const bf16 hv = (bf16)v;
-          sh[w][lrow][col] = hv;
-          const int grow = r0 + lrow;
-          if (a.hid && grow < a.M) a.hid[(long)grow * kLat + col] = hv;
+          sh[w][lrow][col] = (bf16)v;
         }
       }
     __syncthreads();
+    if (a.hid) wave_store_tile<kWaveRows>(sh[w], a.hid, r0, a.M, lane);
     f32x4 acc2[2][NB2];
 #pragma unroll
     for (int rb = 0; rb < 2; ++rb)
@@ -157,23 +188,46 @@ __global__ __launch_bounds__(kThreads) void k_mlp_fwd(FwdArgs a) {
         acc2[1][cb] = mfma(av[1], bv, acc2[1][cb]);
       }
     }
+    if (NB2 == 8 && a.nout == kLat && !a.outf) {
+      // 128-wide BF16 output through the wave's (now free) hidden tile, then
+      // whole rows out
+      wave_lds_sync();
 #pragma unroll
-    for (int rb = 0; rb < 2; ++rb)
+      for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
-      for (int cb = 0; cb < NB2; ++cb) {
-        const int col = cb * 16 + lr;
-        if (col >= a.nout) continue;
-        const float bias = a.b2[col];
+        for (int cb = 0; cb < NB2; ++cb) {
+          const int col = cb * 16 + lr;
+          const float bias = a.b2[col];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int grow = r0 + rb * 16 + 4 * (lane >> 4) + r;
-          if (grow >= a.M) continue;
-          float v = acc2[rb][cb][r] + bias;
-          if (a.res) v += (float)a.res[(long)grow * a.nout + col];
-          if (a.out) a.out[(long)grow * a.nout + col] = (bf16)v;
-          if (a.outf) a.outf[(long)grow * a.nout + col] = v;
+          for (int r = 0; r < 4; ++r) {
+            const int lrow = rb * 16 + 4 * (lane >> 4) + r;
+            const int srow = r0 + lrow < a.M ? r0 + lrow : a.M - 1;
+            float v = acc2[rb][cb][r] + bias;
+            if (a.res) v += (float)a.res[(long)srow * kLat + col];
+            sh[w][lrow][col] = (bf16)v;
+          }
         }
-      }
+      wave_lds_sync();
+      if (a.out) wave_store_tile<kWaveRows>(sh[w], a.out, r0, a.M, lane);
+    } else {
+#pragma unroll
+      for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+        for (int cb = 0; cb < NB2; ++cb) {
+          const int col = cb * 16 + lr;
+          if (col >= a.nout) continue;
+          const float bias = a.b2[col];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int grow = r0 + rb * 16 + 4 * (lane >> 4) + r;
+            if (grow >= a.M) continue;
+            float v = acc2[rb][cb][r] + bias;
+            if (a.res) v += (float)a.res[(long)grow * a.nout + col];
+            if (a.out) a.out[(long)grow * a.nout + col] = (bf16)v;
+            if (a.outf) a.outf[(long)grow * a.nout + col] = v;
+          }
+        }
+    }
     __syncthreads(); // the hidden tile is rewritten by the next tile
   }
 }
@@ -303,6 +357,371 @@ __global__ __launch_bounds__(kThreads) void k_mlp_bwd(BwdArgs a) {
     __syncthreads();
   }
 }
+
+// ---------------------------------------------------------------------------
+// Edge MLP with its first layer factorised over the three sources:
+//   pre = e W1a + (h W1b)[recv] + (h W1c)[send] + b1,
+// W1 = [W1a; W1b; W1c] (rows 0..127, 128..255, 256..383 of W1 [384][128]).
+// k_node_pq forms PQ = [h W1b | h W1c] (float [N][256]) once per layer, so
+// the edge kernel multiplies K = 128 instead of 384 and gathers two FP32
+// rows per edge instead of two BF16 rows per k step.
+//
+// The edge kernel computes transposed products, hidden^T = W1a^T e^T and
+// out^T = W2^T hidden^T: the weights are the A operand, held in registers
+// for the kernel's whole persistent loop, and 16 edges are the columns of
+// each MFMA tile, so a lane's 4 accumulators are 4 consecutive units of one
+// edge (one float4 of P, of Q and of each bias per lane and tile).
+// ---------------------------------------------------------------------------
+
+struct NodePQArgs {
+  int N;
+  const bf16* h;     // [N][128]
+  const bf16* W1t;   // [128][384]: row = hidden unit
+  float* PQ;         // [N][256]
+};
+
+// PQ[n][c] = sum_k h[n][k] W1[128 + k][c] for c < 128, W1[256 + k][c - 128]
+// above.  A wave owns 32 nodes; the block's 4 waves 128.
+__global__ __launch_bounds__(kThreads) void k_node_pq(NodePQArgs a) {
+  __shared__ float sp[4][kWaveRows][kLat + 4]; // output staging per wave
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int lr = lane & 15, lk = 8 * (lane >> 4);
+  const int r0 = (blockIdx.x * 4 + w) * kWaveRows;
+  if (r0 >= a.N) return;
+  int rows[2];
+#pragma unroll
+  for (int rb = 0; rb < 2; ++rb) {
+    const int r = r0 + rb * 16 + lr;
+    rows[rb] = r < a.N ? r : a.N - 1;
+  }
+  bf16x8 av[2][4];
+#pragma unroll
+  for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) av[rb][ks] = ld8(a.h + (long)rows[rb] * kLat + ks * 32 + lk);
+#pragma unroll
+  for (int half = 0; half < 2; ++half) { // P, then Q
+    f32x4 acc[2][8];
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+      for (int cb = 0; cb < 8; ++cb) acc[rb][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks)
+#pragma unroll
+      for (int cb = 0; cb < 8; ++cb) {
+        // B[k][c] = W1[128 (1 + half) + k][c] = W1t[c][128 (1 + half) + k]
+        const bf16x8 bv = ld8(a.W1t + (long)(cb * 16 + lr) * (3 * kLat) + kLat * (1 + half) + ks * 32 + lk);
+        acc[0][cb] = mfma(av[0][ks], bv, acc[0][cb]);
+        acc[1][cb] = mfma(av[1][ks], bv, acc[1][cb]);
+      }
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+      for (int cb = 0; cb < 8; ++cb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) sp[w][rb * 16 + 4 * (lane >> 4) + r][cb * 16 + lr] = acc[rb][cb][r];
+    wave_lds_sync();
+    // whole half-rows out: 2 rows of 128 floats per instruction
+#pragma unroll
+    for (int it = 0; it < kWaveRows / 2; ++it) {
+      const int r = it * 2 + (lane >> 5), c = (lane & 31) * 4;
+      if (r0 + r < a.N)
+        *reinterpret_cast<float4*>(a.PQ + (long)(r0 + r) * 2 * kLat + half * kLat + c) =
+            *reinterpret_cast<const float4*>(&sp[w][r][c]);
+    }
+    wave_lds_sync(); // the tile is rewritten by the next half
+  }
+}
+
+struct EdgeFwdArgs {
+  int E;
+  const bf16* e;      // [E][128]
+  const int* recv;
+  const int* send;
+  const float* PQ;    // [N][256]
+  const bf16* W1t;    // [128][384]
+  const float* b1;    // [128]
+  const bf16* W2t;    // [128][128]
+  const float* b2;    // [128]
+  bf16* hid;          // [E][128] post-ReLU hidden, or null
+  bf16* out;          // [E][128]: e + MLP
+};
+
+// One wave tile's operands (16 EB edges): the edge rows, the e^T fragments
+// and the first layer's accumulator start b1 + P[recv] + Q[send].
+template <int EB>
+struct EdgeTile {
+  int ed[EB];
+  bf16x8 Bv[EB][4];
+  f32x4 acc[8][EB];
+};
+
+template <int EB>
+__device__ inline void edge_tile_load(const EdgeFwdArgs& a, int e0, int lr, int lg, EdgeTile<EB>& T) {
+  int ri[EB], sj[EB];
+#pragma unroll
+  for (int eb = 0; eb < EB; ++eb) {
+    const int ee = e0 + eb * 16 + lr;
+    T.ed[eb] = ee < a.E ? ee : a.E - 1; // clamped loads; stores are guarded
+    ri[eb] = a.recv[T.ed[eb]];
+    sj[eb] = a.send[T.ed[eb]];
+  }
+#pragma unroll
+  for (int eb = 0; eb < EB; ++eb)
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) T.Bv[eb][ks] = ld8(a.e + (long)T.ed[eb] * kLat + ks * 32 + 8 * lg);
+#pragma unroll
+  for (int ub = 0; ub < 8; ++ub) {
+    const int u = ub * 16 + 4 * lg;
+    const float4 bb = *reinterpret_cast<const float4*>(a.b1 + u);
+#pragma unroll
+    for (int eb = 0; eb < EB; ++eb) {
+      const float4 p = *reinterpret_cast<const float4*>(a.PQ + (long)ri[eb] * 2 * kLat + u);
+      const float4 q = *reinterpret_cast<const float4*>(a.PQ + (long)sj[eb] * 2 * kLat + kLat + u);
+      T.acc[ub][eb] = f32x4{bb.x + p.x + q.x, bb.y + p.y + q.y, bb.z + p.z + q.z, bb.w + p.w + q.w};
+    }
+  }
+}
+
+// Persistent and wave-granular: every wave takes tiles of 16 EB edges with
+// stride (waves in the grid); with PF it loads the next tile's operands
+// while it multiplies the current one.  W1a^T and W2^T sit in LDS for the
+// block's WAVES waves, and each wave has its own hidden tile.  With a grid
+// that is a multiple of 8, XCD (blockIdx % 8) takes a contiguous eighth of
+// the tiles, so the P/Q rows its edges gather stay in that XCD's L2.
+template <int EB, int WAVES, bool PF>
+__global__ __launch_bounds__(64 * WAVES, 1) void k_edge_fwd(EdgeFwdArgs a) {
+  constexpr int TE = 16 * EB; // edges per wave tile
+  __shared__ bf16 sw1[kLat][kWPitch];        // W1a^T: [unit][k]
+  __shared__ bf16 sw2[kLat][kWPitch];        // W2^T:  [out][k]
+  __shared__ bf16 sh[WAVES][TE][kWPitch];    // hidden, [edge][unit]
+  for (int c = threadIdx.x; c < kLat * 16; c += 64 * WAVES) {
+    const int r = c >> 4, k8 = (c & 15) * 8;
+    *reinterpret_cast<bf16x8*>(&sw1[r][k8]) = ld8(a.W1t + (long)r * (3 * kLat) + k8);
+    *reinterpret_cast<bf16x8*>(&sw2[r][k8]) = ld8(a.W2t + (long)r * kLat + k8);
+  }
+  __syncthreads(); // the only block barrier: the loop below is per wave
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int lr = lane & 15, lg = lane >> 4, lk = 8 * lg;
+  const int ntiles = (a.E + TE - 1) / TE;
+  int t, stride, t_end;
+  if (gridDim.x % 8 == 0) {
+    const int per = (ntiles + 7) / 8;
+    const int x = blockIdx.x % 8;
+    t = x * per + (blockIdx.x / 8) * WAVES + w;
+    stride = (gridDim.x / 8) * WAVES;
+    t_end = min(ntiles, (x + 1) * per);
+  } else {
+    t = blockIdx.x * WAVES + w;
+    stride = gridDim.x * WAVES;
+    t_end = ntiles;
+  }
+  if (t >= t_end) return;
+  EdgeTile<EB> cur, nxt;
+  if (PF) edge_tile_load<EB>(a, t * TE, lr, lg, cur);
+  for (; t < t_end; t += stride) {
+    const int e0 = t * TE;
+    const int tn = t + stride;
+    if (PF) {
+      if (tn < t_end) edge_tile_load<EB>(a, tn * TE, lr, lg, nxt);
+    } else {
+      edge_tile_load<EB>(a, e0, lr, lg, cur);
+    }
+    // layer 1: hidden^T = W1a^T e^T on top of b1 + P + Q
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks)
+#pragma unroll
+      for (int ub = 0; ub < 8; ++ub) {
+        const bf16x8 A = *reinterpret_cast<const bf16x8*>(&sw1[ub * 16 + lr][ks * 32 + lk]);
+#pragma unroll
+        for (int eb = 0; eb < EB; ++eb) cur.acc[ub][eb] = mfma(A, cur.Bv[eb][ks], cur.acc[ub][eb]);
+      }
+    // relu -> BF16 hidden tile [edge][unit] (and global for the backward)
+#pragma unroll
+    for (int ub = 0; ub < 8; ++ub)
+#pragma unroll
+      for (int eb = 0; eb < EB; ++eb) {
+        bf16x4 hv;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const float v = cur.acc[ub][eb][r];
+          hv[r] = (bf16)(v > 0.f ? v : 0.f);
+        }
+        const int le = eb * 16 + lr, u = ub * 16 + 4 * lg;
+        *reinterpret_cast<bf16x4*>(&sh[w][le][u]) = hv;
+      }
+    wave_lds_sync();
+    if (a.hid) wave_store_tile<TE>(sh[w], a.hid, e0, a.E, lane);
+    // layer 2: out^T = W2^T hidden^T on top of b2 + e (the residual)
+    f32x4 acc2[8][EB];
+#pragma unroll
+    for (int ub = 0; ub < 8; ++ub) {
+      const int u = ub * 16 + 4 * lg;
+      const float4 bb = *reinterpret_cast<const float4*>(a.b2 + u);
+#pragma unroll
+      for (int eb = 0; eb < EB; ++eb) {
+        const bf16x4 rv = *reinterpret_cast<const bf16x4*>(a.e + (long)cur.ed[eb] * kLat + u);
+        acc2[ub][eb] = f32x4{bb.x + (float)rv[0], bb.y + (float)rv[1], bb.z + (float)rv[2], bb.w + (float)rv[3]};
+      }
+    }
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      bf16x8 hb[EB];
+#pragma unroll
+      for (int eb = 0; eb < EB; ++eb) hb[eb] = *reinterpret_cast<const bf16x8*>(&sh[w][eb * 16 + lr][ks * 32 + lk]);
+#pragma unroll
+      for (int ub = 0; ub < 8; ++ub) {
+        const bf16x8 A = *reinterpret_cast<const bf16x8*>(&sw2[ub * 16 + lr][ks * 32 + lk]);
+#pragma unroll
+        for (int eb = 0; eb < EB; ++eb) acc2[ub][eb] = mfma(A, hb[eb], acc2[ub][eb]);
+      }
+    }
+    // the output through the (now free) hidden tile, then whole rows out
+    wave_lds_sync();
+#pragma unroll
+    for (int ub = 0; ub < 8; ++ub)
+#pragma unroll
+      for (int eb = 0; eb < EB; ++eb) {
+        bf16x4 ov;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) ov[r] = (bf16)acc2[ub][eb][r];
+        *reinterpret_cast<bf16x4*>(&sh[w][eb * 16 + lr][ub * 16 + 4 * lg]) = ov;
+      }
+    wave_lds_sync();
+    wave_store_tile<TE>(sh[w], a.out, e0, a.E, lane);
+    wave_lds_sync(); // the next tile rewrites the tile
+    if (PF && tn < t_end) cur = nxt;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Dense MLP y = res + W2 relu(W1 [x0, x1] + b1) + b2 with 128 outputs, the
+// inputs ungathered rows (K = 32, 128 or 256: x1 holds columns 128..255).
+// The edge kernel's scheme: persistent, wave-granular, weights in LDS for
+// the whole block, transposed products (rows are the 16 MFMA columns), the
+// next tile's rows loaded while the current one multiplies, and the output
+// through LDS so that whole rows are stored.
+// ---------------------------------------------------------------------------
+
+struct DenseFwdArgs {
+  int M;
+  const bf16* x0;     // [M][min(K, 128)]
+  const bf16* x1;     // [M][128] (K = 256) or null
+  const bf16* W1t;    // [128][K]
+  const float* b1;
+  const bf16* W2t;    // [128][128]
+  const float* b2;
+  const bf16* res;    // [M][128] or null
+  bf16* hid;          // [M][128] or null
+  bf16* out;          // [M][128]
+};
+
+template <int K, int WAVES>
+__global__ __launch_bounds__(64 * WAVES, 1) void k_dense_fwd(DenseFwdArgs a) {
+  constexpr int KS = K / 32;          // k steps of the first layer
+  constexpr int W0 = K < kLat ? K : kLat; // width of x0
+  __shared__ bf16 sw1[kLat][K + 8];   // W1^T: [unit][k]
+  __shared__ bf16 sw2[kLat][kWPitch]; // W2^T: [out][k]
+  __shared__ bf16 sh[WAVES][16][kWPitch];
+  for (int c = threadIdx.x; c < kLat * (K / 8); c += 64 * WAVES) {
+    const int r = c / (K / 8), k8 = (c % (K / 8)) * 8;
+    *reinterpret_cast<bf16x8*>(&sw1[r][k8]) = ld8(a.W1t + (long)r * K + k8);
+  }
+  for (int c = threadIdx.x; c < kLat * 16; c += 64 * WAVES) {
+    const int r = c >> 4, k8 = (c & 15) * 8;
+    *reinterpret_cast<bf16x8*>(&sw2[r][k8]) = ld8(a.W2t + (long)r * kLat + k8);
+  }
+  __syncthreads(); // the only block barrier
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int lr = lane & 15, lg = lane >> 4, lk = 8 * lg;
+  const int ntiles = (a.M + 15) / 16;
+  const int stride = gridDim.x * WAVES;
+  int t = blockIdx.x * WAVES + w;
+  if (t >= ntiles) return;
+  auto load_rows = [&](int m0, bf16x8 (&B)[KS]) {
+    const int m = m0 + lr < a.M ? m0 + lr : a.M - 1;
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+      const int k = ks * 32 + lk;
+      B[ks] = ks * 32 < W0 ? ld8(a.x0 + (long)m * W0 + k) : ld8(a.x1 + (long)m * kLat + (k - kLat));
+    }
+  };
+  bf16x8 cur[KS], nxt[KS];
+  load_rows(t * 16, cur);
+  for (; t < ntiles; t += stride) {
+    const int m0 = t * 16;
+    const int tn = t + stride;
+    if (tn < ntiles) load_rows(tn * 16, nxt);
+    const int mrow = m0 + lr < a.M ? m0 + lr : a.M - 1;
+    f32x4 acc[8];
+#pragma unroll
+    for (int ub = 0; ub < 8; ++ub) {
+      const float4 bb = *reinterpret_cast<const float4*>(a.b1 + ub * 16 + 4 * lg);
+      acc[ub] = f32x4{bb.x, bb.y, bb.z, bb.w};
+    }
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+      for (int ub = 0; ub < 8; ++ub)
+        acc[ub] = mfma(*reinterpret_cast<const bf16x8*>(&sw1[ub * 16 + lr][ks * 32 + lk]), cur[ks], acc[ub]);
+#pragma unroll
+    for (int ub = 0; ub < 8; ++ub) {
+      bf16x4 hv;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) hv[r] = (bf16)(acc[ub][r] > 0.f ? acc[ub][r] : 0.f);
+      *reinterpret_cast<bf16x4*>(&sh[w][lr][ub * 16 + 4 * lg]) = hv;
+    }
+    wave_lds_sync();
+    if (a.hid) wave_store_tile<16>(sh[w], a.hid, m0, a.M, lane);
+    f32x4 acc2[8];
+#pragma unroll
+    for (int ub = 0; ub < 8; ++ub) {
+      const int u = ub * 16 + 4 * lg;
+      const float4 bb = *reinterpret_cast<const float4*>(a.b2 + u);
+      acc2[ub] = f32x4{bb.x, bb.y, bb.z, bb.w};
+      if (a.res) {
+        const bf16x4 rv = *reinterpret_cast<const bf16x4*>(a.res + (long)mrow * kLat + u);
+        acc2[ub] += f32x4{(float)rv[0], (float)rv[1], (float)rv[2], (float)rv[3]};
+      }
+    }
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      const bf16x8 hb = *reinterpret_cast<const bf16x8*>(&sh[w][lr][ks * 32 + lk]);
+#pragma unroll
+      for (int ub = 0; ub < 8; ++ub)
+        acc2[ub] = mfma(*reinterpret_cast<const bf16x8*>(&sw2[ub * 16 + lr][ks * 32 + lk]), hb, acc2[ub]);
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int ub = 0; ub < 8; ++ub) {
+      bf16x4 ov;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) ov[r] = (bf16)acc2[ub][r];
+      *reinterpret_cast<bf16x4*>(&sh[w][lr][ub * 16 + 4 * lg]) = ov;
+    }
+    wave_lds_sync();
+    wave_store_tile<16>(sh[w], a.out, m0, a.M, lane);
+    wave_lds_sync(); // the next tile rewrites the tile
+    if (tn < ntiles) {
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) cur[ks] = nxt[ks];
+    }
+  }
+}
+
+// Launch shape of the edge kernel (the template instance below).
+#ifndef GNN_EDGE_EB
+#define GNN_EDGE_EB 2
+#endif
+#ifndef GNN_EDGE_WAVES
+#define GNN_EDGE_WAVES 8
+#endif
+#ifndef GNN_EDGE_PF
+#define GNN_EDGE_PF 1
+#endif
+constexpr int kEdgeEB = GNN_EDGE_EB, kEdgeWaves = GNN_EDGE_WAVES;
+constexpr bool kEdgePF = GNN_EDGE_PF != 0;
 
 struct WgradArgs {
   int M, K, NC;      // rows, input features, output columns (multiple of 16)
@@ -575,11 +994,312 @@ __global__ __launch_bounds__(kThreads) void k_integrate(int N, const float* a, f
   }
 }
 
+// ---------------------------------------------------------------------------
+// radius graph: uniform cells of side >= radius, a counting sort of the
+// nodes into cells, a count pass, an exclusive scan into row offsets and a
+// fill pass that writes each node's senders in ascending order.
+// ---------------------------------------------------------------------------
+
+struct CellGrid {
+  double lo[3], inv[3];
+  int nc[3];
+  int ncell;
+};
+
+constexpr int kScanPerBlock = 4 * kThreads; // elements per block of the scan
+
+__device__ inline int cell_coord(double x, double lo, double inv, int nc) {
+  double t = (x - lo) * inv;
+  if (!(t >= 0.)) t = 0.;          // below the box, or NaN
+  if (t >= (double)nc) t = nc - 1; // above it (clamped before the conversion)
+  return (int)t;
+}
+
+__global__ __launch_bounds__(kThreads) void k_g_zero(int* p, long n) {
+  const long i = (long)blockIdx.x * kThreads + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+__global__ __launch_bounds__(kThreads) void k_g_bin(int N, const double* x, CellGrid g, int* cell_of,
+                                                    int* ccount) {
+  const int i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= N) return;
+  const int cx = cell_coord(x[3l * i], g.lo[0], g.inv[0], g.nc[0]);
+  const int cy = cell_coord(x[3l * i + 1], g.lo[1], g.inv[1], g.nc[1]);
+  const int cz = cell_coord(x[3l * i + 2], g.lo[2], g.inv[2], g.nc[2]);
+  const int c = (cz * g.nc[1] + cy) * g.nc[0] + cx;
+  cell_of[i] = c;
+  atomicAdd(&ccount[c], 1);
+}
+
+// Exclusive scan of in[0..n) into out[0..n], out[n] = total: block-local
+// scans (kScanPerBlock elements each, block totals to bsum), a scan of the
+// block totals in one block, and the carry added back.
+__global__ __launch_bounds__(kThreads) void k_scan_local(const int* in, long n, int* out, int* bsum) {
+  __shared__ int s[kThreads];
+  const long base = (long)blockIdx.x * kScanPerBlock + 4l * threadIdx.x;
+  int v[4], t = 0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    v[q] = base + q < n ? in[base + q] : 0;
+    t += v[q];
+  }
+  s[threadIdx.x] = t;
+  __syncthreads();
+  for (int d = 1; d < kThreads; d <<= 1) {
+    const int u = threadIdx.x >= d ? s[threadIdx.x - d] : 0;
+    __syncthreads();
+    s[threadIdx.x] += u;
+    __syncthreads();
+  }
+  int run = s[threadIdx.x] - t; // exclusive prefix of this thread
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if (base + q < n) out[base + q] = run;
+    run += v[q];
+  }
+  if (threadIdx.x == kThreads - 1) bsum[blockIdx.x] = s[kThreads - 1];
+}
+
+// The block totals are scanned in 64 bits: the caller reads the exact total
+// (total64) to reject a graph whose edge count does not fit an int.
+__global__ __launch_bounds__(kThreads) void k_scan_totals(int* bsum, int nb, int* out_total,
+                                                          long long* total64) {
+  __shared__ long long s[kThreads];
+  long long carry = 0;
+  for (int b0 = 0; b0 < nb; b0 += kThreads) {
+    const int i = b0 + threadIdx.x;
+    const long long t = i < nb ? bsum[i] : 0;
+    s[threadIdx.x] = t;
+    __syncthreads();
+    for (int d = 1; d < kThreads; d <<= 1) {
+      const long long u = threadIdx.x >= d ? s[threadIdx.x - d] : 0;
+      __syncthreads();
+      s[threadIdx.x] += u;
+      __syncthreads();
+    }
+    if (i < nb) bsum[i] = (int)(carry + s[threadIdx.x] - t);
+    carry += s[kThreads - 1];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    *out_total = (int)carry;
+    if (total64) *total64 = carry;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_scan_carry(int* out, long n, const int* bsum) {
+  const long i = (long)blockIdx.x * kThreads + threadIdx.x;
+  if (i < n) out[i] += bsum[i / kScanPerBlock];
+}
+
+__global__ __launch_bounds__(kThreads) void k_g_scatter(int N, const int* cell_of, const int* cstart,
+                                                        int* cursor, int* cnodes) {
+  const int i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= N) return;
+  const int c = cell_of[i];
+  cnodes[cstart[c] + atomicAdd(&cursor[c], 1)] = i;
+}
+
+// Visits the nodes j != i with |x_i - x_j|^2 <= r2 in the 27 cells around
+// node i's cell (the order depends on the scatter; callers sort).
+template <class F>
+__device__ inline void for_each_neighbor(int i, const double* x, const CellGrid& g, double r2,
+                                         const int* cell_of, const int* cstart, const int* cnodes, F f) {
+  const int c = cell_of[i];
+  const int cx = c % g.nc[0], cy = (c / g.nc[0]) % g.nc[1], cz = c / (g.nc[0] * g.nc[1]);
+  const double xi0 = x[3l * i], xi1 = x[3l * i + 1], xi2 = x[3l * i + 2];
+  for (int z = cz > 0 ? cz - 1 : 0; z <= cz + 1 && z < g.nc[2]; ++z)
+    for (int y = cy > 0 ? cy - 1 : 0; y <= cy + 1 && y < g.nc[1]; ++y) {
+      const int row0 = (z * g.nc[1] + y) * g.nc[0];
+      const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < g.nc[0] ? cx + 1 : g.nc[0] - 1;
+      // cells x0..x1 of one row are contiguous in cnodes
+      for (int p = cstart[row0 + x0]; p < cstart[row0 + x1 + 1]; ++p) {
+        const int j = cnodes[p];
+        if (j == i) continue;
+        const double d0 = xi0 - x[3l * j], d1 = xi1 - x[3l * j + 1], d2 = xi2 - x[3l * j + 2];
+        if (d0 * d0 + d1 * d1 + d2 * d2 <= r2) f(j);
+      }
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_g_count(int N, const double* x, CellGrid g, double r2,
+                                                      const int* cell_of, const int* cstart,
+                                                      const int* cnodes, int* deg) {
+  const int i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= N) return;
+  int n = 0;
+  for_each_neighbor(i, x, g, r2, cell_of, cstart, cnodes, [&](int) { ++n; });
+  deg[i] = n;
+}
+
+__global__ __launch_bounds__(kThreads) void k_g_fill(int N, const double* x, CellGrid g, double r2,
+                                                     const int* cell_of, const int* cstart,
+                                                     const int* cnodes, const int* row, int E,
+                                                     int* send, int* recv) {
+  const int i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= N) return;
+  const int e0 = row[i], e1 = row[i + 1] < E ? row[i + 1] : E;
+  if (e0 < 0) return;
+  int n = e0;
+  for_each_neighbor(i, x, g, r2, cell_of, cstart, cnodes, [&](int j) {
+    if (n < e1) { // the count pass saw the same set; the bound only guards memory
+      // insertion into the sorted prefix send[e0..n)
+      int p = n;
+      while (p > e0 && send[p - 1] > j) {
+        send[p] = send[p - 1];
+        --p;
+      }
+      send[p] = j;
+      recv[n] = i;
+      ++n;
+    }
+  });
+}
+
+// rev[e] for edge j -> i: the position of i among node j's senders
+// (binary search in the sorted row of j; the graph is symmetric).
+// An edge whose reverse is missing (impossible for this symmetric criterion)
+// gets rev[e] = e, which keeps every later gather in bounds.
+__global__ __launch_bounds__(kThreads) void k_g_rev(int N, const int* row, int E, const int* send,
+                                                    int* rev) {
+  const int i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= N) return;
+  const int e1 = row[i + 1] < E ? row[i + 1] : E;
+  for (int e = row[i] > 0 ? row[i] : 0; e < e1; ++e) {
+    const int j = send[e];
+    if (j < 0 || j >= N) {
+      rev[e] = e;
+      continue;
+    }
+    int lo = row[j], hi = row[j + 1] < E ? row[j + 1] : E;
+    const int end = hi;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (send[mid] < i) lo = mid + 1;
+      else hi = mid;
+    }
+    rev[e] = lo < end && send[lo] == i ? lo : e;
+  }
+}
+
 inline unsigned nblk(long n) { return (unsigned)((n + kThreads - 1) / kThreads); }
+
+// Cell grid of the box [lo, hi] with cells of side > radius (by a relative
+// 1e-6, so that rounding in the cell coordinates never puts two nodes closer
+// than the radius two cells apart); -1 if the box is degenerate or needs
+// more than 2^26 cells.
+int make_grid(const double lo[3], const double hi[3], double radius, CellGrid* g) {
+  if (!lo || !hi || !(radius > 0.)) return -1;
+  long nc = 1;
+  for (int d = 0; d < 3; ++d) {
+    const double w = hi[d] - lo[d];
+    if (!(w > 0.)) return -1;
+    double k = w / (radius * (1. + 1e-6));
+    if (!(k < 1024.)) k = 1024.;
+    g->nc[d] = k < 1. ? 1 : (int)k;
+    g->lo[d] = lo[d];
+    g->inv[d] = g->nc[d] / w;
+    nc *= g->nc[d];
+  }
+  if (nc > (1l << 26)) return -1;
+  g->ncell = (int)nc;
+  return 0;
+}
+
+long scan_blocks(long n) { return (n + kScanPerBlock - 1) / kScanPerBlock; }
+
+// Workspace of the radius graph, in ints.
+struct GraphWork {
+  int *ccount, *cstart, *cursor, *cell_of, *cnodes, *deg, *bsum;
+  long long* total64;
+  long ints;
+};
+
+GraphWork carve(void* base, int N, int ncell) {
+  GraphWork w{};
+  int* p = reinterpret_cast<int*>(base);
+  long o = 0;
+  auto take = [&](long n) {
+    int* r = p ? p + o : nullptr;
+    o += (n + 63) / 64 * 64;
+    return r;
+  };
+  w.ccount = take(ncell);
+  w.cstart = take((long)ncell + 1);
+  w.cursor = take(ncell);
+  w.cell_of = take(N);
+  w.cnodes = take(N);
+  w.deg = take(N);
+  w.bsum = take(scan_blocks((long)(N > ncell ? N : ncell) + 1) + 1);
+  w.total64 = reinterpret_cast<long long*>(take(2)); // offsets are 256-byte aligned
+  w.ints = o;
+  return w;
+}
+
+void scan(const int* in, long n, int* out, int* bsum, long long* total64, hipStream_t s) {
+  const long nb = scan_blocks(n);
+  hipLaunchKernelGGL(k_scan_local, dim3((unsigned)(nb > 0 ? nb : 1)), dim3(kThreads), 0, s, in, n, out, bsum);
+  hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(kThreads), 0, s, bsum, (int)(nb > 0 ? nb : 1), out + n,
+                     total64);
+  if (n > 0) hipLaunchKernelGGL(k_scan_carry, dim3(nblk(n)), dim3(kThreads), 0, s, out, n, bsum);
+}
 
 int grid_rows(int M, int grid_cap) {
   const int nt = (M + kTileRows - 1) / kTileRows;
   return grid_cap > 0 && grid_cap < nt ? grid_cap : nt;
+}
+
+// The persistent dense kernel takes ungathered inputs of width 32, 128 or
+// 128 + 128 and a 128-wide BF16 output.
+bool dense_shape(const gnn_mlp_io* io, const gnn_mlp_weights* wt, const void* out, const float* outf) {
+  if (!out || outf || wt->nout != kLat || wt->nout_pad != kLat) return false;
+  for (int k = 0; k < io->nseg; ++k)
+    if (io->idx[k]) return false;
+  if (io->nseg == 1) return io->width[0] == 32 || io->width[0] == kLat;
+  return io->nseg == 2 && io->width[0] == kLat && io->width[1] == kLat;
+}
+
+int device_cus() {
+  static int cus = 0;
+  if (cus == 0) {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+      return -1;
+    cus = n;
+  }
+  return cus;
+}
+
+template <int K, int WAVES>
+int launch_dense_k(const DenseFwdArgs& a, hipStream_t s) {
+  const int cus = device_cus();
+  if (cus <= 0) return -2;
+  const long blocks = ((long)a.M + 16l * WAVES - 1) / (16l * WAVES);
+  const int grid = blocks < cus ? (int)blocks : cus;
+  hipLaunchKernelGGL((k_dense_fwd<K, WAVES>), dim3(grid), dim3(64 * WAVES), 0, s, a);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_dense(const gnn_mlp_io* io, const gnn_mlp_weights* wt, const void* residual, void* hidden,
+                 void* out, hipStream_t s) {
+  DenseFwdArgs a{};
+  a.M = io->M;
+  a.x0 = reinterpret_cast<const bf16*>(io->src[0]);
+  a.x1 = io->nseg == 2 ? reinterpret_cast<const bf16*>(io->src[1]) : nullptr;
+  a.W1t = reinterpret_cast<const bf16*>(wt->W1t);
+  a.b1 = wt->b1;
+  a.W2t = reinterpret_cast<const bf16*>(wt->W2t);
+  a.b2 = wt->b2;
+  a.res = reinterpret_cast<const bf16*>(residual);
+  a.hid = reinterpret_cast<bf16*>(hidden);
+  a.out = reinterpret_cast<bf16*>(out);
+  const int K = io->nseg == 2 ? 2 * kLat : io->width[0];
+  if (K != wt->K) return -1;
+  if (K == 32) return launch_dense_k<32, 16>(a, s);
+  if (K == kLat) return launch_dense_k<kLat, 16>(a, s);
+  return launch_dense_k<2 * kLat, 12>(a, s);
 }
 
 int check_io(const gnn_mlp_io* io) {
@@ -599,6 +1319,8 @@ int gnn_mlp_forward(const gnn_mlp_io* io, const gnn_mlp_weights* wt, const void*
                     void* hidden, void* out, float* out_f32, int grid_cap, void* stream) {
   if (check_io(io) || !wt || (wt->nout_pad != 16 && wt->nout_pad != 128) || wt->nout > wt->nout_pad)
     return -1;
+  if (grid_cap == 0 && dense_shape(io, wt, out, out_f32))
+    return launch_dense(io, wt, residual, hidden, out, reinterpret_cast<hipStream_t>(stream));
   FwdArgs a{};
   a.M = io->M;
   a.in = make_segs(io);
@@ -752,6 +1474,92 @@ int gnn_mse(const float* y, const float* t, long count, float* dY, float* part, 
   if (!y || !t || !dY || !part || count <= 0) return -1;
   hipLaunchKernelGGL(k_mse, dim3(nblk(count)), dim3(kThreads), 0, reinterpret_cast<hipStream_t>(stream),
                      y, t, count, dY, part);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int gnn_node_pq(int N, const void* h, const gnn_mlp_weights* wt, float* PQ, void* stream) {
+  if (N <= 0 || !h || !wt || !wt->W1t || !PQ || wt->K != 3 * kLat) return -1;
+  NodePQArgs a{N, reinterpret_cast<const bf16*>(h), reinterpret_cast<const bf16*>(wt->W1t), PQ};
+  const int blocks = (N + kTileRows - 1) / kTileRows;
+  hipLaunchKernelGGL(k_node_pq, dim3(blocks), dim3(kThreads), 0, reinterpret_cast<hipStream_t>(stream), a);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int gnn_edge_mlp_forward(long E, const void* e, const int* recv, const int* send, const float* PQ,
+                         const gnn_mlp_weights* wt, void* hidden, void* out, int grid_cap, void* stream) {
+  if (E < 0 || E > 0x7fffffffl || !wt || wt->K != 3 * kLat || wt->nout != kLat || wt->nout_pad != kLat)
+    return -1;
+  if (E == 0) return 0;
+  if (!e || !recv || !send || !PQ || !out || !wt->W1t || !wt->W2t || !wt->b1 || !wt->b2) return -1;
+  EdgeFwdArgs a{};
+  a.E = (int)E;
+  a.e = reinterpret_cast<const bf16*>(e);
+  a.recv = recv;
+  a.send = send;
+  a.PQ = PQ;
+  a.W1t = reinterpret_cast<const bf16*>(wt->W1t);
+  a.b1 = wt->b1;
+  a.W2t = reinterpret_cast<const bf16*>(wt->W2t);
+  a.b2 = wt->b2;
+  a.hid = reinterpret_cast<bf16*>(hidden);
+  a.out = reinterpret_cast<bf16*>(out);
+  // one block per CU (the weights in LDS, shared by its waves), persistent
+  const int cus = device_cus();
+  if (cus <= 0) return -2;
+  const long per_block = 16l * kEdgeEB * kEdgeWaves;
+  const long nblocks = (E + per_block - 1) / per_block;
+  int grid = grid_cap > 0 ? grid_cap : cus;
+  if (grid > nblocks) grid = (int)nblocks;
+  hipLaunchKernelGGL((k_edge_fwd<kEdgeEB, kEdgeWaves, kEdgePF>), dim3(grid), dim3(64 * kEdgeWaves), 0,
+                     reinterpret_cast<hipStream_t>(stream), a);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+long gnn_radius_graph_workspace(int N, const double lo[3], const double hi[3], double radius) {
+  CellGrid g;
+  if (N < 0 || make_grid(lo, hi, radius, &g)) return -1;
+  return carve(nullptr, N, g.ncell).ints * (long)sizeof(int);
+}
+
+int gnn_radius_graph_count(int N, const double* x, double radius, const double lo[3], const double hi[3],
+                           void* work, long work_bytes, int* row, long* n_edges, void* stream) {
+  CellGrid g;
+  if (N < 0 || (N > 0 && !x) || !work || !row || !n_edges || make_grid(lo, hi, radius, &g)) return -1;
+  const GraphWork w = carve(work, N, g.ncell);
+  if (work_bytes < w.ints * (long)sizeof(int)) return -1;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_g_zero, dim3(nblk(g.ncell)), dim3(kThreads), 0, s, w.ccount, (long)g.ncell);
+  hipLaunchKernelGGL(k_g_zero, dim3(nblk(g.ncell)), dim3(kThreads), 0, s, w.cursor, (long)g.ncell);
+  if (N > 0) {
+    hipLaunchKernelGGL(k_g_bin, dim3(nblk(N)), dim3(kThreads), 0, s, N, x, g, w.cell_of, w.ccount);
+    scan(w.ccount, g.ncell, w.cstart, w.bsum, nullptr, s);
+    hipLaunchKernelGGL(k_g_scatter, dim3(nblk(N)), dim3(kThreads), 0, s, N, w.cell_of, w.cstart, w.cursor,
+                       w.cnodes);
+    hipLaunchKernelGGL(k_g_count, dim3(nblk(N)), dim3(kThreads), 0, s, N, x, g, radius * radius, w.cell_of,
+                       w.cstart, w.cnodes, w.deg);
+  }
+  scan(w.deg, N, row, w.bsum, w.total64, s);
+  long long total = 0;
+  if (hipMemcpyAsync(&total, w.total64, sizeof(total), hipMemcpyDeviceToHost, s) != hipSuccess) return -2;
+  if (hipStreamSynchronize(s) != hipSuccess) return -2;
+  *n_edges = (long)total;
+  return total > 0x7fffffffll ? -3 : 0;
+}
+
+int gnn_radius_graph_fill(int N, const double* x, double radius, const double lo[3], const double hi[3],
+                          const void* work, long work_bytes, const int* row, long n_edges, int* send,
+                          int* recv, int* rev, void* stream) {
+  CellGrid g;
+  if (N < 0 || n_edges < 0 || n_edges > 0x7fffffffl || !work || !row || make_grid(lo, hi, radius, &g))
+    return -1;
+  if (n_edges > 0 && (!x || !send || !recv || !rev)) return -1;
+  const GraphWork w = carve(const_cast<void*>(work), N, g.ncell);
+  if (work_bytes < w.ints * (long)sizeof(int)) return -1;
+  if (N == 0 || n_edges == 0) return 0;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_g_fill, dim3(nblk(N)), dim3(kThreads), 0, s, N, x, g, radius * radius, w.cell_of,
+                     w.cstart, w.cnodes, row, (int)n_edges, send, recv);
+  hipLaunchKernelGGL(k_g_rev, dim3(nblk(N)), dim3(kThreads), 0, s, N, row, (int)n_edges, send, rev);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

@@ -37,7 +37,7 @@ GNN_LIB_PATH = Path(os.environ.get("CABANADEM_GNN_LIB") or
 LAT = 128
 FEAT = 32
 HIST = 5
-GNN_ABI_VERSION = 1
+GNN_ABI_VERSION = 2
 
 _vp = C.c_void_p
 _ip = C.POINTER(C.c_int)
@@ -74,6 +74,14 @@ _SIG = {
     "gnn_edge_features": (C.c_int, [C.c_long, _vp, _vp, _vp, C.c_double, _vp, _vp]),
     "gnn_integrate": (C.c_int, [C.c_int, _vp, C.c_float, _vp, C.c_double, _vp, _vp, C.c_int, _vp]),
     "gnn_mse": (C.c_int, [_vp, _vp, C.c_long, _vp, _vp, _vp]),
+    "gnn_node_pq": (C.c_int, [C.c_int, _vp, C.POINTER(GnnMlpWeights), _vp, _vp]),
+    "gnn_edge_mlp_forward": (C.c_int, [C.c_long, _vp, _vp, _vp, _vp, C.POINTER(GnnMlpWeights), _vp, _vp,
+                                       C.c_int, _vp]),
+    "gnn_radius_graph_workspace": (C.c_long, [C.c_int, _dp, _dp, C.c_double]),
+    "gnn_radius_graph_count": (C.c_int, [C.c_int, _vp, C.c_double, _dp, _dp, _vp, C.c_long, _vp,
+                                         C.POINTER(C.c_long), _vp]),
+    "gnn_radius_graph_fill": (C.c_int, [C.c_int, _vp, C.c_double, _dp, _dp, _vp, C.c_long, _vp, C.c_long,
+                                        _vp, _vp, _vp, _vp]),
 }
 
 _GLIB = None
@@ -148,21 +156,50 @@ class Graph:
 
 
 class GraphBuilder:
-    """Radius graph of a particle set through the DEM library's neighbour
-    build (the same cell-binned list the DEM uses)."""
+    """Radius graph of a particle set, built on the device
+    (gnn_radius_graph_count / _fill): the pairs with |x_i - x_j|^2 <= radius^2,
+    the DEM's neighbour criterion, receiver-major with ascending senders.
+    The Graph that build() returns views this builder's buffers and stays
+    valid until its next build()."""
 
-    def __init__(self, n, radius, lo, hi, max_n=64):
-        self.n, self.radius, self.max_n = n, radius, max_n
-        self.sys = abi.System(n, dt=1.0, cutoff=radius, lo=lo, hi=hi, max_contacts=1)
-        self.st = abi.ParticleState(n, max_cnts=1, max_walls=1)
-        self.st.rad[:] = radius / 4.
+    def __init__(self, n, radius, lo, hi):
+        torch = _torch()
+        self.n, self.radius = n, float(radius)
+        self.lo = (C.c_double * 3)(*[float(v) for v in lo])
+        self.hi = (C.c_double * 3)(*[float(v) for v in hi])
+        wb = glib().gnn_radius_graph_workspace(n, self.lo, self.hi, self.radius)
+        if wb < 0:
+            raise ValueError(f"radius graph: degenerate box or too many cells ({lo}, {hi}, {radius})")
+        self.wbytes = wb
+        self.work = torch.empty(wb // 4 + 1, dtype=torch.int32, device="cuda")
+        self.row = torch.empty(n + 1, dtype=torch.int32, device="cuda")
+        self.cap = 0
+        self.send = self.recv = self.rev = None
 
-    def build(self, x_host):
-        self.st.x[:] = x_host
-        self.sys.upload(self.st, fields=("x", "rad", "m", "I", "E", "G"))
-        self.sys.build_neighbors()
-        counts, ids = self.sys.neighbors(self.max_n)
-        return Graph.from_rows(counts, ids)
+    def build(self, x):
+        """x: positions, double [n][3], a device tensor (or a host array,
+        copied to the device)."""
+        torch = _torch()
+        if isinstance(x, np.ndarray):
+            x = torch.from_numpy(np.ascontiguousarray(x, dtype=np.float64)).cuda()
+        if x.dtype != torch.float64 or tuple(x.shape) != (self.n, 3) or not x.is_contiguous() or not x.is_cuda:
+            raise ValueError("radius graph: x must be a contiguous float64 [n][3] device tensor")
+        L, s = glib(), stream()
+        E = C.c_long(0)
+        _ok(L.gnn_radius_graph_count(self.n, ptr(x), self.radius, self.lo, self.hi, ptr(self.work), self.wbytes,
+                                     ptr(self.row), C.byref(E), s), "radius graph count")
+        E = int(E.value)
+        if E > self.cap:
+            self.cap = max(E * 5 // 4, 64)
+            z = lambda: torch.empty(self.cap, dtype=torch.int32, device="cuda")
+            self.send, self.recv, self.rev = z(), z(), z()
+        if E > 0:
+            _ok(L.gnn_radius_graph_fill(self.n, ptr(x), self.radius, self.lo, self.hi, ptr(self.work),
+                                        self.wbytes, ptr(self.row), E, ptr(self.send), ptr(self.recv),
+                                        ptr(self.rev), s), "radius graph fill")
+        if self.send is None:
+            self.send = self.recv = self.rev = torch.empty(1, dtype=torch.int32, device="cuda")
+        return Graph(self.n, E, self.row, self.send[:E], self.recv[:E], self.rev[:E])
 
 
 # ---------------------------------------------------------------------------
@@ -195,12 +232,16 @@ class MLP:
                 flat[self.o_W2:self.o_b2].view(LAT, self.NP), flat[self.o_b2:self.end])
 
     def weights(self, flat):
-        w = GnnMlpWeights()
-        w.K, w.nout, w.nout_pad = self.K, self.nout, self.NP
-        w.W1t, w.W1, w.W2t, w.W2 = ptr(self.W1t), ptr(self.W1), ptr(self.W2t), ptr(self.W2)
-        w.b1 = C.c_void_p(flat.data_ptr() + 4 * self.o_b1)
-        w.b2 = C.c_void_p(flat.data_ptr() + 4 * self.o_b2)
-        return w
+        """The gnn_mlp_weights of this MLP (cached: it only holds addresses)."""
+        key = (flat.data_ptr(), self.W1t.data_ptr(), self.W2t.data_ptr())
+        if getattr(self, "_wkey", None) != key:
+            w = GnnMlpWeights()
+            w.K, w.nout, w.nout_pad = self.K, self.nout, self.NP
+            w.W1t, w.W1, w.W2t, w.W2 = ptr(self.W1t), ptr(self.W1), ptr(self.W2t), ptr(self.W2)
+            w.b1 = C.c_void_p(flat.data_ptr() + 4 * self.o_b1)
+            w.b2 = C.c_void_p(flat.data_ptr() + 4 * self.o_b2)
+            self._w, self._wkey = w, key
+        return self._w
 
 
 def io(M, *segs):
@@ -215,9 +256,14 @@ def io(M, *segs):
 
 
 class Network:
-    def __init__(self, n_layers=10, seed=0, device="cuda", residual_scale=0.1):
+    def __init__(self, n_layers=10, seed=0, device="cuda", residual_scale=0.1, factorised=True):
+        """factorised: run the edge MLPs' forward through gnn_node_pq +
+        gnn_edge_mlp_forward (the first layer split over e, h_i, h_j) rather
+        than gnn_mlp_forward over the three gathered sources; the same
+        function up to FP32 summation order."""
         torch = _torch()
         self.L = n_layers
+        self.factorised = factorised
         self.dev = device
         off = 0
         self.enc_n = MLP(FEAT, LAT, off); off = self.enc_n.end
@@ -299,6 +345,7 @@ class Network:
         self.hid_n = [z(N, LAT) for _ in range(L)]
         self.hid_enc_n, self.hid_enc_e, self.hid_dec = z(N, LAT), z(E, LAT), z(N, LAT)
         self.out = z(N, 3, dt=f32)
+        self.PQ = z(N, 2 * LAT, dt=f32)
         if train:
             self.d_h = z(N, LAT, dt=f32)
             self.d_agg = z(N, LAT, dt=f32)
@@ -316,15 +363,22 @@ class Network:
         self._ensure(g.N, g.E, train)
         Lb, s, N, E = glib(), stream(), g.N, g.E
         F = self.flat
+        # the hidden layers are kept for the backward pass only
         fwd = lambda mlp, inp, res, hid, out, outf: _ok(
-            Lb.gnn_mlp_forward(C.byref(inp), C.byref(mlp.weights(F)), ptr(res), ptr(hid), ptr(out), ptr(outf),
-                               0, s), "mlp forward")
+            Lb.gnn_mlp_forward(C.byref(inp), C.byref(mlp.weights(F)), ptr(res), ptr(hid) if train else None,
+                               ptr(out), ptr(outf), 0, s), "mlp forward")
         fwd(self.enc_n, io(N, (xn, None)), None, self.hid_enc_n, self.h[0], None)
         if E > 0:
             fwd(self.enc_e, io(E, (xe, None)), None, self.hid_enc_e, self.e[0], None)
         for l in range(self.L):
             h, e = self.h[l], self.e[l]
-            if E > 0:
+            if E > 0 and self.factorised:
+                w = self.edge[l].weights(F)
+                _ok(Lb.gnn_node_pq(N, ptr(h), C.byref(w), ptr(self.PQ), s), "node PQ")
+                _ok(Lb.gnn_edge_mlp_forward(E, ptr(e), ptr(g.recv), ptr(g.send), ptr(self.PQ), C.byref(w),
+                                            ptr(self.hid_e[l]) if train else None, ptr(self.e[l + 1]), 0, s),
+                    "edge MLP forward")
+            elif E > 0:
                 fwd(self.edge[l], io(E, (e[:E], None), (h, g.recv), (h, g.send)), e, self.hid_e[l],
                     self.e[l + 1], None)
             _ok(Lb.gnn_segment_sum(ptr(self.e[l + 1]), ptr(g.row), N, ptr(self.agg[l]), s), "segment sum")
@@ -509,7 +563,7 @@ class Trainer:
     def sample(self, k, t):
         torch = _torch()
         tr = self.trajs[k]
-        g = self.builders[k].build(tr.x[t])
+        g = self.builders[k].build(tr.x_dev[t])
         vh = tr.v_dev[t - HIST + 1:t + 1].contiguous()
         xn = torch.empty((g.N, FEAT), dtype=torch.bfloat16, device="cuda")
         xe = torch.empty((max(g.E, 1), FEAT), dtype=torch.bfloat16, device="cuda")
@@ -590,7 +644,7 @@ class Rollout:
 
     def surrogate_step(self):
         torch = _torch()
-        g = self.builder.build(self.x.cpu().numpy())
+        g = self.builder.build(self.x)
         if self.xe is None or self.xe.shape[0] < max(g.E, 1):
             self.xe = torch.empty((max(g.E, 1) * 5 // 4, FEAT), dtype=torch.bfloat16, device="cuda")
         features(g, self.x, self.vhist, self.rad, self.planes, self.n_planes, self.sc, self.xn, self.xe)
@@ -604,12 +658,26 @@ class Rollout:
         self.dem.download(self.st, fields=("x", "u"))
         return self.st.x.copy(), self.st.u.astype(np.float32)
 
+    def dem_advance(self, x, vhist, n_macro):
+        """n_macro macro-steps of DEM from positions x and the last velocity
+        of vhist (host arrays); returns the positions and the updated
+        velocity history."""
+        dem_state_to(self.dem, x, vhist[-1], self.n)
+        vh = list(vhist)
+        for _ in range(n_macro):
+            self.dem.step(MACRO)
+            self.dem.download(self.st, fields=("x", "u"))
+            vh = vh[1:] + [self.st.u.astype(np.float32)]
+        return self.st.x.copy(), np.stack(vh)
+
     def run(self, n_macro):
         torch = _torch()
         self.warm_up()
-        errs, fallbacks = [], 0
+        errs, fallbacks, dem_failures = [], 0, 0
         t_sur = t_dem = 0.
         n_sur = n_dem = 0
+        # the state the run last continued from after a check (host copies)
+        good = (self.x.cpu().numpy().copy(), self.vhist.cpu().numpy().copy(), 0)
         for m in range(n_macro):
             if m % self.check_every == 0:
                 x0 = self.x.cpu().numpy().copy()
@@ -621,15 +689,30 @@ class Rollout:
                 t_sur += time.perf_counter() - t0
                 n_sur += 1
                 t0 = time.perf_counter()
-                xd, vd = self.dem_step_from(x0, v0)
+                try:
+                    xd, vd = self.dem_step_from(x0, v0)
+                    err = float(np.linalg.norm(self.x.cpu().numpy() - xd, axis=1).mean()) / self.r_mean
+                except abi.DemError:
+                    # the surrogate's state is not one the DEM can continue
+                    # from (e.g. deep overlaps): a failed check
+                    xd = vd = None
+                    err = float("inf")
+                    dem_failures += 1
                 t_dem += time.perf_counter() - t0
                 n_dem += 1
-                err = float(np.linalg.norm(self.x.cpu().numpy() - xd, axis=1).mean()) / self.r_mean
                 errs.append(err)
                 if err > self.tol:
                     fallbacks += 1
-                    self.x.copy_(torch.from_numpy(xd))
-                    self.vhist[-1].copy_(torch.from_numpy(vd))
+                    if xd is not None:
+                        self.x.copy_(torch.from_numpy(xd))
+                        self.vhist[-1].copy_(torch.from_numpy(vd))
+                    else:
+                        # the DEM redoes the window since the last good state
+                        xg, vg, mg = good
+                        xd, vh = self.dem_advance(xg, vg, m + 1 - mg)
+                        self.x.copy_(torch.from_numpy(xd))
+                        self.vhist.copy_(torch.from_numpy(vh))
+                good = (self.x.cpu().numpy().copy(), self.vhist.cpu().numpy().copy(), m + 1)
             else:
                 torch.cuda.synchronize()
                 t0 = time.perf_counter()
@@ -637,8 +720,10 @@ class Rollout:
                 torch.cuda.synchronize()
                 t_sur += time.perf_counter() - t0
                 n_sur += 1
+        fin = [e for e in errs if np.isfinite(e)]
         return {"macro_steps": n_macro, "checks": len(errs), "fallbacks": fallbacks,
-                "mean_position_error_over_r": float(np.mean(errs)) if errs else None,
-                "max_position_error_over_r": float(np.max(errs)) if errs else None,
+                "dem_check_failures": dem_failures,
+                "mean_position_error_over_r": float(np.mean(fin)) if fin else None,
+                "max_position_error_over_r": float(np.max(fin)) if fin else None,
                 "surrogate_ms_per_macro": 1e3 * t_sur / max(n_sur, 1),
                 "dem_ms_per_macro": 1e3 * t_dem / max(n_dem, 1)}

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define GNN_ABI_VERSION 1
+#define GNN_ABI_VERSION 2
 #define GNN_LATENT 128
 
 /* Input rows of an MLP: the concatenation of up to three BF16 sources;
@@ -57,7 +57,10 @@ int gnn_abi_version(void);
 
 /* out = residual + MLP(x); hidden (bf16 [M][128], optional) keeps relu(W1 x + b1)
  * for the backward pass; out (bf16 [M][nout]) and/or out_f32 (float [M][nout]).
- * grid_cap > 0 limits the workgroups (persistent tiles of 128 rows). */
+ * Ungathered inputs of width 32, 128 or 128 + 128 with a 128-wide BF16
+ * output (and no out_f32) run on a persistent kernel that keeps the weights
+ * in LDS; everything else, or grid_cap > 0, on the row-tiled kernel
+ * (grid_cap then limits its workgroups; tiles of 128 rows). */
 int gnn_mlp_forward(const gnn_mlp_io* io, const gnn_mlp_weights* w, const void* residual,
                     void* hidden, void* out, float* out_f32, int grid_cap, void* stream);
 /* dH (bf16 [M][128]) = (dY W2^T) masked by hidden > 0; dX[q] (float
@@ -105,6 +108,45 @@ int gnn_integrate(int N, const float* a, float a_scale, const double* gravity, d
                   float* vhist, int H, void* stream);
 /* dY = 2 (y - t) / count, part[b] = partial sums of (y - t)^2 per block of 256. */
 int gnn_mse(const float* y, const float* t, long count, float* dY, float* part, void* stream);
+
+/* The edge MLP of a message-passing layer, inputs (e, h[recv], h[send]),
+ * with its first layer factorised: W1 = [W1a; W1b; W1c] (K = 384, nout 128)
+ * splits into e W1a + (h W1b)[recv] + (h W1c)[send].
+ * gnn_node_pq: PQ (float [N][256]) = [h W1b | h W1c] for the node latents h
+ * (bf16 [N][128]).
+ * gnn_edge_mlp_forward: out[e] = e[e] + W2 relu(e[e] W1a + PQ[recv[e]][0:128]
+ * + PQ[send[e]][128:256] + b1) + b2 (bf16 [E][128]); hidden (bf16 [E][128],
+ * optional) as in gnn_mlp_forward.  The same function as gnn_mlp_forward
+ * over the three sources, up to FP32 summation order.  grid_cap > 0 limits
+ * the persistent workgroups (default one per CU). */
+int gnn_node_pq(int N, const void* h, const gnn_mlp_weights* w, float* PQ, void* stream);
+int gnn_edge_mlp_forward(long E, const void* e, const int* recv, const int* send, const float* PQ,
+                         const gnn_mlp_weights* w, void* hidden, void* out, int grid_cap,
+                         void* stream);
+
+/* Radius graph on the device: the edges j -> i for every pair i != j of the
+ * N points x (double [N][3]) with |x_i - x_j|^2 <= radius^2 (the DEM
+ * neighbour criterion), in the receiver-major layout above with each node's
+ * senders ascending.  Points are binned into cells of side > radius over the
+ * box [lo, hi] (host double[3] each; points outside are clamped into the
+ * edge cells, which keeps the result exact and only costs time).
+ *
+ * gnn_radius_graph_workspace: bytes of device workspace for N points, or -1
+ * for a degenerate box or one needing more than 2^26 cells.
+ * gnn_radius_graph_count: bins the points and fills row (int [N+1]); writes
+ * the edge count E to *n_edges (host) after synchronising `stream`; -3 when
+ * E does not fit an int.
+ * gnn_radius_graph_fill: with the same arguments and the workspace as left by
+ * the count, fills send, recv and rev (int [E] each; rev[e] is the edge
+ * recv[e] -> send[e]).  n_edges is the count's E (the capacity of the three
+ * arrays). */
+long gnn_radius_graph_workspace(int N, const double lo[3], const double hi[3], double radius);
+int gnn_radius_graph_count(int N, const double* x, double radius, const double lo[3],
+                           const double hi[3], void* work, long work_bytes, int* row,
+                           long* n_edges, void* stream);
+int gnn_radius_graph_fill(int N, const double* x, double radius, const double lo[3],
+                          const double hi[3], const void* work, long work_bytes, const int* row,
+                          long n_edges, int* send, int* recv, int* rev, void* stream);
 
 #ifdef __cplusplus
 }

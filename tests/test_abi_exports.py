@@ -11,14 +11,31 @@ from cabanadem_amd import abi
 ROOT = Path(__file__).resolve().parents[1]
 
 
-def declared_symbols():
+def declared_symbols(prefix="dem_"):
     names = set()
     for h in (ROOT / "include").glob("*.h"):
         txt = h.read_text()
         txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-        for m in re.finditer(r"^\s*[A-Za-z_][\w \*]*?\b(dem_\w+)\s*\(", txt, flags=re.M):
+        for m in re.finditer(r"^\s*[A-Za-z_][\w \*]*?\b(" + prefix + r"\w+)\s*\(", txt, flags=re.M):
             names.add(m.group(1))
     return names
+
+
+def test_surrogate_library_exports_every_declared_symbol():
+    from cabanadem_amd import surrogate as S
+    lib = ctypes.CDLL(str(S.GNN_LIB_PATH))
+    names = declared_symbols("gnn_")
+    assert {"gnn_mlp_forward", "gnn_radius_graph_count", "gnn_radius_graph_fill"} <= names
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+    assert names == set(S._SIG)
+    L = S.glib()
+    assert L.gnn_abi_version() == S.GNN_ABI_VERSION == 2
+    # host-side argument checks answer without a device
+    lo, hi = (ctypes.c_double * 3)(0, 0, 0), (ctypes.c_double * 3)(1, 1, 1)
+    assert L.gnn_radius_graph_workspace(100, lo, hi, 0.1) > 0
+    assert L.gnn_radius_graph_workspace(100, lo, lo, 0.1) == -1
+    assert L.gnn_radius_graph_workspace(100, lo, hi, 1e-9) == -1  # beyond 2^26 cells
 
 
 def test_library_exports_every_declared_symbol():

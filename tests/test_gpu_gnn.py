@@ -77,6 +77,80 @@ def test_fused_mlp_forward_with_gathers(gpu_lib, M):
     assert rel(hid.float().cpu(), hr) < 1e-2
 
 
+@pytest.mark.parametrize("M", [1, 77, 1000, 4133, 70001])
+def test_factorised_edge_mlp_matches_gathered_form(gpu_lib, M):
+    """gnn_node_pq + gnn_edge_mlp_forward against gnn_mlp_forward over the
+    three gathered sources (the same function up to FP32 summation order)
+    and against the FP32 reference."""
+    rng = np.random.default_rng(M + 1)
+    Nn = 97 if M < 10000 else 5000
+    e = bf(rng.normal(size=(M, 128)))
+    h = bf(rng.normal(size=(Nn, 128)))
+    ia = torch.from_numpy(rng.integers(0, Nn, size=M).astype(np.int32))
+    ib = torch.from_numpy(rng.integers(0, Nn, size=M).astype(np.int32))
+    m, fl, flat = make_mlp(384, 128, seed=M + 2)
+    ec, hc, iac, ibc = e.cuda(), h.cuda(), ia.cuda(), ib.cuda()
+    Lb, s = S.glib(), S.stream()
+    w = m.weights(fl)
+    PQ = torch.empty((Nn, 256), dtype=torch.float32, device="cuda")
+    out = torch.empty((M, 128), dtype=torch.bfloat16, device="cuda")
+    hid = torch.empty((M, 128), dtype=torch.bfloat16, device="cuda")
+    assert Lb.gnn_node_pq(Nn, S.ptr(hc), w, S.ptr(PQ), s) == 0
+    assert Lb.gnn_edge_mlp_forward(M, S.ptr(ec), S.ptr(iac), S.ptr(ibc), S.ptr(PQ), w, S.ptr(hid),
+                                   S.ptr(out), 0, s) == 0
+    out_g = torch.empty((M, 128), dtype=torch.bfloat16, device="cuda")
+    hid_g = torch.empty((M, 128), dtype=torch.bfloat16, device="cuda")
+    assert Lb.gnn_mlp_forward(S.io(M, (ec, None), (hc, iac), (hc, ibc)), w, S.ptr(ec), S.ptr(hid_g),
+                              S.ptr(out_g), None, 0, s) == 0
+    torch.cuda.synchronize()
+    # P and Q: FP32 sums of exact BF16 products, against FP64 on the host
+    W1 = bf(flat[m.o_W1:m.o_b1].reshape(384, 128)).double()
+    hd = h.double()
+    assert rel(PQ.cpu(), torch.cat([hd @ W1[128:256], hd @ W1[256:]], 1)) < 1e-5
+    assert rel(out.float().cpu(), out_g.float().cpu()) < 5e-3
+    assert rel(hid.float().cpu(), hid_g.float().cpu()) < 5e-3
+    x = torch.cat([e.float(), h.float()[ia.long()], h.float()[ib.long()]], dim=1)
+    y, hr = torch_mlp(x, m, flat, res=e.float())
+    assert rel(out.float().cpu(), y) < 1e-2
+    assert rel(hid.float().cpu(), hr) < 1e-2
+    # a capped persistent grid gives the same bits
+    out2 = torch.empty_like(out)
+    assert Lb.gnn_edge_mlp_forward(M, S.ptr(ec), S.ptr(iac), S.ptr(ibc), S.ptr(PQ), w, None, S.ptr(out2),
+                                   3, s) == 0
+    torch.cuda.synchronize()
+    assert torch.equal(out2, out)
+
+
+@pytest.mark.parametrize("K,M", [(32, 1), (32, 70001), (128, 77), (256, 5000), (256, 64001)])
+def test_dense_mlp_kernel_matches_row_tiled_kernel(gpu_lib, K, M):
+    """Ungathered inputs with a 128-wide BF16 output take the persistent
+    dense kernel (grid_cap 0); a grid cap selects the row-tiled kernel.  Both
+    against each other (FP32 summation order apart) and the FP32 reference."""
+    rng = np.random.default_rng(K * 7 + M)
+    widths = [K] if K <= 128 else [128, 128]
+    xs = [bf(rng.normal(size=(M, wq))) for wq in widths]
+    res = bf(rng.normal(size=(M, 128)))
+    m, fl, flat = make_mlp(K, 128, seed=K + 3)
+    xc = [x.cuda() for x in xs]
+    rc_ = res.cuda()
+    io = S.io(M, *[(x, None) for x in xc])
+    outs = []
+    for cap in (0, 1 << 20):
+        out = torch.empty((M, 128), dtype=torch.bfloat16, device="cuda")
+        hid = torch.empty((M, 128), dtype=torch.bfloat16, device="cuda")
+        assert S.glib().gnn_mlp_forward(io, m.weights(fl), S.ptr(rc_), S.ptr(hid), S.ptr(out), None, cap,
+                                        S.stream()) == 0
+        outs.append((out, hid))
+    torch.cuda.synchronize()
+    (od, hd), (og, hg) = outs
+    assert rel(od.float().cpu(), og.float().cpu()) < 5e-3
+    assert rel(hd.float().cpu(), hg.float().cpu()) < 5e-3
+    x = torch.cat([t.float() for t in xs], dim=1)
+    y, hr = torch_mlp(x, m, flat, res=res.float())
+    assert rel(od.float().cpu(), y) < 1e-2
+    assert rel(hd.float().cpu(), hr) < 1e-2
+
+
 def test_decoder_forward_fp32_out(gpu_lib):
     M = 513
     rng = np.random.default_rng(5)
@@ -136,11 +210,73 @@ def test_fused_mlp_backward_and_weight_grads(gpu_lib, K, nout, M):
     assert torch.all(dW2.cpu()[:, nout:] == 0)
 
 
+def brute_force_graph(x, radius):
+    """(row, send, recv, rev) of the pairs |x_i - x_j|^2 <= r^2, i != j, by
+    all-pairs distances in the kernel's operation order."""
+    d = x[:, None, :] - x[None, :, :]
+    d2 = (d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2]
+    adj = (d2 <= radius * radius) & ~np.eye(len(x), dtype=bool)
+    counts = adj.sum(1)
+    row = np.concatenate([[0], np.cumsum(counts)])
+    recv, send = np.nonzero(adj)      # row-major: receivers ascending, senders ascending
+    key = recv.astype(np.int64) * len(x) + send
+    rev = np.searchsorted(key, send.astype(np.int64) * len(x) + recv)
+    return row, send, recv, rev
+
+
+@pytest.mark.parametrize("case", ["random", "outside_and_coincident", "exact_radius_lattice", "no_edges"])
+def test_device_radius_graph_matches_brute_force(gpu_lib, case):
+    rng = np.random.default_rng(11)
+    lo, hi, radius = (0., 0., 0.), (12e-3, 12e-3, 12e-3), 2.5e-3
+    if case == "random":
+        x = rng.uniform(0, 12e-3, size=(3000, 3))
+    elif case == "outside_and_coincident":
+        x = rng.uniform(-3e-3, 15e-3, size=(1500, 3))
+        x[100:120] = x[0]                    # coincident points are neighbours (d = 0)
+    elif case == "exact_radius_lattice":
+        # spacing exactly the radius (binary fractions): axis pairs at d^2 == r^2
+        radius = 0.25
+        lo, hi = (0., 0., 0.), (2.0, 2.0, 2.0)
+        g = np.arange(8) * 0.25
+        x = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    else:
+        x = np.stack([np.arange(50) * 1e-2, np.zeros(50), np.zeros(50)], -1)
+        hi = (0.6, 1e-2, 1e-2)
+    gb = S.GraphBuilder(len(x), radius, lo, hi)
+    gr = gb.build(x)
+    torch.cuda.synchronize()
+    row, send, recv, rev = brute_force_graph(x, radius)
+    assert gr.E == len(send)
+    assert np.array_equal(gr.row.cpu().numpy(), row)
+    if gr.E:
+        assert np.array_equal(gr.send.cpu().numpy(), send)
+        assert np.array_equal(gr.recv.cpu().numpy(), recv)
+        assert np.array_equal(gr.rev.cpu().numpy(), rev)
+    if case == "exact_radius_lattice":
+        assert gr.E == 6 * 8 ** 3 - 6 * 8 * 8  # 6 axis neighbours, less one per node on each face
+    # a second build on the same builder (buffers reused) gives the same graph
+    g2 = gb.build(torch.from_numpy(x).cuda())
+    assert g2.E == gr.E and np.array_equal(g2.row.cpu().numpy(), row)
+
+
 def radius_graph(n, seed, radius=2.5e-3):
     rng = np.random.default_rng(seed)
     x = rng.uniform(0, 12e-3, size=(n, 3))
     gb = S.GraphBuilder(n, radius, (-1e-3,) * 3, (13e-3,) * 3)
     return gb.build(x), x
+
+
+def test_network_factorised_and_gathered_edge_forward_agree(gpu_lib):
+    g, _ = radius_graph(2000, seed=4)
+    rng = np.random.default_rng(5)
+    xn = np.zeros((g.N, 32)); xn[:, :20] = rng.normal(size=(g.N, 20))
+    xe = np.zeros((g.E, 32)); xe[:, :4] = rng.normal(size=(g.E, 4))
+    xn_d, xe_d = bf(xn).cuda(), bf(xe).cuda()
+    a = S.Network(n_layers=4, seed=6, residual_scale=0.5, factorised=True)
+    b = S.Network(n_layers=4, seed=6, residual_scale=0.5, factorised=False)
+    ya = a.forward(g, xn_d, xe_d).cpu().numpy()
+    yb = b.forward(g, xn_d, xe_d).cpu().numpy()
+    assert rel(ya, yb) < 1e-2
 
 
 def test_network_forward_backward_match_oracle(gpu_lib):
@@ -162,13 +298,38 @@ def test_network_forward_backward_match_oracle(gpu_lib):
     net.backward(g, xn_d, xe_d, torch.from_numpy(d_out).cuda())
     torch.cuda.synchronize()
     got = net.grad.cpu().numpy().astype(np.float64)
+    print(f"forward rel {rel(out, want):.2e}")
+    # The backward alone: the oracle's backward over the GPU's own stored
+    # forward (its BF16 latents, aggregations and hidden layers), with the
+    # backward's BF16 operands where the GPU has them.
+    refq = GO.Net(flat, L, quant=True, quant_grad=True)
+    refq.forward(row, send, recv, GO.bf16(xn), GO.bf16(xe))
+    N, E = g.N, g.E
+    f64 = lambda t, n: t[:n].float().cpu().numpy().astype(np.float64)
+    refq.c["h"] = [f64(t, N) for t in net.h]
+    refq.c["e"] = [f64(t, E) for t in net.e]
+    refq.c["agg"] = [f64(t, N) for t in net.agg]
+    refq.c["hid_e"] = [f64(t, E) for t in net.hid_e]
+    refq.c["hid_n"] = [f64(t, N) for t in net.hid_n]
+    refq.c["hid_enc_n"], refq.c["hid_enc_e"] = f64(net.hid_enc_n, N), f64(net.hid_enc_e, E)
+    refq.c["hid_dec"] = f64(net.hid_dec, N)
+    gq = refq.backward(d_out.astype(np.float64))
+    # and the exact FP64 backward of the oracle's own forward
     gw = ref.backward(d_out.astype(np.float64))
-    assert rel(got, gw) < 5e-2
+    names = ["enc_n", "enc_e"] + [f"edge{l}" for l in range(L)] + [f"node{l}" for l in range(L)] + ["dec"]
+    for name, (K, nout, NP, o) in zip(names, GO.layout(L)[0]):
+        end = o + K * 128 + 128 + 128 * NP + NP
+        print(f"{name}: rel vs oracle backward of the GPU forward {rel(got[o:end], gq[o:end]):.2e}, "
+              f"vs FP64 oracle {rel(got[o:end], gw[o:end]):.2e}")
+        assert np.abs(got[o:o + K * 128]).max() > 0  # every MLP receives gradient
+    # the backward alone agrees to the FP32 summation order (observed <= 3e-3
+    # per MLP); against the FP64 oracle the forward's BF16 roundings differ
+    # and the gradients move by ~5 % (oracle-side sensitivity test in
+    # tests/test_gnn_oracle.py)
+    assert rel(got, gq) < 1e-2
+    assert rel(got, gw) < 1e-1
     cos = got @ gw / (np.linalg.norm(got) * np.linalg.norm(gw))
-    assert cos > 0.998
-    # every MLP received gradient
-    for K, nout, NP, o in GO.layout(L)[0]:
-        assert np.abs(got[o:o + K * 128]).max() > 0
+    assert cos > 0.995
 
 
 def small_column_data(n_traj=2, frames=40, dims=(10, 10, 10)):
