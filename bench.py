@@ -254,10 +254,14 @@ def run_ensemble(args):
 
 
 def forward_flops(L, N, E):
-    """Multiply-add FLOPs of one network evaluation (2 per MAC)."""
+    """FLOPs the forward pass executes on the MFMA units (2 per MAC): per
+    layer, each edge multiplies e and h[send] by W1a and W1c and its hidden
+    layer by W2 (3 x 128^2), each node forms P = h W1b (128^2) and runs its
+    node MLP ((256 + 128) x 128); plus the encoders and the decoder (output
+    padded to 16)."""
     lat = 128
     enc = N * (32 * lat + lat * lat) + E * (32 * lat + lat * lat)
-    layers = L * (E * (3 * lat * lat + lat * lat) + N * (2 * lat * lat + lat * lat))
+    layers = L * (E * 3 * lat * lat + N * (lat * lat + 2 * lat * lat + lat * lat))
     dec = N * (lat * lat + lat * 16)
     return 2.0 * (enc + layers + dec)
 

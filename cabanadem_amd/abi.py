@@ -15,7 +15,7 @@ import numpy as np
 LIB_PATH = Path(os.environ.get("CABANADEM_LIB") or
                 Path(__file__).resolve().parent / "lib" / "libcabanadem_hip.so")
 
-DEM_ABI_VERSION = 2  # include/cabanadem.h DEM_ABI_VERSION
+DEM_ABI_VERSION = 3  # include/cabanadem.h DEM_ABI_VERSION
 DEM_MAX_WALLS = 8
 DEM_MAX_GROUPS = 8
 DEM_MAX_BONDS = 32
@@ -137,6 +137,8 @@ SYMBOLS = {
     "dem_build_neighbors": (C.c_int, [C.c_void_p]),
     "dem_get_neighbors": (C.c_int, [C.c_void_p, _ip, _ip, C.c_int]),
     "dem_zero_forces": (C.c_int, [C.c_void_p]),
+    "dem_set_kinematics_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
+    "dem_get_kinematics_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "dem_update_tangential": (C.c_int, [C.c_void_p]),
     "dem_force_pp": (C.c_int, [C.c_void_p]),
     "dem_force_pw": (C.c_int, [C.c_void_p]),
@@ -403,6 +405,14 @@ class System:
         return counts, ids
 
     def zero_forces(self): self._check(self._lib.dem_zero_forces(self._h))
+
+    def set_kinematics_device(self, x_ptr, u_ptr, clear_history=True):
+        """Positions / velocities from device arrays (double [n][3], reference
+        order; addresses as ints or None), on the context's stream."""
+        self._check(self._lib.dem_set_kinematics_device(self._h, x_ptr, u_ptr, 1 if clear_history else 0))
+
+    def get_kinematics_device(self, x_ptr, u_ptr):
+        self._check(self._lib.dem_get_kinematics_device(self._h, x_ptr, u_ptr))
     def update_tangential(self): self._check(self._lib.dem_update_tangential(self._h))
     def force_pp(self): self._check(self._lib.dem_force_pp(self._h))
     def force_pw(self): self._check(self._lib.dem_force_pw(self._h))

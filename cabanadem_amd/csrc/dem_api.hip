@@ -979,6 +979,23 @@ int dem_get_neighbors(dem_ctx* c, int* counts, int* ids, int max_n) {
   return DEM_OK;
 }
 
+int dem_set_kinematics_device(dem_ctx* c, const double* x, const double* u, int clear_history) {
+  if (int r = check_ctx(c)) return r;
+  if (c->bonds_on && clear_history)
+    return c->fail(DEM_ERR_STATE, "clearing histories of a bonded context is not supported");
+  hipError_t e = launch_set_kinematics(c->n, c->id[c->cS], x, u, clear_history, c->xr[c->cP], c->uw[c->cP],
+                                       c->ft[c->cS], c->cmeta[c->cC], c->stream);
+  if (e != hipSuccess) return c->hip_fail(e, "set kinematics");
+  if (x) c->list_valid = false;
+  return DEM_OK;
+}
+
+int dem_get_kinematics_device(dem_ctx* c, double* x, double* u) {
+  if (int r = check_ctx(c)) return r;
+  hipError_t e = launch_get_kinematics(c->n, c->id[c->cS], c->xr[c->cP], c->uw[c->cP], x, u, c->stream);
+  return e == hipSuccess ? DEM_OK : c->hip_fail(e, "get kinematics");
+}
+
 int dem_zero_forces(dem_ctx* c) {
   if (int r = check_ctx(c)) return r;
   hipError_t e = launch_zero_forces(c->step_args(), c->stream);

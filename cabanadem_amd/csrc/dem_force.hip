@@ -606,6 +606,48 @@ __global__ __launch_bounds__(kBlock) void k_zero_forces(StepArgs a) {
   store6(a.ft + 6l * i, ft);
 }
 
+// Positions and velocities from device arrays in the reference order (slot
+// p holds particle id[p]; either array may be null); with clear, spins,
+// forces, torques and every contact history start from zero (meta 0: no
+// tracked contact, no wall bit).
+__global__ __launch_bounds__(kBlock) void k_set_kinematics(int n, const int* id, const double* x,
+                                                           const double* u, int clear, double4* xr,
+                                                           double* uw, double* ft, int* cmeta) {
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= n) return;
+  const long I = id[p];
+  if (x) {
+    double4 v = xr[p];
+    v.x = x[3 * I];
+    v.y = x[3 * I + 1];
+    v.z = x[3 * I + 2];
+    xr[p] = v;
+  }
+  if (u)
+    for (int d = 0; d < 3; ++d) uw[6l * p + d] = u[3 * I + d];
+  if (clear) {
+    for (int d = 0; d < 3; ++d) uw[6l * p + 3 + d] = 0.;
+    for (int c = 0; c < 6; ++c) ft[6l * p + c] = 0.;
+    cmeta[p] = make_meta(0, 0);
+  }
+}
+
+// Positions and velocities to device arrays in the reference order.
+__global__ __launch_bounds__(kBlock) void k_get_kinematics(int n, const int* id, const double4* xr,
+                                                           const double* uw, double* x, double* u) {
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= n) return;
+  const long I = id[p];
+  if (x) {
+    const double4 v = xr[p];
+    x[3 * I] = v.x;
+    x[3 * I + 1] = v.y;
+    x[3 * I + 2] = v.z;
+  }
+  if (u)
+    for (int d = 0; d < 3; ++d) u[3 * I + d] = uw[6l * p + d];
+}
+
 // src/CabanaDEM_Force.hpp:343-445: keep the tracked contacts that still
 // overlap.  The list stays ascending in j; the reference's swap-remove slot
 // order is not kept (the history is compared as a map keyed by (i, j)).
@@ -764,6 +806,19 @@ hipError_t launch_force(ForceKind kind, const StepArgs& a, bool sample, int grid
 
 hipError_t launch_zero_forces(const StepArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_zero_forces, dim3(nblk(a.n)), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_set_kinematics(int n, const int* id, const double* x, const double* u, int clear,
+                                 double4* xr, double* uw, double* ft, int* cmeta, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_kinematics, dim3(nblk(n)), dim3(kBlock), 0, s, n, id, x, u, clear, xr, uw, ft,
+                     cmeta);
+  return hipGetLastError();
+}
+
+hipError_t launch_get_kinematics(int n, const int* id, const double4* xr, const double* uw, double* x,
+                                 double* u, hipStream_t s) {
+  hipLaunchKernelGGL(k_get_kinematics, dim3(nblk(n)), dim3(kBlock), 0, s, n, id, xr, uw, x, u);
   return hipGetLastError();
 }
 

@@ -237,6 +237,10 @@ hipError_t launch_force(ForceKind kind, const StepArgs& a, bool sample, int grid
                         hipStream_t s);
 hipError_t launch_zero_forces(const StepArgs& a, hipStream_t s);
 hipError_t launch_update_tangential(const StepArgs& a, hipStream_t s);
+hipError_t launch_set_kinematics(int n, const int* id, const double* x, const double* u, int clear,
+                                 double4* xr, double* uw, double* ft, int* cmeta, hipStream_t s);
+hipError_t launch_get_kinematics(int n, const int* id, const double4* xr, const double* uw, double* x,
+                                 double* u, hipStream_t s);
 hipError_t launch_kick(const StepArgs& a, double4* xr, double* uw, const double* ft,
                        int do_kick, int do_drift, hipStream_t s);
 // Per-tile partial sums of the current state (one partial row per tile).

@@ -359,30 +359,31 @@ __global__ __launch_bounds__(kThreads) void k_mlp_bwd(BwdArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Edge MLP with its first layer factorised over the three sources:
-//   pre = e W1a + (h W1b)[recv] + (h W1c)[send] + b1,
+// Edge MLP with the receiver's share of its first layer precomputed per node:
+//   pre = e W1a + (h W1b)[recv] + h[send] W1c + b1,
 // W1 = [W1a; W1b; W1c] (rows 0..127, 128..255, 256..383 of W1 [384][128]).
-// k_node_pq forms PQ = [h W1b | h W1c] (float [N][256]) once per layer, so
-// the edge kernel multiplies K = 128 instead of 384 and gathers two FP32
-// rows per edge instead of two BF16 rows per k step.
+// k_node_p forms P = h W1b (float [N][128]) once per layer; consecutive edges
+// share their receiver (the list is receiver-major), so P rows are read from
+// cache.  The sender's share goes through the MFMA units from the gathered
+// BF16 row h[send] (256 B per edge, the node table fits an XCD's L2).
 //
-// The edge kernel computes transposed products, hidden^T = W1a^T e^T and
-// out^T = W2^T hidden^T: the weights are the A operand, held in registers
-// for the kernel's whole persistent loop, and 16 edges are the columns of
-// each MFMA tile, so a lane's 4 accumulators are 4 consecutive units of one
-// edge (one float4 of P, of Q and of each bias per lane and tile).
+// The edge kernel computes transposed products, hidden^T = W1a^T e^T +
+// W1c^T h[send]^T and out^T = W2^T hidden^T: the weights are the A operand,
+// in LDS for the block, and 16 edges are the columns of each MFMA tile, so a
+// lane's 4 accumulators are 4 consecutive units of one edge (one float4 of
+// P and of each bias per lane and tile).
 // ---------------------------------------------------------------------------
 
-struct NodePQArgs {
+struct NodePArgs {
   int N;
   const bf16* h;     // [N][128]
   const bf16* W1t;   // [128][384]: row = hidden unit
-  float* PQ;         // [N][256]
+  float* P;          // [N][128]
 };
 
-// PQ[n][c] = sum_k h[n][k] W1[128 + k][c] for c < 128, W1[256 + k][c - 128]
-// above.  A wave owns 32 nodes; the block's 4 waves 128.
-__global__ __launch_bounds__(kThreads) void k_node_pq(NodePQArgs a) {
+// P[n][c] = sum_k h[n][k] W1[128 + k][c].  A wave owns 32 nodes; the
+// output goes through LDS so that whole rows are stored.
+__global__ __launch_bounds__(kThreads) void k_node_p(NodePArgs a) {
   __shared__ float sp[4][kWaveRows][kLat + 4]; // output staging per wave
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int lr = lane & 15, lk = 8 * (lane >> 4);
@@ -394,43 +395,37 @@ __global__ __launch_bounds__(kThreads) void k_node_pq(NodePQArgs a) {
     const int r = r0 + rb * 16 + lr;
     rows[rb] = r < a.N ? r : a.N - 1;
   }
-  bf16x8 av[2][4];
+  f32x4 acc[2][8];
 #pragma unroll
   for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
-    for (int ks = 0; ks < 4; ++ks) av[rb][ks] = ld8(a.h + (long)rows[rb] * kLat + ks * 32 + lk);
+    for (int cb = 0; cb < 8; ++cb) acc[rb][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-  for (int half = 0; half < 2; ++half) { // P, then Q
-    f32x4 acc[2][8];
+  for (int ks = 0; ks < 4; ++ks) {
+    bf16x8 av[2];
 #pragma unroll
-    for (int rb = 0; rb < 2; ++rb)
+    for (int rb = 0; rb < 2; ++rb) av[rb] = ld8(a.h + (long)rows[rb] * kLat + ks * 32 + lk);
 #pragma unroll
-      for (int cb = 0; cb < 8; ++cb) acc[rb][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int ks = 0; ks < 4; ++ks)
-#pragma unroll
-      for (int cb = 0; cb < 8; ++cb) {
-        // B[k][c] = W1[128 (1 + half) + k][c] = W1t[c][128 (1 + half) + k]
-        const bf16x8 bv = ld8(a.W1t + (long)(cb * 16 + lr) * (3 * kLat) + kLat * (1 + half) + ks * 32 + lk);
-        acc[0][cb] = mfma(av[0][ks], bv, acc[0][cb]);
-        acc[1][cb] = mfma(av[1][ks], bv, acc[1][cb]);
-      }
-#pragma unroll
-    for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-      for (int cb = 0; cb < 8; ++cb)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) sp[w][rb * 16 + 4 * (lane >> 4) + r][cb * 16 + lr] = acc[rb][cb][r];
-    wave_lds_sync();
-    // whole half-rows out: 2 rows of 128 floats per instruction
-#pragma unroll
-    for (int it = 0; it < kWaveRows / 2; ++it) {
-      const int r = it * 2 + (lane >> 5), c = (lane & 31) * 4;
-      if (r0 + r < a.N)
-        *reinterpret_cast<float4*>(a.PQ + (long)(r0 + r) * 2 * kLat + half * kLat + c) =
-            *reinterpret_cast<const float4*>(&sp[w][r][c]);
+    for (int cb = 0; cb < 8; ++cb) {
+      // B[k][c] = W1[128 + k][c] = W1t[c][128 + k]
+      const bf16x8 bv = ld8(a.W1t + (long)(cb * 16 + lr) * (3 * kLat) + kLat + ks * 32 + lk);
+      acc[0][cb] = mfma(av[0], bv, acc[0][cb]);
+      acc[1][cb] = mfma(av[1], bv, acc[1][cb]);
     }
-    wave_lds_sync(); // the tile is rewritten by the next half
+  }
+#pragma unroll
+  for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+    for (int cb = 0; cb < 8; ++cb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) sp[w][rb * 16 + 4 * (lane >> 4) + r][cb * 16 + lr] = acc[rb][cb][r];
+  wave_lds_sync();
+  // whole rows out: 2 rows of 128 floats per instruction
+#pragma unroll
+  for (int it = 0; it < kWaveRows / 2; ++it) {
+    const int r = it * 2 + (lane >> 5), c = (lane & 31) * 4;
+    if (r0 + r < a.N)
+      *reinterpret_cast<float4*>(a.P + (long)(r0 + r) * kLat + c) = *reinterpret_cast<const float4*>(&sp[w][r][c]);
   }
 }
 
@@ -439,7 +434,8 @@ struct EdgeFwdArgs {
   const bf16* e;      // [E][128]
   const int* recv;
   const int* send;
-  const float* PQ;    // [N][256]
+  const float* P;     // [N][128]
+  const bf16* h;      // [N][128]
   const bf16* W1t;    // [128][384]
   const float* b1;    // [128]
   const bf16* W2t;    // [128][128]
@@ -448,57 +444,48 @@ struct EdgeFwdArgs {
   bf16* out;          // [E][128]: e + MLP
 };
 
-// One wave tile's operands (16 EB edges): the edge rows, the e^T fragments
-// and the first layer's accumulator start b1 + P[recv] + Q[send].
+// A wave tile of 16 EB edges: their rows, receivers, and the e^T and
+// h[send]^T fragments.
 template <int EB>
 struct EdgeTile {
-  int ed[EB];
-  bf16x8 Bv[EB][4];
-  f32x4 acc[8][EB];
+  int ed[EB], ri[EB];
+  bf16x8 Be[EB][4], Bh[EB][4];
 };
 
 template <int EB>
-__device__ inline void edge_tile_load(const EdgeFwdArgs& a, int e0, int lr, int lg, EdgeTile<EB>& T) {
-  int ri[EB], sj[EB];
+__device__ inline void edge_tile_load(const EdgeFwdArgs& a, int e0, int lr, int lk, EdgeTile<EB>& T) {
 #pragma unroll
   for (int eb = 0; eb < EB; ++eb) {
     const int ee = e0 + eb * 16 + lr;
     T.ed[eb] = ee < a.E ? ee : a.E - 1; // clamped loads; stores are guarded
-    ri[eb] = a.recv[T.ed[eb]];
-    sj[eb] = a.send[T.ed[eb]];
-  }
+    T.ri[eb] = a.recv[T.ed[eb]];
+    const int sj = a.send[T.ed[eb]];
 #pragma unroll
-  for (int eb = 0; eb < EB; ++eb)
-#pragma unroll
-    for (int ks = 0; ks < 4; ++ks) T.Bv[eb][ks] = ld8(a.e + (long)T.ed[eb] * kLat + ks * 32 + 8 * lg);
-#pragma unroll
-  for (int ub = 0; ub < 8; ++ub) {
-    const int u = ub * 16 + 4 * lg;
-    const float4 bb = *reinterpret_cast<const float4*>(a.b1 + u);
-#pragma unroll
-    for (int eb = 0; eb < EB; ++eb) {
-      const float4 p = *reinterpret_cast<const float4*>(a.PQ + (long)ri[eb] * 2 * kLat + u);
-      const float4 q = *reinterpret_cast<const float4*>(a.PQ + (long)sj[eb] * 2 * kLat + kLat + u);
-      T.acc[ub][eb] = f32x4{bb.x + p.x + q.x, bb.y + p.y + q.y, bb.z + p.z + q.z, bb.w + p.w + q.w};
+    for (int ks = 0; ks < 4; ++ks) {
+      T.Be[eb][ks] = ld8(a.e + (long)T.ed[eb] * kLat + ks * 32 + lk);
+      T.Bh[eb][ks] = ld8(a.h + (long)sj * kLat + ks * 32 + lk);
     }
   }
 }
 
 // Persistent and wave-granular: every wave takes tiles of 16 EB edges with
-// stride (waves in the grid); with PF it loads the next tile's operands
-// while it multiplies the current one.  W1a^T and W2^T sit in LDS for the
-// block's WAVES waves, and each wave has its own hidden tile.  With a grid
-// that is a multiple of 8, XCD (blockIdx % 8) takes a contiguous eighth of
-// the tiles, so the P/Q rows its edges gather stay in that XCD's L2.
-template <int EB, int WAVES, bool PF>
+// stride (waves in the grid) and loads the next tile's rows while it
+// multiplies the current one.  W1a^T, W1c^T and W2^T sit in LDS for the
+// block's waves, each weight fragment read once per tile and applied to its
+// EB edge blocks; each wave has its own hidden tile.  With a grid that is a
+// multiple of 8, XCD (blockIdx % 8) takes a contiguous eighth of the tiles,
+// so the rows its edges gather stay in that XCD's L2.
+template <int EB, int WAVES>
 __global__ __launch_bounds__(64 * WAVES, 1) void k_edge_fwd(EdgeFwdArgs a) {
-  constexpr int TE = 16 * EB; // edges per wave tile
-  __shared__ bf16 sw1[kLat][kWPitch];        // W1a^T: [unit][k]
-  __shared__ bf16 sw2[kLat][kWPitch];        // W2^T:  [out][k]
-  __shared__ bf16 sh[WAVES][TE][kWPitch];    // hidden, [edge][unit]
+  constexpr int TE = 16 * EB;
+  __shared__ bf16 swa[kLat][kWPitch];     // W1a^T: [unit][k]
+  __shared__ bf16 swc[kLat][kWPitch];     // W1c^T: [unit][k]
+  __shared__ bf16 sw2[kLat][kWPitch];     // W2^T:  [out][k]
+  __shared__ bf16 sh[WAVES][TE][kWPitch]; // hidden / output, [edge][unit]
   for (int c = threadIdx.x; c < kLat * 16; c += 64 * WAVES) {
     const int r = c >> 4, k8 = (c & 15) * 8;
-    *reinterpret_cast<bf16x8*>(&sw1[r][k8]) = ld8(a.W1t + (long)r * (3 * kLat) + k8);
+    *reinterpret_cast<bf16x8*>(&swa[r][k8]) = ld8(a.W1t + (long)r * (3 * kLat) + k8);
+    *reinterpret_cast<bf16x8*>(&swc[r][k8]) = ld8(a.W1t + (long)r * (3 * kLat) + 2 * kLat + k8);
     *reinterpret_cast<bf16x8*>(&sw2[r][k8]) = ld8(a.W2t + (long)r * kLat + k8);
   }
   __syncthreads(); // the only block barrier: the loop below is per wave
@@ -519,23 +506,34 @@ __global__ __launch_bounds__(64 * WAVES, 1) void k_edge_fwd(EdgeFwdArgs a) {
   }
   if (t >= t_end) return;
   EdgeTile<EB> cur, nxt;
-  if (PF) edge_tile_load<EB>(a, t * TE, lr, lg, cur);
+  edge_tile_load<EB>(a, t * TE, lr, lk, cur);
   for (; t < t_end; t += stride) {
     const int e0 = t * TE;
     const int tn = t + stride;
-    if (PF) {
-      if (tn < t_end) edge_tile_load<EB>(a, tn * TE, lr, lg, nxt);
-    } else {
-      edge_tile_load<EB>(a, e0, lr, lg, cur);
+    if (tn < t_end) edge_tile_load<EB>(a, tn * TE, lr, lk, nxt);
+    // layer 1 on top of b1 + P[recv] (units 16 ub + 4 lg .. +3 of edge 16 eb + lr)
+    f32x4 acc[8][EB];
+#pragma unroll
+    for (int ub = 0; ub < 8; ++ub) {
+      const int u = ub * 16 + 4 * lg;
+      const float4 bb = *reinterpret_cast<const float4*>(a.b1 + u);
+#pragma unroll
+      for (int eb = 0; eb < EB; ++eb) {
+        const float4 p = *reinterpret_cast<const float4*>(a.P + (long)cur.ri[eb] * kLat + u);
+        acc[ub][eb] = f32x4{bb.x + p.x, bb.y + p.y, bb.z + p.z, bb.w + p.w};
+      }
     }
-    // layer 1: hidden^T = W1a^T e^T on top of b1 + P + Q
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks)
 #pragma unroll
       for (int ub = 0; ub < 8; ++ub) {
-        const bf16x8 A = *reinterpret_cast<const bf16x8*>(&sw1[ub * 16 + lr][ks * 32 + lk]);
+        const bf16x8 Aa = *reinterpret_cast<const bf16x8*>(&swa[ub * 16 + lr][ks * 32 + lk]);
+        const bf16x8 Ac = *reinterpret_cast<const bf16x8*>(&swc[ub * 16 + lr][ks * 32 + lk]);
 #pragma unroll
-        for (int eb = 0; eb < EB; ++eb) cur.acc[ub][eb] = mfma(A, cur.Bv[eb][ks], cur.acc[ub][eb]);
+        for (int eb = 0; eb < EB; ++eb) {
+          acc[ub][eb] = mfma(Aa, cur.Be[eb][ks], acc[ub][eb]);
+          acc[ub][eb] = mfma(Ac, cur.Bh[eb][ks], acc[ub][eb]);
+        }
       }
     // relu -> BF16 hidden tile [edge][unit] (and global for the backward)
 #pragma unroll
@@ -544,12 +542,8 @@ __global__ __launch_bounds__(64 * WAVES, 1) void k_edge_fwd(EdgeFwdArgs a) {
       for (int eb = 0; eb < EB; ++eb) {
         bf16x4 hv;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const float v = cur.acc[ub][eb][r];
-          hv[r] = (bf16)(v > 0.f ? v : 0.f);
-        }
-        const int le = eb * 16 + lr, u = ub * 16 + 4 * lg;
-        *reinterpret_cast<bf16x4*>(&sh[w][le][u]) = hv;
+        for (int r = 0; r < 4; ++r) hv[r] = (bf16)(acc[ub][eb][r] > 0.f ? acc[ub][eb][r] : 0.f);
+        *reinterpret_cast<bf16x4*>(&sh[w][eb * 16 + lr][ub * 16 + 4 * lg]) = hv;
       }
     wave_lds_sync();
     if (a.hid) wave_store_tile<TE>(sh[w], a.hid, e0, a.E, lane);
@@ -572,12 +566,12 @@ __global__ __launch_bounds__(64 * WAVES, 1) void k_edge_fwd(EdgeFwdArgs a) {
       for (int eb = 0; eb < EB; ++eb) hb[eb] = *reinterpret_cast<const bf16x8*>(&sh[w][eb * 16 + lr][ks * 32 + lk]);
 #pragma unroll
       for (int ub = 0; ub < 8; ++ub) {
-        const bf16x8 A = *reinterpret_cast<const bf16x8*>(&sw2[ub * 16 + lr][ks * 32 + lk]);
+        const bf16x8 A2 = *reinterpret_cast<const bf16x8*>(&sw2[ub * 16 + lr][ks * 32 + lk]);
 #pragma unroll
-        for (int eb = 0; eb < EB; ++eb) acc2[ub][eb] = mfma(A, hb[eb], acc2[ub][eb]);
+        for (int eb = 0; eb < EB; ++eb) acc2[ub][eb] = mfma(A2, hb[eb], acc2[ub][eb]);
       }
     }
-    // the output through the (now free) hidden tile, then whole rows out
+    // the output through the hidden tile, then whole rows out
     wave_lds_sync();
 #pragma unroll
     for (int ub = 0; ub < 8; ++ub)
@@ -591,7 +585,7 @@ __global__ __launch_bounds__(64 * WAVES, 1) void k_edge_fwd(EdgeFwdArgs a) {
     wave_lds_sync();
     wave_store_tile<TE>(sh[w], a.out, e0, a.E, lane);
     wave_lds_sync(); // the next tile rewrites the tile
-    if (PF && tn < t_end) cur = nxt;
+    if (tn < t_end) cur = nxt;
   }
 }
 
@@ -710,18 +704,16 @@ __global__ __launch_bounds__(64 * WAVES, 1) void k_dense_fwd(DenseFwdArgs a) {
   }
 }
 
-// Launch shape of the edge kernel (the template instance below).
+// Launch shape of the edge kernel: 16 EB edges per wave tile, WAVES waves per
+// block (one block per CU, LDS-limited: three 34 KB weight tiles plus
+// WAVES x 16 EB hidden rows).
 #ifndef GNN_EDGE_EB
-#define GNN_EDGE_EB 2
+#define GNN_EDGE_EB 1
 #endif
 #ifndef GNN_EDGE_WAVES
 #define GNN_EDGE_WAVES 8
 #endif
-#ifndef GNN_EDGE_PF
-#define GNN_EDGE_PF 1
-#endif
 constexpr int kEdgeEB = GNN_EDGE_EB, kEdgeWaves = GNN_EDGE_WAVES;
-constexpr bool kEdgePF = GNN_EDGE_PF != 0;
 
 struct WgradArgs {
   int M, K, NC;      // rows, input features, output columns (multiple of 16)
@@ -1477,26 +1469,28 @@ int gnn_mse(const float* y, const float* t, long count, float* dY, float* part, 
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
-int gnn_node_pq(int N, const void* h, const gnn_mlp_weights* wt, float* PQ, void* stream) {
-  if (N <= 0 || !h || !wt || !wt->W1t || !PQ || wt->K != 3 * kLat) return -1;
-  NodePQArgs a{N, reinterpret_cast<const bf16*>(h), reinterpret_cast<const bf16*>(wt->W1t), PQ};
+int gnn_node_p(int N, const void* h, const gnn_mlp_weights* wt, float* P, void* stream) {
+  if (N <= 0 || !h || !wt || !wt->W1t || !P || wt->K != 3 * kLat) return -1;
+  NodePArgs a{N, reinterpret_cast<const bf16*>(h), reinterpret_cast<const bf16*>(wt->W1t), P};
   const int blocks = (N + kTileRows - 1) / kTileRows;
-  hipLaunchKernelGGL(k_node_pq, dim3(blocks), dim3(kThreads), 0, reinterpret_cast<hipStream_t>(stream), a);
+  hipLaunchKernelGGL(k_node_p, dim3(blocks), dim3(kThreads), 0, reinterpret_cast<hipStream_t>(stream), a);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
-int gnn_edge_mlp_forward(long E, const void* e, const int* recv, const int* send, const float* PQ,
-                         const gnn_mlp_weights* wt, void* hidden, void* out, int grid_cap, void* stream) {
+int gnn_edge_mlp_forward(long E, const void* e, const int* recv, const int* send, const float* P,
+                         const void* h, const gnn_mlp_weights* wt, void* hidden, void* out, int grid_cap,
+                         void* stream) {
   if (E < 0 || E > 0x7fffffffl || !wt || wt->K != 3 * kLat || wt->nout != kLat || wt->nout_pad != kLat)
     return -1;
   if (E == 0) return 0;
-  if (!e || !recv || !send || !PQ || !out || !wt->W1t || !wt->W2t || !wt->b1 || !wt->b2) return -1;
+  if (!e || !recv || !send || !P || !h || !out || !wt->W1t || !wt->W2t || !wt->b1 || !wt->b2) return -1;
   EdgeFwdArgs a{};
   a.E = (int)E;
   a.e = reinterpret_cast<const bf16*>(e);
   a.recv = recv;
   a.send = send;
-  a.PQ = PQ;
+  a.P = P;
+  a.h = reinterpret_cast<const bf16*>(h);
   a.W1t = reinterpret_cast<const bf16*>(wt->W1t);
   a.b1 = wt->b1;
   a.W2t = reinterpret_cast<const bf16*>(wt->W2t);
@@ -1510,7 +1504,7 @@ int gnn_edge_mlp_forward(long E, const void* e, const int* recv, const int* send
   const long nblocks = (E + per_block - 1) / per_block;
   int grid = grid_cap > 0 ? grid_cap : cus;
   if (grid > nblocks) grid = (int)nblocks;
-  hipLaunchKernelGGL((k_edge_fwd<kEdgeEB, kEdgeWaves, kEdgePF>), dim3(grid), dim3(64 * kEdgeWaves), 0,
+  hipLaunchKernelGGL((k_edge_fwd<kEdgeEB, kEdgeWaves>), dim3(grid), dim3(64 * kEdgeWaves), 0,
                      reinterpret_cast<hipStream_t>(stream), a);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }

@@ -35,7 +35,7 @@ extern "C" {
 #define DEM_MAX_WALLS 8
 #define DEM_MAX_GROUPS 8
 #define DEM_MAX_BONDS 32
-#define DEM_ABI_VERSION 2
+#define DEM_ABI_VERSION 3
 
 enum dem_status {
   DEM_OK = 0,
@@ -204,6 +204,18 @@ int dem_build_neighbors(dem_ctx* ctx);
 int dem_get_neighbors(dem_ctx* ctx, int* counts, int* ids, int max_n);
 
 /* ---- force stages ---------------------------------------------------------- */
+/* No counterpart in the reference: state hand-over between this library and
+ * other device code (the surrogate's judge) without host copies.  Arrays
+ * are device memory in the reference order, double [n][3]; either may be
+ * NULL.  dem_set_kinematics_device sets positions and velocities and, with
+ * clear_history, zeroes spins, forces, torques and every pp and wall history
+ * (not for bonded contexts); new positions invalidate the neighbour list.
+ * dem_get_kinematics_device copies positions and velocities out.  Both are
+ * stream-ordered on the context's stream: synchronise other streams first,
+ * and dem_synchronize before reading the results from another stream. */
+int dem_set_kinematics_device(dem_ctx* ctx, const double* x, const double* u, int clear_history);
+int dem_get_kinematics_device(dem_ctx* ctx, double* x, double* u);
+
 /* Replaces: Force::makeForceTorqueZeroOnParticle (src/CabanaDEM_Force.hpp:39-47)
  * and resetForcesAndTorques (src/CabanaBondedDEM_Force.hpp:15-50);
  * F starts at m*gravity plus the external force, if one is set. */

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define GNN_ABI_VERSION 2
+#define GNN_ABI_VERSION 3
 #define GNN_LATENT 128
 
 /* Input rows of an MLP: the concatenation of up to three BF16 sources;
@@ -110,19 +110,19 @@ int gnn_integrate(int N, const float* a, float a_scale, const double* gravity, d
 int gnn_mse(const float* y, const float* t, long count, float* dY, float* part, void* stream);
 
 /* The edge MLP of a message-passing layer, inputs (e, h[recv], h[send]),
- * with its first layer factorised: W1 = [W1a; W1b; W1c] (K = 384, nout 128)
- * splits into e W1a + (h W1b)[recv] + (h W1c)[send].
- * gnn_node_pq: PQ (float [N][256]) = [h W1b | h W1c] for the node latents h
- * (bf16 [N][128]).
- * gnn_edge_mlp_forward: out[e] = e[e] + W2 relu(e[e] W1a + PQ[recv[e]][0:128]
- * + PQ[send[e]][128:256] + b1) + b2 (bf16 [E][128]); hidden (bf16 [E][128],
+ * with the receiver's share of its first layer precomputed per node:
+ * W1 = [W1a; W1b; W1c] (K = 384, nout 128) splits into
+ * e W1a + (h W1b)[recv] + h[send] W1c.
+ * gnn_node_p: P (float [N][128]) = h W1b for the node latents h (bf16 [N][128]).
+ * gnn_edge_mlp_forward: out[e] = e[e] + W2 relu(e[e] W1a + P[recv[e]]
+ * + h[send[e]] W1c + b1) + b2 (bf16 [E][128]); hidden (bf16 [E][128],
  * optional) as in gnn_mlp_forward.  The same function as gnn_mlp_forward
  * over the three sources, up to FP32 summation order.  grid_cap > 0 limits
  * the persistent workgroups (default one per CU). */
-int gnn_node_pq(int N, const void* h, const gnn_mlp_weights* w, float* PQ, void* stream);
-int gnn_edge_mlp_forward(long E, const void* e, const int* recv, const int* send, const float* PQ,
-                         const gnn_mlp_weights* w, void* hidden, void* out, int grid_cap,
-                         void* stream);
+int gnn_node_p(int N, const void* h, const gnn_mlp_weights* w, float* P, void* stream);
+int gnn_edge_mlp_forward(long E, const void* e, const int* recv, const int* send, const float* P,
+                         const void* h, const gnn_mlp_weights* w, void* hidden, void* out,
+                         int grid_cap, void* stream);
 
 /* Radius graph on the device: the edges j -> i for every pair i != j of the
  * N points x (double [N][3]) with |x_i - x_j|^2 <= radius^2 (the DEM
