@@ -63,7 +63,7 @@ def test_sphere_wall_matches_oracle(gpu_lib, tmp_path):
     compare_to_oracle(res, st)
 
 
-@pytest.mark.parametrize("cor", [0.2, 0.6])
+@pytest.mark.parametrize("cor", [0.01, 0.2, 0.4, 0.6, 0.8, 1.0])  # automate.py:266-290
 def test_restitution_matches_oracle(gpu_lib, tmp_path, cor):
     res = run_driver("cor", "03_normal_particle_wall_different_cor.json", tmp_path, f"--cor-pw={cor}")
     st, w, prm, run, inp = S.ex03(cor)
@@ -72,12 +72,38 @@ def test_restitution_matches_oracle(gpu_lib, tmp_path, cor):
     assert abs(res["particles"][0]["u"][1] / inp["velocity_p"] - cor) < 0.02 + 0.02 * cor
 
 
-def test_oblique_matches_oracle(gpu_lib, tmp_path):
-    res = run_driver("oblique", "04_oblique_particle_wall_different_angles.json", tmp_path, "--angle=30.")
-    st, w, prm, run, inp = S.ex04(30.)
+ANGLES = [2., 5., 10., 15., 20., 25., 30., 35., 40., 50., 55., 60., 65., 70.]  # automate.py:358-414
+
+
+@pytest.mark.parametrize("angle", ANGLES)
+def test_oblique_matches_oracle(gpu_lib, tmp_path, angle):
+    res = run_driver("oblique", "04_oblique_particle_wall_different_angles.json", tmp_path, f"--angle={angle}")
+    st, w, prm, run, inp = S.ex04(angle)
     solver_run(st, w, prm, run)
     compare_to_oracle(res, st)
     assert abs(res["particles"][0]["omega"][2] - st.omega[0, 2]) <= 1e-9 * abs(st.omega[0, 2])
+
+
+def test_oblique_spin_curve_against_reference_csvs(gpu_lib, tmp_path):
+    """The spin after an oblique wall impact against the incidence angle, from
+    the GPU drop-in driver, against the two data sets the reference plots
+    (tests/golden/validation_data: Lethe DEM and Kharaz-Gorham-Salman)."""
+    vd = ROOT / "tests" / "golden" / "validation_data"
+    angles = [10., 20., 30., 40., 50., 60.]
+    om = []
+    for a in angles:
+        d = tmp_path / f"a{int(a)}"
+        d.mkdir()
+        res = run_driver("oblique", "04_oblique_particle_wall_different_angles.json", d, f"--angle={a}")
+        om.append(res["particles"][0]["omega"][2])
+    om = np.array(om)
+    assert np.all(om > 0)  # the reference's sign (contact point at +a n)
+    for name in ("04_oblique_particle_wall_different_angles_omega_vs_incident_angle_Lethe_DEM_data.csv",
+                 "04_oblique_particle_wall_different_angles_omega_vs_incident_angle_Kharaz_Gorham_Salman_experimental_data.csv"):
+        csv = np.loadtxt(vd / name, delimiter=",")
+        ref = np.interp(angles, csv[:, 0], csv[:, 1])
+        assert np.max(np.abs(om - ref)) < 0.10 * ref.max(), name
+    assert angles[int(np.argmax(om))] == 30.
 
 
 def test_one_hits_two_matches_oracle(gpu_lib, tmp_path):
