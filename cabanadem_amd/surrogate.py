@@ -601,8 +601,9 @@ class Rollout:
     """The surrogate advancing a DEM workload by macro-steps, with the FP64
     DEM as judge and fallback: every `check_every` macro-steps the DEM
     re-simulates the same macro-step from the same state; the mean position
-    error |x_surrogate - x_DEM| / r_mean is recorded and, above `tol`, the
-    run continues from the DEM state."""
+    error |x_surrogate - x_DEM| / r_mean and the kinetic and mechanical
+    (kinetic + gravitational) energy errors are recorded and, above `tol`
+    in position, the run continues from the DEM state."""
 
     def __init__(self, net: Network, wl, scales: Scales, check_every=10, tol=0.02):
         torch = _torch()
@@ -623,6 +624,14 @@ class Rollout:
         # the DEM's positions and velocities on the device (reference order)
         self.xd = torch.empty((n, 3), dtype=torch.float64, device="cuda")
         self.ud = torch.empty((n, 3), dtype=torch.float64, device="cuda")
+        self.m_dev = torch.from_numpy(wl.state.m.copy()).cuda()
+
+    def energies(self, x, v):
+        """Translational kinetic energy and gravitational potential energy
+        (-sum m g.x, zero at the origin) of positions x and velocities v."""
+        ke = 0.5 * (self.m_dev * (v.double() ** 2).sum(1)).sum()
+        pe = -(self.m_dev * (x @ self.g_dev)).sum()
+        return float(ke), float(pe)
 
     def _dem_pull(self):
         """The DEM context's positions and velocities into self.xd / self.ud
@@ -642,8 +651,9 @@ class Rollout:
         self.vhist = torch.stack(vh).contiguous()
         # one untimed check, so that first-call costs (kernel loads) stay out
         # of the timed checks; the check context is reset at every check
-        xd, _ = self.dem_step_from(self.x, self.vhist[-1].double())
+        xd, vd = self.dem_step_from(self.x, self.vhist[-1].double())
         float(torch.linalg.vector_norm(self.x - xd, dim=1).mean())
+        self.energies(xd, vd)
         # and one untimed surrogate step (buffer allocation), undone
         x_keep, v_keep = self.x.clone(), self.vhist.clone()
         self.surrogate_step()
@@ -691,6 +701,7 @@ class Rollout:
         torch = _torch()
         self.warm_up()
         errs, fallbacks, dem_failures = [], 0, 0
+        ke_errs, e_errs = [], []
         t_sur = t_dem = 0.
         n_sur = n_dem = 0
         # the state the run last continued from after a check (host copies)
@@ -717,6 +728,12 @@ class Rollout:
                     dem_failures += 1
                 t_dem += time.perf_counter() - t0
                 n_dem += 1
+                if xd is not None:  # energies, outside the timed check
+                    ke_s, pe_s = self.energies(self.x, self.vhist[-1])
+                    ke_d, pe_d = self.energies(xd, vd)
+                    if ke_d > 0.:
+                        ke_errs.append(abs(ke_s - ke_d) / ke_d)
+                    e_errs.append(abs((ke_s + pe_s) - (ke_d + pe_d)) / max(abs(ke_d + pe_d), 1e-300))
                 errs.append(err)
                 if err > self.tol:
                     fallbacks += 1
@@ -742,5 +759,8 @@ class Rollout:
                 "dem_check_failures": dem_failures,
                 "mean_position_error_over_r": float(np.mean(fin)) if fin else None,
                 "max_position_error_over_r": float(np.max(fin)) if fin else None,
+                "mean_kinetic_energy_error_rel": float(np.mean(ke_errs)) if ke_errs else None,
+                "max_kinetic_energy_error_rel": float(np.max(ke_errs)) if ke_errs else None,
+                "mean_mechanical_energy_error_rel": float(np.mean(e_errs)) if e_errs else None,
                 "surrogate_ms_per_macro": 1e3 * t_sur / max(n_sur, 1),
                 "dem_ms_per_macro": 1e3 * t_dem / max(n_dem, 1)}

@@ -364,5 +364,7 @@ def test_judged_rollout_records_error_and_falls_back(gpu_lib):
     res = ro.run(9)
     assert res["checks"] == 3
     assert np.isfinite(res["mean_position_error_over_r"])
+    assert np.isfinite(res["mean_kinetic_energy_error_rel"]) and np.isfinite(res["mean_mechanical_energy_error_rel"])
+    assert res["max_kinetic_energy_error_rel"] >= res["mean_kinetic_energy_error_rel"] >= 0.
     assert res["fallbacks"] == 3
     assert res["surrogate_ms_per_macro"] > 0 and res["dem_ms_per_macro"] > 0
