@@ -244,10 +244,50 @@ __device__ inline int wave_max_int(int v) {
   return v;
 }
 
+__device__ inline int wave_min_int(int v) {
+  for (int off = 32; off > 0; off >>= 1) {
+    const int o = __shfl_xor(v, off);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+
+// A wave's own LDS tile: its writes before its later reads, and back.
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr int kNbrStage = 256; // slots of one range round staged in LDS per wave
+
+// Cabana's full Verlet list (VerletList with FullNeighborTag, built with the
+// cell list, src/CabanaDEM_Solver.hpp:60-63): j != i with |x_i - x_j|^2 <= rc^2,
+// rows ascending in j.  Slot p searches the 27 cells around its own as nine
+// slot ranges (three consecutive x-cells of one (y, z) row are contiguous in
+// cell order), walked in ascending order.
+//
+// LDS-staged cell tiles: in round q every lane walks its q-th range.  The
+// lanes of a wave are consecutive slots, mostly of one cell row, so their
+// q-th ranges lie in one short span of slots; the wave loads that span into
+// LDS once (coalesced) when it holds at most kNbrStage slots, and the lanes
+// test against LDS instead of each gathering its own candidates.  Wider
+// spans (a wave across a y-tile or system boundary) read from global memory.
+// Either way the tested values and their order are the same.
 __global__ __launch_bounds__(kBlock) void k_neighbors(BuildArgs a) {
+  __shared__ double4 sx[kBlock / 64][kNbrStage];
   const int p = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const bool active = p < a.n && !a.flags->cell_overflow;
   int cnt = 0;
-  if (p < a.n && !a.flags->cell_overflow) {
+  int rs[9], re[9];
+  double4 xi = make_double4(0., 0., 0., 0.);
+#pragma unroll
+  for (int q = 0; q < 9; ++q) {
+    rs[q] = INT_MAX;
+    re[q] = INT_MAX;
+  }
+  if (active) {
     const int sys = a.sysid ? a.sysid[p] : 0;
     const Grid& g = a.sysid ? a.grids[sys] : a.grid;
     const long base = a.sysid ? a.kbase[sys] : 0;
@@ -259,18 +299,14 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(BuildArgs a) {
     key /= kTileRows;
     const int cz = key % nz;
     const int cy = (key / nz) * kTileRows + ly;
-    const double4 xi = a.xr_out[p];
+    xi = a.xr_out[p];
     const int xlo = cx > 0 ? cx - 1 : 0;
     const int xhi = cx < nx - 1 ? cx + 1 : nx - 1;
     // the nine slot ranges, ascending by start (they never overlap)
-    int rs[9], re[9];
 #pragma unroll
     for (int q = 0; q < 9; ++q) {
       const int z = cz + q / 3 - 1, y = cy + q % 3 - 1;
-      if (z < 0 || z >= nz || y < 0 || y >= ny) {
-        rs[q] = INT_MAX;
-        re[q] = INT_MAX;
-      } else {
+      if (z >= 0 && z < nz && y >= 0 && y < ny) {
         const long k0 = base + cell_key(g, xlo, y, z);
         rs[q] = a.cell_start[k0];
         re[q] = a.cell_start[k0 + (xhi - xlo) + 1];
@@ -284,12 +320,23 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(BuildArgs a) {
           int t = rs[r]; rs[r] = rs[r - 1]; rs[r - 1] = t;
           t = re[r]; re[r] = re[r - 1]; re[r - 1] = t;
         }
+  }
 #pragma unroll
-    for (int q = 0; q < 9; ++q) {
-      if (rs[q] == INT_MAX) break;
-      for (int j = rs[q]; j < re[q]; ++j) {
+  for (int q = 0; q < 9; ++q) {
+    const int s = rs[q], e = re[q];
+    const bool has = s != INT_MAX;
+    const int lo = wave_min_int(has ? s : INT_MAX);
+    const int hi = wave_max_int(has ? e : INT_MIN);
+    if (lo == INT_MAX) break; // no lane has a q-th range, nor a later one
+    const bool staged = hi - lo <= kNbrStage; // the same for the whole wave
+    if (staged) {
+      for (int t = lane; t < hi - lo; t += 64) sx[w][t] = a.xr_out[lo + t];
+      wave_lds_sync();
+    }
+    if (has) {
+      for (int j = s; j < e; ++j) {
         if (j == p) continue;
-        const double4 xj = a.xr_out[j];
+        const double4 xj = staged ? sx[w][j - lo] : a.xr_out[j];
         const double d0 = xi.x - xj.x;
         const double d1 = xi.y - xj.y;
         const double d2 = xi.z - xj.z;
@@ -300,11 +347,12 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(BuildArgs a) {
         }
       }
     }
-    a.nn[p] = cnt < a.max_n ? cnt : a.max_n;
+    if (staged) wave_lds_sync(); // the next round rewrites the tile
   }
+  if (active) a.nn[p] = cnt < a.max_n ? cnt : a.max_n;
   // one global atomic per wave, and only when it can raise the maximum
   const int wmax = wave_max_int(cnt);
-  if ((threadIdx.x & 63) == 0 && wmax > 0) {
+  if (lane == 0 && wmax > 0) {
     if (wmax > *(volatile int*)&a.flags->max_nn) atomicMax(&a.flags->max_nn, wmax);
     if (wmax > a.max_n) atomicMax(&a.flags->nbr_overflow, wmax);
   }
