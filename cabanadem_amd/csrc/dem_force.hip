@@ -39,17 +39,6 @@ __device__ inline void load6(const double* p, double* v) {
   double2 a = q[0], b = q[1], c = q[2];
   v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y; v[4] = c.x; v[5] = c.y;
 }
-// Streaming stores of the step's own outputs (non-temporal hint): they are
-// next read one step later, long after L2 has turned over at these sizes,
-// so they need not displace the neighbour positions being gathered.
-typedef double d2v __attribute__((ext_vector_type(2)));
-__device__ inline void store6_nt(double* p, const double* v) {
-  d2v* q = reinterpret_cast<d2v*>(p);
-  __builtin_nontemporal_store(d2v{v[0], v[1]}, q);
-  __builtin_nontemporal_store(d2v{v[2], v[3]}, q + 1);
-  __builtin_nontemporal_store(d2v{v[4], v[5]}, q + 2);
-}
-
 __device__ inline void store6(double* p, const double* v) {
   double2* q = reinterpret_cast<double2*>(p);
   q[0] = make_double2(v[0], v[1]);
@@ -534,7 +523,7 @@ __device__ __forceinline__ void force_tile(const StepArgs& a, const int tile, in
         // no overlap: the history is zero (:666-670); the cleared bit says so
       }
     }
-    if (PP) __builtin_nontemporal_store(make_meta(nc, wn), a.cmeta_out + i);
+    if (PP) a.cmeta_out[i] = make_meta(nc, wn);
     else if (PW) a.cmeta_in[i] = make_meta(nc, wn); // walls only: in place
     if (!ACCUM) apply_force_bc(a, i, F);
     if (KICK == 0) {
@@ -572,10 +561,8 @@ __device__ __forceinline__ void force_tile(const StepArgs& a, const int tile, in
         xo.y = xi.y + a.dt * s[1];
         xo.z = xi.z + a.dt * s[2];
         xo.w = xi.w;
-        d2v* xq = reinterpret_cast<d2v*>(a.xr_out + i);
-        __builtin_nontemporal_store(d2v{xo.x, xo.y}, xq);
-        __builtin_nontemporal_store(d2v{xo.z, xo.w}, xq + 1);
-        store6_nt(a.uw_out + 6l * i, s);
+        a.xr_out[i] = xo;
+        store6(a.uw_out + 6l * i, s);
       } else {
         a.xr_out[i] = xi;
         store6(a.uw_out + 6l * i, s);
