@@ -704,6 +704,165 @@ __global__ __launch_bounds__(64 * WAVES, 1) void k_dense_fwd(DenseFwdArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Node side of a message-passing layer in one pass:
+//   agg = sum of the node's edge rows (gnn_segment_sum, same order),
+//   h'  = h + W2 relu(W1 [h, agg] + b1) + b2,
+//   P'  = h' W1b of the next layer's edge MLP (gnn_node_p),
+// with the dense kernel's scheme (weights in LDS, transposed products, rows
+// through LDS).  The next layer's W1b is read from L2 as the A operand.
+// ---------------------------------------------------------------------------
+
+struct NodeFwdArgs {
+  int N;
+  const bf16* h;      // [N][128]
+  const bf16* e;      // [E][128]: this layer's edge latents, receiver-major
+  const int* row;     // [N + 1]
+  const bf16* W1t;    // [128][256]
+  const float* b1;
+  const bf16* W2t;    // [128][128]
+  const float* b2;
+  bf16* agg;          // [N][128] or null
+  bf16* hid;          // [N][128] or null
+  bf16* out;          // [N][128]
+  const bf16* Wn1t;   // next layer's edge W1t [128][384], or null
+  float* P;           // [N][128] when Wn1t is set
+};
+
+template <int WAVES>
+__global__ __launch_bounds__(64 * WAVES, 1) void k_node_fwd(NodeFwdArgs a) {
+  constexpr int K = 2 * kLat, KS = K / 32;
+  __shared__ bf16 sw1[kLat][K + 8];   // W1^T: [unit][k]
+  __shared__ bf16 sw2[kLat][kWPitch]; // W2^T: [out][k]
+  __shared__ bf16 sh[WAVES][16][kWPitch];
+  for (int c = threadIdx.x; c < kLat * (K / 8); c += 64 * WAVES) {
+    const int r = c / (K / 8), k8 = (c % (K / 8)) * 8;
+    *reinterpret_cast<bf16x8*>(&sw1[r][k8]) = ld8(a.W1t + (long)r * K + k8);
+  }
+  for (int c = threadIdx.x; c < kLat * 16; c += 64 * WAVES) {
+    const int r = c >> 4, k8 = (c & 15) * 8;
+    *reinterpret_cast<bf16x8*>(&sw2[r][k8]) = ld8(a.W2t + (long)r * kLat + k8);
+  }
+  __syncthreads(); // the only block barrier
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int lr = lane & 15, lg = lane >> 4, lk = 8 * lg;
+  const int ntiles = (a.N + 15) / 16;
+  for (int t = blockIdx.x * WAVES + w; t < ntiles; t += gridDim.x * WAVES) {
+    const int m0 = t * 16;
+    const int m = m0 + lr < a.N ? m0 + lr : a.N - 1; // clamped loads; stores are guarded
+    bf16x8 B[KS];
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) B[ks] = ld8(a.h + (long)m * kLat + ks * 32 + lk);
+    // aggregation over the node's edges in edge order, FP32 sums (two edges'
+    // rows in flight at a time)
+    float s[4][8];
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks)
+#pragma unroll
+      for (int q = 0; q < 8; ++q) s[ks][q] = 0.f;
+    const int eb = a.row[m], ee = a.row[m + 1];
+    int e = eb;
+    for (; e + 1 < ee; e += 2) {
+      bf16x8 v0[4], v1[4];
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) {
+        v0[ks] = ld8(a.e + (long)e * kLat + ks * 32 + lk);
+        v1[ks] = ld8(a.e + (long)(e + 1) * kLat + ks * 32 + lk);
+      }
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          s[ks][q] += (float)v0[ks][q];
+          s[ks][q] += (float)v1[ks][q];
+        }
+    }
+    if (e < ee) {
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) {
+        const bf16x8 v = ld8(a.e + (long)e * kLat + ks * 32 + lk);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) s[ks][q] += (float)v[q];
+      }
+    }
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) B[4 + ks][q] = (bf16)s[ks][q];
+      if (a.agg && m0 + lr < a.N)
+        *reinterpret_cast<bf16x8*>(a.agg + (long)m * kLat + ks * 32 + lk) = B[4 + ks];
+    }
+    // layer 1 on top of b1
+    f32x4 acc[8];
+#pragma unroll
+    for (int ub = 0; ub < 8; ++ub) {
+      const float4 bb = *reinterpret_cast<const float4*>(a.b1 + ub * 16 + 4 * lg);
+      acc[ub] = f32x4{bb.x, bb.y, bb.z, bb.w};
+    }
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+      for (int ub = 0; ub < 8; ++ub)
+        acc[ub] = mfma(*reinterpret_cast<const bf16x8*>(&sw1[ub * 16 + lr][ks * 32 + lk]), B[ks], acc[ub]);
+#pragma unroll
+    for (int ub = 0; ub < 8; ++ub) {
+      bf16x4 hv;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) hv[r] = (bf16)(acc[ub][r] > 0.f ? acc[ub][r] : 0.f);
+      *reinterpret_cast<bf16x4*>(&sh[w][lr][ub * 16 + 4 * lg]) = hv;
+    }
+    wave_lds_sync();
+    if (a.hid) wave_store_tile<16>(sh[w], a.hid, m0, a.N, lane);
+    // layer 2 on top of b2 + h (the residual)
+    f32x4 acc2[8];
+#pragma unroll
+    for (int ub = 0; ub < 8; ++ub) {
+      const int u = ub * 16 + 4 * lg;
+      const float4 bb = *reinterpret_cast<const float4*>(a.b2 + u);
+      const bf16x4 rv = *reinterpret_cast<const bf16x4*>(a.h + (long)m * kLat + u);
+      acc2[ub] = f32x4{bb.x, bb.y, bb.z, bb.w};
+      acc2[ub] += f32x4{(float)rv[0], (float)rv[1], (float)rv[2], (float)rv[3]};
+    }
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      const bf16x8 hb = *reinterpret_cast<const bf16x8*>(&sh[w][lr][ks * 32 + lk]);
+#pragma unroll
+      for (int ub = 0; ub < 8; ++ub)
+        acc2[ub] = mfma(*reinterpret_cast<const bf16x8*>(&sw2[ub * 16 + lr][ks * 32 + lk]), hb, acc2[ub]);
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int ub = 0; ub < 8; ++ub) {
+      bf16x4 ov;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) ov[r] = (bf16)acc2[ub][r];
+      *reinterpret_cast<bf16x4*>(&sh[w][lr][ub * 16 + 4 * lg]) = ov;
+    }
+    wave_lds_sync();
+    wave_store_tile<16>(sh[w], a.out, m0, a.N, lane);
+    if (a.Wn1t) {
+      // P' = h' W1b (next layer): A = W1b^T fragments from L2, B = h'^T from the tile
+      f32x4 accp[8];
+#pragma unroll
+      for (int ub = 0; ub < 8; ++ub) accp[ub] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) {
+        const bf16x8 hb = *reinterpret_cast<const bf16x8*>(&sh[w][lr][ks * 32 + lk]);
+#pragma unroll
+        for (int ub = 0; ub < 8; ++ub)
+          accp[ub] = mfma(ld8(a.Wn1t + (long)(ub * 16 + lr) * (3 * kLat) + kLat + ks * 32 + lk), hb, accp[ub]);
+      }
+      if (m0 + lr < a.N) {
+#pragma unroll
+        for (int ub = 0; ub < 8; ++ub)
+          *reinterpret_cast<float4*>(a.P + (long)m * kLat + ub * 16 + 4 * lg) =
+              make_float4(accp[ub][0], accp[ub][1], accp[ub][2], accp[ub][3]);
+      }
+    }
+    wave_lds_sync(); // the next tile rewrites the tile
+  }
+}
+
 // Launch shape of the edge kernel: 16 EB edges per wave tile, WAVES waves per
 // block (one block per CU, LDS-limited: three 34 KB weight tiles plus
 // WAVES x 16 EB hidden rows).
@@ -1506,6 +1665,37 @@ int gnn_edge_mlp_forward(long E, const void* e, const int* recv, const int* send
   if (grid > nblocks) grid = (int)nblocks;
   hipLaunchKernelGGL((k_edge_fwd<kEdgeEB, kEdgeWaves>), dim3(grid), dim3(64 * kEdgeWaves), 0,
                      reinterpret_cast<hipStream_t>(stream), a);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int gnn_node_layer_forward(int N, const void* h, const void* e, const int* row, const gnn_mlp_weights* wt,
+                           void* agg, void* hidden, void* out, const gnn_mlp_weights* next_edge, float* P,
+                           void* stream) {
+  if (N <= 0 || !h || !e || !row || !wt || !out || wt->K != 2 * kLat || wt->nout != kLat ||
+      wt->nout_pad != kLat || !wt->W1t || !wt->W2t || !wt->b1 || !wt->b2)
+    return -1;
+  if (next_edge && (next_edge->K != 3 * kLat || !next_edge->W1t || !P)) return -1;
+  NodeFwdArgs a{};
+  a.N = N;
+  a.h = reinterpret_cast<const bf16*>(h);
+  a.e = reinterpret_cast<const bf16*>(e);
+  a.row = row;
+  a.W1t = reinterpret_cast<const bf16*>(wt->W1t);
+  a.b1 = wt->b1;
+  a.W2t = reinterpret_cast<const bf16*>(wt->W2t);
+  a.b2 = wt->b2;
+  a.agg = reinterpret_cast<bf16*>(agg);
+  a.hid = reinterpret_cast<bf16*>(hidden);
+  a.out = reinterpret_cast<bf16*>(out);
+  a.Wn1t = next_edge ? reinterpret_cast<const bf16*>(next_edge->W1t) : nullptr;
+  a.P = next_edge ? P : nullptr;
+  const int cus = device_cus();
+  if (cus <= 0) return -2;
+  constexpr int kWaves = 8; // two per SIMD: the aggregation's FP32 sums need the registers
+  const long blocks = ((long)N + 16l * kWaves - 1) / (16l * kWaves);
+  const int grid = blocks < cus ? (int)blocks : cus;
+  hipLaunchKernelGGL(k_node_fwd<kWaves>, dim3(grid), dim3(64 * kWaves), 0, reinterpret_cast<hipStream_t>(stream),
+                     a);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

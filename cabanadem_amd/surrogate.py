@@ -77,6 +77,8 @@ _SIG = {
     "gnn_node_p": (C.c_int, [C.c_int, _vp, C.POINTER(GnnMlpWeights), _vp, _vp]),
     "gnn_edge_mlp_forward": (C.c_int, [C.c_long, _vp, _vp, _vp, _vp, _vp, C.POINTER(GnnMlpWeights), _vp,
                                        _vp, C.c_int, _vp]),
+    "gnn_node_layer_forward": (C.c_int, [C.c_int, _vp, _vp, _vp, C.POINTER(GnnMlpWeights), _vp, _vp, _vp,
+                                         C.POINTER(GnnMlpWeights), _vp, _vp]),
     "gnn_radius_graph_workspace": (C.c_long, [C.c_int, _dp, _dp, C.c_double]),
     "gnn_radius_graph_count": (C.c_int, [C.c_int, _vp, C.c_double, _dp, _dp, _vp, C.c_long, _vp,
                                          C.POINTER(C.c_long), _vp]),
@@ -371,19 +373,31 @@ class Network:
         fwd(self.enc_n, io(N, (xn, None)), None, self.hid_enc_n, self.h[0], None)
         if E > 0:
             fwd(self.enc_e, io(E, (xe, None)), None, self.hid_enc_e, self.e[0], None)
-        for l in range(self.L):
-            h, e = self.h[l], self.e[l]
-            if E > 0 and self.factorised:
-                w = self.edge[l].weights(F)
-                _ok(Lb.gnn_node_p(N, ptr(h), C.byref(w), ptr(self.P), s), "node P")
+        if E > 0 and self.factorised:
+            # per layer: the edge MLP, then the node side in one pass (the
+            # aggregation, the node MLP and the next layer's P = h W1b)
+            _ok(Lb.gnn_node_p(N, ptr(self.h[0]), C.byref(self.edge[0].weights(F)), ptr(self.P), s), "node P")
+            for l in range(self.L):
+                h, e = self.h[l], self.e[l]
                 _ok(Lb.gnn_edge_mlp_forward(E, ptr(e), ptr(g.recv), ptr(g.send), ptr(self.P), ptr(h),
-                                            C.byref(w), ptr(self.hid_e[l]) if train else None,
-                                            ptr(self.e[l + 1]), 0, s), "edge MLP forward")
-            elif E > 0:
-                fwd(self.edge[l], io(E, (e[:E], None), (h, g.recv), (h, g.send)), e, self.hid_e[l],
-                    self.e[l + 1], None)
-            _ok(Lb.gnn_segment_sum(ptr(self.e[l + 1]), ptr(g.row), N, ptr(self.agg[l]), s), "segment sum")
-            fwd(self.node[l], io(N, (h[:N], None), (self.agg[l], None)), h, self.hid_n[l], self.h[l + 1], None)
+                                            C.byref(self.edge[l].weights(F)),
+                                            ptr(self.hid_e[l]) if train else None, ptr(self.e[l + 1]), 0, s),
+                    "edge MLP forward")
+                nxt = C.byref(self.edge[l + 1].weights(F)) if l + 1 < self.L else None
+                _ok(Lb.gnn_node_layer_forward(N, ptr(h), ptr(self.e[l + 1]), ptr(g.row),
+                                              C.byref(self.node[l].weights(F)),
+                                              ptr(self.agg[l]) if train else None,
+                                              ptr(self.hid_n[l]) if train else None, ptr(self.h[l + 1]), nxt,
+                                              ptr(self.P), s), "node layer forward")
+        else:
+            for l in range(self.L):
+                h, e = self.h[l], self.e[l]
+                if E > 0:
+                    fwd(self.edge[l], io(E, (e[:E], None), (h, g.recv), (h, g.send)), e, self.hid_e[l],
+                        self.e[l + 1], None)
+                _ok(Lb.gnn_segment_sum(ptr(self.e[l + 1]), ptr(g.row), N, ptr(self.agg[l]), s), "segment sum")
+                fwd(self.node[l], io(N, (h[:N], None), (self.agg[l], None)), h, self.hid_n[l], self.h[l + 1],
+                    None)
         fwd(self.dec, io(N, (self.h[self.L][:N], None)), None, self.hid_dec, None, self.out)
         return self.out[:N]
 

@@ -124,6 +124,17 @@ int gnn_edge_mlp_forward(long E, const void* e, const int* recv, const int* send
                          const void* h, const gnn_mlp_weights* w, void* hidden, void* out,
                          int grid_cap, void* stream);
 
+/* The node side of a message-passing layer in one pass:
+ *   agg (bf16 [N][128], optional output) = the sum of e (bf16 [E][128]) over
+ *     each node's edges [row[i], row[i+1]), as gnn_segment_sum;
+ *   out = h + MLP([h, agg]) with w (K = 256, nout 128); hidden (optional) as
+ *     in gnn_mlp_forward;
+ *   when next_edge (the next layer's edge MLP, K = 384) is given,
+ *     P (float [N][128]) = out W1b of it, as gnn_node_p. */
+int gnn_node_layer_forward(int N, const void* h, const void* e, const int* row,
+                           const gnn_mlp_weights* w, void* agg, void* hidden, void* out,
+                           const gnn_mlp_weights* next_edge, float* P, void* stream);
+
 /* Radius graph on the device: the edges j -> i for every pair i != j of the
  * N points x (double [N][3]) with |x_i - x_j|^2 <= radius^2 (the DEM
  * neighbour criterion), in the receiver-major layout above with each node's

@@ -120,6 +120,44 @@ def test_factorised_edge_mlp_matches_gathered_form(gpu_lib, M):
     assert torch.equal(out2, out)
 
 
+@pytest.mark.parametrize("N,last", [(1, False), (77, True), (5000, False), (64001, False)])
+def test_node_layer_pass_matches_separate_kernels(gpu_lib, N, last):
+    """gnn_node_layer_forward (aggregation, node MLP and the next layer's P in
+    one pass) against gnn_segment_sum + gnn_mlp_forward + gnn_node_p, with
+    nodes of no edge among them."""
+    rng = np.random.default_rng(N)
+    deg = rng.integers(0, 13, size=N)
+    deg[:: 7] = 0
+    row = np.concatenate([[0], np.cumsum(deg)]).astype(np.int32)
+    E = int(row[-1])
+    h = bf(rng.normal(size=(N, 128))).cuda()
+    e = bf(rng.normal(size=(max(E, 1), 128))).cuda()
+    row_d = torch.from_numpy(row).cuda()
+    mn, fln, _ = make_mlp(256, 128, seed=N + 5)
+    me, fle, flat_e = make_mlp(384, 128, seed=N + 6)
+    Lb, s = S.glib(), S.stream()
+    wn, we = mn.weights(fln), me.weights(fle)
+    z = lambda dt=torch.bfloat16: torch.empty((N, 128), dtype=dt, device="cuda")
+    agg_r, hid_r, out_r = z(), z(), z()
+    assert Lb.gnn_segment_sum(S.ptr(e), S.ptr(row_d), N, S.ptr(agg_r), s) == 0
+    assert Lb.gnn_mlp_forward(S.io(N, (h, None), (agg_r, None)), wn, S.ptr(h), S.ptr(hid_r), S.ptr(out_r), None, 0,
+                              s) == 0
+    agg_f, hid_f, out_f = z(), z(), z()
+    P_f = torch.full((N, 128), 7.0, device="cuda")
+    assert Lb.gnn_node_layer_forward(N, S.ptr(h), S.ptr(e), S.ptr(row_d), wn, S.ptr(agg_f), S.ptr(hid_f),
+                                     S.ptr(out_f), None if last else we, S.ptr(P_f), s) == 0
+    P_r = z(torch.float32)
+    assert Lb.gnn_node_p(N, S.ptr(out_f), we, S.ptr(P_r), s) == 0
+    torch.cuda.synchronize()
+    assert torch.equal(agg_f, agg_r)            # same sums in the same order
+    assert rel(out_f.float().cpu(), out_r.float().cpu()) < 5e-3
+    assert rel(hid_f.float().cpu(), hid_r.float().cpu()) < 5e-3
+    if last:
+        assert torch.all(P_f == 7.0)            # no next layer: P untouched
+    else:
+        assert rel(P_f.cpu(), P_r.cpu()) < 1e-5
+
+
 @pytest.mark.parametrize("K,M", [(32, 1), (32, 70001), (128, 77), (256, 5000), (256, 64001)])
 def test_dense_mlp_kernel_matches_row_tiled_kernel(gpu_lib, K, M):
     """Ungathered inputs with a 128-wide BF16 output take the persistent
