@@ -547,17 +547,30 @@ __global__ __launch_bounds__(64 * WAVES, 1) void k_edge_fwd(EdgeFwdArgs a) {
       }
     wave_lds_sync();
     if (a.hid) wave_store_tile<TE>(sh[w], a.hid, e0, a.E, lane);
-    // layer 2: out^T = W2^T hidden^T on top of b2 + e (the residual)
+    // layer 2: out^T = W2^T hidden^T + I e^T on top of b2.  The residual e
+    // goes through the MFMA units from the e^T fragments already in
+    // registers: block ub of units takes an identity block as A and the
+    // fragment k step ub / 2 as B (exact: BF16 values times 1 in FP32).
     f32x4 acc2[8][EB];
 #pragma unroll
     for (int ub = 0; ub < 8; ++ub) {
-      const int u = ub * 16 + 4 * lg;
-      const float4 bb = *reinterpret_cast<const float4*>(a.b2 + u);
+      const float4 bb = *reinterpret_cast<const float4*>(a.b2 + ub * 16 + 4 * lg);
 #pragma unroll
-      for (int eb = 0; eb < EB; ++eb) {
-        const bf16x4 rv = *reinterpret_cast<const bf16x4*>(a.e + (long)cur.ed[eb] * kLat + u);
-        acc2[ub][eb] = f32x4{bb.x + (float)rv[0], bb.y + (float)rv[1], bb.z + (float)rv[2], bb.w + (float)rv[3]};
-      }
+      for (int eb = 0; eb < EB; ++eb) acc2[ub][eb] = f32x4{bb.x, bb.y, bb.z, bb.w};
+    }
+    {
+      // A[m][k] = 1 for unit m = 16 ub + lr and channel k = 32 (ub / 2) + 8 lg + j,
+      // i.e. lane (lr, lg) holds the 1 at j = lr % 8 when lg = 2 (ub % 2) + lr / 8
+      bf16x8 I2[2];
+#pragma unroll
+      for (int par = 0; par < 2; ++par)
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          I2[par][j] = (bf16)((lg == 2 * par + (lr >> 3) && j == (lr & 7)) ? 1.f : 0.f);
+#pragma unroll
+      for (int ub = 0; ub < 8; ++ub)
+#pragma unroll
+        for (int eb = 0; eb < EB; ++eb) acc2[ub][eb] = mfma(I2[ub & 1], cur.Be[eb][ub >> 1], acc2[ub][eb]);
     }
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) {
