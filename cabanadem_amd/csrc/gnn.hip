@@ -826,15 +826,23 @@ __global__ __launch_bounds__(64 * WAVES, 1) void k_node_fwd(NodeFwdArgs a) {
     }
     wave_lds_sync();
     if (a.hid) wave_store_tile<16>(sh[w], a.hid, m0, a.N, lane);
-    // layer 2 on top of b2 + h (the residual)
+    // layer 2 on top of b2, with the residual h through the MFMA units (an
+    // identity block on the h^T fragments in registers, as in the edge kernel)
     f32x4 acc2[8];
 #pragma unroll
     for (int ub = 0; ub < 8; ++ub) {
-      const int u = ub * 16 + 4 * lg;
-      const float4 bb = *reinterpret_cast<const float4*>(a.b2 + u);
-      const bf16x4 rv = *reinterpret_cast<const bf16x4*>(a.h + (long)m * kLat + u);
+      const float4 bb = *reinterpret_cast<const float4*>(a.b2 + ub * 16 + 4 * lg);
       acc2[ub] = f32x4{bb.x, bb.y, bb.z, bb.w};
-      acc2[ub] += f32x4{(float)rv[0], (float)rv[1], (float)rv[2], (float)rv[3]};
+    }
+    {
+      bf16x8 I2[2];
+#pragma unroll
+      for (int par = 0; par < 2; ++par)
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          I2[par][j] = (bf16)((lg == 2 * par + (lr >> 3) && j == (lr & 7)) ? 1.f : 0.f);
+#pragma unroll
+      for (int ub = 0; ub < 8; ++ub) acc2[ub] = mfma(I2[ub & 1], B[ub >> 1], acc2[ub]);
     }
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) {
