@@ -468,25 +468,52 @@ __device__ inline void edge_tile_load(const EdgeFwdArgs& a, int e0, int lr, int 
   }
 }
 
+// Row order of the weight tiles in LDS.  MFMA block ub of 16 A rows yields,
+// in lane (lr, lg), the D rows 4 lg .. 4 lg + 3 for column (edge) lr.  Placing
+// unit o(p) = 32 s + 8 g + 4 t + r at A row p = 16 (2 s + t) + 4 g + r puts
+// units 32 s + 8 lg .. + 7 of edge lr in lane (lr, lg), blocks 2 s and 2 s + 1:
+// exactly that lane's B fragment for k step s of the next product and its
+// e^T fragment for channels 32 s .. 32 s + 31.  So the hidden layer feeds
+// layer 2 from registers, the residual e is a register add, and each lane
+// stores 16 contiguous bytes of a row.
+__device__ inline int unit_of_row(int p) {
+  const int blk = p >> 4, g = (p >> 2) & 3, r = p & 3;
+  return 32 * (blk >> 1) + 8 * g + 4 * (blk & 1) + r;
+}
+
+// Two D fragments (4 FP32 each) as one B fragment of 8 BF16, optionally
+// through the ReLU.
+template <bool RELU>
+__device__ inline bf16x8 pack8(f32x4 lo, f32x4 hi) {
+  if (RELU) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      lo[r] = lo[r] > 0.f ? lo[r] : 0.f;
+      hi[r] = hi[r] > 0.f ? hi[r] : 0.f;
+    }
+  }
+  const bf16x4 l = __builtin_convertvector(lo, bf16x4), h = __builtin_convertvector(hi, bf16x4);
+  return __builtin_shufflevector(l, h, 0, 1, 2, 3, 4, 5, 6, 7);
+}
+
 // Persistent and wave-granular: every wave takes tiles of 16 EB edges with
 // stride (waves in the grid) and loads the next tile's rows while it
 // multiplies the current one.  W1a^T, W1c^T and W2^T sit in LDS for the
-// block's waves, each weight fragment read once per tile and applied to its
-// EB edge blocks; each wave has its own hidden tile.  With a grid that is a
-// multiple of 8, XCD (blockIdx % 8) takes a contiguous eighth of the tiles,
-// so the rows its edges gather stay in that XCD's L2.
+// block's waves (rows in unit_of_row order), each weight fragment read once
+// per tile and applied to its EB edge blocks; nothing else goes through LDS.
+// With a grid that is a multiple of 8, XCD (blockIdx % 8) takes a contiguous
+// eighth of the tiles, so the rows its edges gather stay in that XCD's L2.
 template <int EB, int WAVES>
 __global__ __launch_bounds__(64 * WAVES, 1) void k_edge_fwd(EdgeFwdArgs a) {
   constexpr int TE = 16 * EB;
-  __shared__ bf16 swa[kLat][kWPitch];     // W1a^T: [unit][k]
-  __shared__ bf16 swc[kLat][kWPitch];     // W1c^T: [unit][k]
-  __shared__ bf16 sw2[kLat][kWPitch];     // W2^T:  [out][k]
-  __shared__ bf16 sh[WAVES][TE][kWPitch]; // hidden / output, [edge][unit]
+  __shared__ bf16 swa[kLat][kWPitch]; // W1a^T: [row][k]
+  __shared__ bf16 swc[kLat][kWPitch]; // W1c^T: [row][k]
+  __shared__ bf16 sw2[kLat][kWPitch]; // W2^T:  [row][k]
   for (int c = threadIdx.x; c < kLat * 16; c += 64 * WAVES) {
-    const int r = c >> 4, k8 = (c & 15) * 8;
-    *reinterpret_cast<bf16x8*>(&swa[r][k8]) = ld8(a.W1t + (long)r * (3 * kLat) + k8);
-    *reinterpret_cast<bf16x8*>(&swc[r][k8]) = ld8(a.W1t + (long)r * (3 * kLat) + 2 * kLat + k8);
-    *reinterpret_cast<bf16x8*>(&sw2[r][k8]) = ld8(a.W2t + (long)r * kLat + k8);
+    const int r = c >> 4, k8 = (c & 15) * 8, u = unit_of_row(r);
+    *reinterpret_cast<bf16x8*>(&swa[r][k8]) = ld8(a.W1t + (long)u * (3 * kLat) + k8);
+    *reinterpret_cast<bf16x8*>(&swc[r][k8]) = ld8(a.W1t + (long)u * (3 * kLat) + 2 * kLat + k8);
+    *reinterpret_cast<bf16x8*>(&sw2[r][k8]) = ld8(a.W2t + (long)u * kLat + k8);
   }
   __syncthreads(); // the only block barrier: the loop below is per wave
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -511,11 +538,14 @@ __global__ __launch_bounds__(64 * WAVES, 1) void k_edge_fwd(EdgeFwdArgs a) {
     const int e0 = t * TE;
     const int tn = t + stride;
     if (tn < t_end) edge_tile_load<EB>(a, tn * TE, lr, lk, nxt);
-    // layer 1 on top of b1 + P[recv] (units 16 ub + 4 lg .. +3 of edge 16 eb + lr)
+    // keeps the weight fragments' LDS reads inside the loop: hoisted out of
+    // it as loop-invariant they would need 384 registers
+    __asm__ volatile("" ::: "memory");
+    // layer 1 on top of b1 + P[recv]: block 2 s + t holds units 32 s + 8 lg + 4 t .. + 3
     f32x4 acc[8][EB];
 #pragma unroll
     for (int ub = 0; ub < 8; ++ub) {
-      const int u = ub * 16 + 4 * lg;
+      const int u = 32 * (ub >> 1) + lk + 4 * (ub & 1);
       const float4 bb = *reinterpret_cast<const float4*>(a.b1 + u);
 #pragma unroll
       for (int eb = 0; eb < EB; ++eb) {
@@ -535,72 +565,66 @@ __global__ __launch_bounds__(64 * WAVES, 1) void k_edge_fwd(EdgeFwdArgs a) {
           acc[ub][eb] = mfma(Ac, cur.Bh[eb][ks], acc[ub][eb]);
         }
       }
-    // relu -> BF16 hidden tile [edge][unit] (and global for the backward)
+    // relu -> BF16: the hidden^T fragment of k step s is blocks 2 s, 2 s + 1
+    bf16x8 hb[EB][4];
 #pragma unroll
-    for (int ub = 0; ub < 8; ++ub)
+    for (int s4 = 0; s4 < 4; ++s4)
+#pragma unroll
+      for (int eb = 0; eb < EB; ++eb) hb[eb][s4] = pack8<true>(acc[2 * s4][eb], acc[2 * s4 + 1][eb]);
+    if (a.hid) {
 #pragma unroll
       for (int eb = 0; eb < EB; ++eb) {
-        bf16x4 hv;
+        const int ee = e0 + eb * 16 + lr;
+        if (ee < a.E)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) hv[r] = (bf16)(acc[ub][eb][r] > 0.f ? acc[ub][eb][r] : 0.f);
-        *reinterpret_cast<bf16x4*>(&sh[w][eb * 16 + lr][ub * 16 + 4 * lg]) = hv;
+          for (int s4 = 0; s4 < 4; ++s4)
+            *reinterpret_cast<bf16x8*>(a.hid + (long)ee * kLat + 32 * s4 + lk) = hb[eb][s4];
       }
-    wave_lds_sync();
-    if (a.hid) wave_store_tile<TE>(sh[w], a.hid, e0, a.E, lane);
-    // layer 2: out^T = W2^T hidden^T + I e^T on top of b2.  The residual e
-    // goes through the MFMA units from the e^T fragments already in
-    // registers: block ub of units takes an identity block as A and the
-    // fragment k step ub / 2 as B (exact: BF16 values times 1 in FP32).
+    }
+    // layer 2 on top of b2 + e (the residual, from the e^T fragments)
     f32x4 acc2[8][EB];
 #pragma unroll
     for (int ub = 0; ub < 8; ++ub) {
-      const float4 bb = *reinterpret_cast<const float4*>(a.b2 + ub * 16 + 4 * lg);
+      const float4 bb = *reinterpret_cast<const float4*>(a.b2 + 32 * (ub >> 1) + lk + 4 * (ub & 1));
 #pragma unroll
-      for (int eb = 0; eb < EB; ++eb) acc2[ub][eb] = f32x4{bb.x, bb.y, bb.z, bb.w};
-    }
-    {
-      // A[m][k] = 1 for unit m = 16 ub + lr and channel k = 32 (ub / 2) + 8 lg + j,
-      // i.e. lane (lr, lg) holds the 1 at j = lr % 8 when lg = 2 (ub % 2) + lr / 8
-      bf16x8 I2[2];
-#pragma unroll
-      for (int par = 0; par < 2; ++par)
-#pragma unroll
-        for (int j = 0; j < 8; ++j)
-          I2[par][j] = (bf16)((lg == 2 * par + (lr >> 3) && j == (lr & 7)) ? 1.f : 0.f);
-#pragma unroll
-      for (int ub = 0; ub < 8; ++ub)
-#pragma unroll
-        for (int eb = 0; eb < EB; ++eb) acc2[ub][eb] = mfma(I2[ub & 1], cur.Be[eb][ub >> 1], acc2[ub][eb]);
+      for (int eb = 0; eb < EB; ++eb) {
+        const bf16x8 ev = cur.Be[eb][ub >> 1];
+        const int j0 = 4 * (ub & 1);
+        acc2[ub][eb] = f32x4{bb.x + (float)ev[j0], bb.y + (float)ev[j0 + 1], bb.z + (float)ev[j0 + 2],
+                             bb.w + (float)ev[j0 + 3]};
+      }
     }
 #pragma unroll
-    for (int ks = 0; ks < 4; ++ks) {
-      bf16x8 hb[EB];
-#pragma unroll
-      for (int eb = 0; eb < EB; ++eb) hb[eb] = *reinterpret_cast<const bf16x8*>(&sh[w][eb * 16 + lr][ks * 32 + lk]);
+    for (int ks = 0; ks < 4; ++ks)
 #pragma unroll
       for (int ub = 0; ub < 8; ++ub) {
         const bf16x8 A2 = *reinterpret_cast<const bf16x8*>(&sw2[ub * 16 + lr][ks * 32 + lk]);
 #pragma unroll
-        for (int eb = 0; eb < EB; ++eb) acc2[ub][eb] = mfma(A2, hb[eb], acc2[ub][eb]);
+        for (int eb = 0; eb < EB; ++eb) acc2[ub][eb] = mfma(A2, hb[eb][ks], acc2[ub][eb]);
       }
+    // 16 contiguous bytes per lane and k step: units 32 s + 8 lg .. + 7 of edge lr
+#pragma unroll
+    for (int eb = 0; eb < EB; ++eb) {
+      const int ee = e0 + eb * 16 + lr;
+      if (ee < a.E)
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4)
+          *reinterpret_cast<bf16x8*>(a.out + (long)ee * kLat + 32 * s4 + lk) =
+              pack8<false>(acc2[2 * s4][eb], acc2[2 * s4 + 1][eb]);
     }
-    // the output through the hidden tile, then whole rows out
-    wave_lds_sync();
-#pragma unroll
-    for (int ub = 0; ub < 8; ++ub)
-#pragma unroll
-      for (int eb = 0; eb < EB; ++eb) {
-        bf16x4 ov;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) ov[r] = (bf16)acc2[ub][eb][r];
-        *reinterpret_cast<bf16x4*>(&sh[w][eb * 16 + lr][ub * 16 + 4 * lg]) = ov;
-      }
-    wave_lds_sync();
-    wave_store_tile<TE>(sh[w], a.out, e0, a.E, lane);
-    wave_lds_sync(); // the next tile rewrites the tile
     if (tn < t_end) cur = nxt;
   }
 }
+
+// Launch shape of the edge kernel: 16 EB edges per wave tile, WAVES waves per
+// block (one block per CU: the three 34 KB weight tiles fill the LDS).
+#ifndef GNN_EDGE_EB
+#define GNN_EDGE_EB 1
+#endif
+#ifndef GNN_EDGE_WAVES
+#define GNN_EDGE_WAVES 8
+#endif
+constexpr int kEdgeEB = GNN_EDGE_EB, kEdgeWaves = GNN_EDGE_WAVES;
 
 // ---------------------------------------------------------------------------
 // Dense MLP y = res + W2 relu(W1 [x0, x1] + b1) + b2 with 128 outputs, the
@@ -883,17 +907,6 @@ __global__ __launch_bounds__(64 * WAVES, 1) void k_node_fwd(NodeFwdArgs a) {
     wave_lds_sync(); // the next tile rewrites the tile
   }
 }
-
-// Launch shape of the edge kernel: 16 EB edges per wave tile, WAVES waves per
-// block (one block per CU, LDS-limited: three 34 KB weight tiles plus
-// WAVES x 16 EB hidden rows).
-#ifndef GNN_EDGE_EB
-#define GNN_EDGE_EB 1
-#endif
-#ifndef GNN_EDGE_WAVES
-#define GNN_EDGE_WAVES 8
-#endif
-constexpr int kEdgeEB = GNN_EDGE_EB, kEdgeWaves = GNN_EDGE_WAVES;
 
 struct WgradArgs {
   int M, K, NC;      // rows, input features, output columns (multiple of 16)
