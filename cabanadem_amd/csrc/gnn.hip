@@ -766,31 +766,47 @@ struct NodeFwdArgs {
   float* P;           // [N][128] when Wn1t is set
 };
 
-template <int WAVES>
+// Launch shape of the node pass: WAVES waves per block (one block per CU:
+// the weights fill 137 KB of LDS) and AGG edge rows in flight per lane in the
+// aggregation.  Four waves per SIMD with two rows in flight measured best at
+// 64k nodes (34 us; two waves with six rows in flight 35 us).
+#ifndef GNN_NODE_WAVES
+#define GNN_NODE_WAVES 16
+#endif
+#ifndef GNN_NODE_AGG
+#define GNN_NODE_AGG 2
+#endif
+
+template <int WAVES, int AGG>
 __global__ __launch_bounds__(64 * WAVES, 1) void k_node_fwd(NodeFwdArgs a) {
   constexpr int K = 2 * kLat, KS = K / 32;
-  __shared__ bf16 sw1[kLat][K + 8];   // W1^T: [unit][k]
-  __shared__ bf16 sw2[kLat][kWPitch]; // W2^T: [out][k]
-  __shared__ bf16 sh[WAVES][16][kWPitch];
+  __shared__ bf16 sw1[kLat][K + 8];   // W1^T: [row][k], rows in unit_of_row order
+  __shared__ bf16 sw2[kLat][kWPitch]; // W2^T: [row][k], likewise
+  __shared__ bf16 swn[kLat][kWPitch]; // next layer's W1b^T: [row][k], likewise
   for (int c = threadIdx.x; c < kLat * (K / 8); c += 64 * WAVES) {
     const int r = c / (K / 8), k8 = (c % (K / 8)) * 8;
-    *reinterpret_cast<bf16x8*>(&sw1[r][k8]) = ld8(a.W1t + (long)r * K + k8);
+    *reinterpret_cast<bf16x8*>(&sw1[r][k8]) = ld8(a.W1t + (long)unit_of_row(r) * K + k8);
   }
   for (int c = threadIdx.x; c < kLat * 16; c += 64 * WAVES) {
     const int r = c >> 4, k8 = (c & 15) * 8;
-    *reinterpret_cast<bf16x8*>(&sw2[r][k8]) = ld8(a.W2t + (long)r * kLat + k8);
+    *reinterpret_cast<bf16x8*>(&sw2[r][k8]) = ld8(a.W2t + (long)unit_of_row(r) * kLat + k8);
+    if (a.Wn1t)
+      *reinterpret_cast<bf16x8*>(&swn[r][k8]) = ld8(a.Wn1t + (long)unit_of_row(r) * (3 * kLat) + kLat + k8);
   }
   __syncthreads(); // the only block barrier
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int lr = lane & 15, lg = lane >> 4, lk = 8 * lg;
   const int ntiles = (a.N + 15) / 16;
   for (int t = blockIdx.x * WAVES + w; t < ntiles; t += gridDim.x * WAVES) {
+    // keeps the weight fragments' LDS reads inside the loop
+    __asm__ volatile("" ::: "memory");
     const int m0 = t * 16;
-    const int m = m0 + lr < a.N ? m0 + lr : a.N - 1; // clamped loads; stores are guarded
+    const bool live = m0 + lr < a.N;
+    const int m = live ? m0 + lr : a.N - 1; // clamped loads; stores are guarded
     bf16x8 B[KS];
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) B[ks] = ld8(a.h + (long)m * kLat + ks * 32 + lk);
-    // aggregation over the node's edges in edge order, FP32 sums (two edges'
+    // aggregation over the node's edges in edge order, FP32 sums (AGG edges'
     // rows in flight at a time)
     float s[4][8];
 #pragma unroll
@@ -799,22 +815,20 @@ __global__ __launch_bounds__(64 * WAVES, 1) void k_node_fwd(NodeFwdArgs a) {
       for (int q = 0; q < 8; ++q) s[ks][q] = 0.f;
     const int eb = a.row[m], ee = a.row[m + 1];
     int e = eb;
-    for (; e + 1 < ee; e += 2) {
-      bf16x8 v0[4], v1[4];
+    for (; e + AGG - 1 < ee; e += AGG) {
+      bf16x8 v[AGG][4];
 #pragma unroll
-      for (int ks = 0; ks < 4; ++ks) {
-        v0[ks] = ld8(a.e + (long)e * kLat + ks * 32 + lk);
-        v1[ks] = ld8(a.e + (long)(e + 1) * kLat + ks * 32 + lk);
-      }
+      for (int u = 0; u < AGG; ++u)
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) v[u][ks] = ld8(a.e + (long)(e + u) * kLat + ks * 32 + lk);
 #pragma unroll
       for (int ks = 0; ks < 4; ++ks)
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          s[ks][q] += (float)v0[ks][q];
-          s[ks][q] += (float)v1[ks][q];
-        }
+        for (int q = 0; q < 8; ++q)
+#pragma unroll
+          for (int u = 0; u < AGG; ++u) s[ks][q] += (float)v[u][ks][q];
     }
-    if (e < ee) {
+    for (; e < ee; ++e) {
 #pragma unroll
       for (int ks = 0; ks < 4; ++ks) {
         const bf16x8 v = ld8(a.e + (long)e * kLat + ks * 32 + lk);
@@ -826,14 +840,13 @@ __global__ __launch_bounds__(64 * WAVES, 1) void k_node_fwd(NodeFwdArgs a) {
     for (int ks = 0; ks < 4; ++ks) {
 #pragma unroll
       for (int q = 0; q < 8; ++q) B[4 + ks][q] = (bf16)s[ks][q];
-      if (a.agg && m0 + lr < a.N)
-        *reinterpret_cast<bf16x8*>(a.agg + (long)m * kLat + ks * 32 + lk) = B[4 + ks];
+      if (a.agg && live) *reinterpret_cast<bf16x8*>(a.agg + (long)m * kLat + ks * 32 + lk) = B[4 + ks];
     }
-    // layer 1 on top of b1
+    // layer 1 on top of b1: block 2 s + t holds units 32 s + 8 lg + 4 t .. + 3
     f32x4 acc[8];
 #pragma unroll
     for (int ub = 0; ub < 8; ++ub) {
-      const float4 bb = *reinterpret_cast<const float4*>(a.b1 + ub * 16 + 4 * lg);
+      const float4 bb = *reinterpret_cast<const float4*>(a.b1 + 32 * (ub >> 1) + lk + 4 * (ub & 1));
       acc[ub] = f32x4{bb.x, bb.y, bb.z, bb.w};
     }
 #pragma unroll
@@ -841,70 +854,51 @@ __global__ __launch_bounds__(64 * WAVES, 1) void k_node_fwd(NodeFwdArgs a) {
 #pragma unroll
       for (int ub = 0; ub < 8; ++ub)
         acc[ub] = mfma(*reinterpret_cast<const bf16x8*>(&sw1[ub * 16 + lr][ks * 32 + lk]), B[ks], acc[ub]);
+    bf16x8 hb[4];
 #pragma unroll
-    for (int ub = 0; ub < 8; ++ub) {
-      bf16x4 hv;
+    for (int s4 = 0; s4 < 4; ++s4) hb[s4] = pack8<true>(acc[2 * s4], acc[2 * s4 + 1]);
+    if (a.hid && live)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) hv[r] = (bf16)(acc[ub][r] > 0.f ? acc[ub][r] : 0.f);
-      *reinterpret_cast<bf16x4*>(&sh[w][lr][ub * 16 + 4 * lg]) = hv;
-    }
-    wave_lds_sync();
-    if (a.hid) wave_store_tile<16>(sh[w], a.hid, m0, a.N, lane);
-    // layer 2 on top of b2, with the residual h through the MFMA units (an
-    // identity block on the h^T fragments in registers, as in the edge kernel)
+      for (int s4 = 0; s4 < 4; ++s4) *reinterpret_cast<bf16x8*>(a.hid + (long)m * kLat + 32 * s4 + lk) = hb[s4];
+    // layer 2 on top of b2 + h (the residual, from the h^T fragments)
     f32x4 acc2[8];
 #pragma unroll
     for (int ub = 0; ub < 8; ++ub) {
-      const float4 bb = *reinterpret_cast<const float4*>(a.b2 + ub * 16 + 4 * lg);
-      acc2[ub] = f32x4{bb.x, bb.y, bb.z, bb.w};
-    }
-    {
-      bf16x8 I2[2];
-#pragma unroll
-      for (int par = 0; par < 2; ++par)
-#pragma unroll
-        for (int j = 0; j < 8; ++j)
-          I2[par][j] = (bf16)((lg == 2 * par + (lr >> 3) && j == (lr & 7)) ? 1.f : 0.f);
-#pragma unroll
-      for (int ub = 0; ub < 8; ++ub) acc2[ub] = mfma(I2[ub & 1], B[ub >> 1], acc2[ub]);
+      const float4 bb = *reinterpret_cast<const float4*>(a.b2 + 32 * (ub >> 1) + lk + 4 * (ub & 1));
+      const bf16x8 hv = B[ub >> 1];
+      const int j0 = 4 * (ub & 1);
+      acc2[ub] = f32x4{bb.x + (float)hv[j0], bb.y + (float)hv[j0 + 1], bb.z + (float)hv[j0 + 2],
+                       bb.w + (float)hv[j0 + 3]};
     }
 #pragma unroll
-    for (int ks = 0; ks < 4; ++ks) {
-      const bf16x8 hb = *reinterpret_cast<const bf16x8*>(&sh[w][lr][ks * 32 + lk]);
+    for (int ks = 0; ks < 4; ++ks)
 #pragma unroll
       for (int ub = 0; ub < 8; ++ub)
-        acc2[ub] = mfma(*reinterpret_cast<const bf16x8*>(&sw2[ub * 16 + lr][ks * 32 + lk]), hb, acc2[ub]);
-    }
-    wave_lds_sync();
+        acc2[ub] = mfma(*reinterpret_cast<const bf16x8*>(&sw2[ub * 16 + lr][ks * 32 + lk]), hb[ks], acc2[ub]);
+    bf16x8 ob[4];
 #pragma unroll
-    for (int ub = 0; ub < 8; ++ub) {
-      bf16x4 ov;
+    for (int s4 = 0; s4 < 4; ++s4) ob[s4] = pack8<false>(acc2[2 * s4], acc2[2 * s4 + 1]);
+    if (live)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) ov[r] = (bf16)acc2[ub][r];
-      *reinterpret_cast<bf16x4*>(&sh[w][lr][ub * 16 + 4 * lg]) = ov;
-    }
-    wave_lds_sync();
-    wave_store_tile<16>(sh[w], a.out, m0, a.N, lane);
+      for (int s4 = 0; s4 < 4; ++s4) *reinterpret_cast<bf16x8*>(a.out + (long)m * kLat + 32 * s4 + lk) = ob[s4];
     if (a.Wn1t) {
-      // P' = h' W1b (next layer): A = W1b^T fragments from L2, B = h'^T from the tile
+      // P' = h' W1b (next layer): A = W1b^T fragments, B = h'^T from the
+      // output fragments
       f32x4 accp[8];
 #pragma unroll
       for (int ub = 0; ub < 8; ++ub) accp[ub] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int ks = 0; ks < 4; ++ks) {
-        const bf16x8 hb = *reinterpret_cast<const bf16x8*>(&sh[w][lr][ks * 32 + lk]);
+      for (int ks = 0; ks < 4; ++ks)
 #pragma unroll
         for (int ub = 0; ub < 8; ++ub)
-          accp[ub] = mfma(ld8(a.Wn1t + (long)(ub * 16 + lr) * (3 * kLat) + kLat + ks * 32 + lk), hb, accp[ub]);
-      }
-      if (m0 + lr < a.N) {
+          accp[ub] = mfma(*reinterpret_cast<const bf16x8*>(&swn[ub * 16 + lr][ks * 32 + lk]), ob[ks], accp[ub]);
+      if (live) {
 #pragma unroll
         for (int ub = 0; ub < 8; ++ub)
-          *reinterpret_cast<float4*>(a.P + (long)m * kLat + ub * 16 + 4 * lg) =
+          *reinterpret_cast<float4*>(a.P + (long)m * kLat + 32 * (ub >> 1) + lk + 4 * (ub & 1)) =
               make_float4(accp[ub][0], accp[ub][1], accp[ub][2], accp[ub][3]);
       }
     }
-    wave_lds_sync(); // the next tile rewrites the tile
   }
 }
 
@@ -1725,10 +1719,10 @@ int gnn_node_layer_forward(int N, const void* h, const void* e, const int* row, 
   a.P = next_edge ? P : nullptr;
   const int cus = device_cus();
   if (cus <= 0) return -2;
-  constexpr int kWaves = 8; // two per SIMD: the aggregation's FP32 sums need the registers
+  constexpr int kWaves = GNN_NODE_WAVES;
   const long blocks = ((long)N + 16l * kWaves - 1) / (16l * kWaves);
   const int grid = blocks < cus ? (int)blocks : cus;
-  hipLaunchKernelGGL(k_node_fwd<kWaves>, dim3(grid), dim3(64 * kWaves), 0, reinterpret_cast<hipStream_t>(stream),
+  hipLaunchKernelGGL((k_node_fwd<kWaves, GNN_NODE_AGG>), dim3(grid), dim3(64 * kWaves), 0, reinterpret_cast<hipStream_t>(stream),
                      a);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
