@@ -1187,6 +1187,11 @@ struct CellGrid {
 
 constexpr int kScanPerBlock = 4 * kThreads; // elements per block of the scan
 
+// The count sorts each node's senders in LDS (one row per thread) and keeps
+// rows of up to kFillLds of them in the workspace, so the fill copies them;
+// longer rows are found again by the fill and sorted in place in send.
+constexpr int kFillLds = 32;
+
 __device__ inline int cell_coord(double x, double lo, double inv, int nc) {
   double t = (x - lo) * inv;
   if (!(t >= 0.)) t = 0.;          // below the box, or NaN
@@ -1272,22 +1277,26 @@ __global__ __launch_bounds__(kThreads) void k_scan_carry(int* out, long n, const
   if (i < n) out[i] += bsum[i / kScanPerBlock];
 }
 
-__global__ __launch_bounds__(kThreads) void k_g_scatter(int N, const int* cell_of, const int* cstart,
-                                                        int* cursor, int* cnodes) {
+__global__ __launch_bounds__(kThreads) void k_g_scatter(int N, const double* x, const int* cell_of,
+                                                        const int* cstart, int* cursor, int* cnodes,
+                                                        double4* cpos) {
   const int i = blockIdx.x * kThreads + threadIdx.x;
   if (i >= N) return;
   const int c = cell_of[i];
-  cnodes[cstart[c] + atomicAdd(&cursor[c], 1)] = i;
+  const int slot = cstart[c] + atomicAdd(&cursor[c], 1);
+  cnodes[slot] = i;
+  cpos[slot] = make_double4(x[3l * i], x[3l * i + 1], x[3l * i + 2], 0.);
 }
 
 // Visits the nodes j != i with |x_i - x_j|^2 <= r2 in the 27 cells around
-// node i's cell (the order depends on the scatter; callers sort).
+// node i's cell c (the order depends on the scatter; callers sort).  The
+// candidates' positions are read from the cell-ordered copy, so a row of
+// cells is one contiguous stretch of memory.
 template <class F>
-__device__ inline void for_each_neighbor(int i, const double* x, const CellGrid& g, double r2,
-                                         const int* cell_of, const int* cstart, const int* cnodes, F f) {
-  const int c = cell_of[i];
+__device__ inline void for_each_neighbor(int i, double4 xi, int c, const CellGrid& g, double r2,
+                                         const int* cstart, const int* cnodes, const double4* cpos, F f) {
   const int cx = c % g.nc[0], cy = (c / g.nc[0]) % g.nc[1], cz = c / (g.nc[0] * g.nc[1]);
-  const double xi0 = x[3l * i], xi1 = x[3l * i + 1], xi2 = x[3l * i + 2];
+  const double xi0 = xi.x, xi1 = xi.y, xi2 = xi.z;
   for (int z = cz > 0 ? cz - 1 : 0; z <= cz + 1 && z < g.nc[2]; ++z)
     for (int y = cy > 0 ? cy - 1 : 0; y <= cy + 1 && y < g.nc[1]; ++y) {
       const int row0 = (z * g.nc[1] + y) * g.nc[0];
@@ -1296,44 +1305,68 @@ __device__ inline void for_each_neighbor(int i, const double* x, const CellGrid&
       for (int p = cstart[row0 + x0]; p < cstart[row0 + x1 + 1]; ++p) {
         const int j = cnodes[p];
         if (j == i) continue;
-        const double d0 = xi0 - x[3l * j], d1 = xi1 - x[3l * j + 1], d2 = xi2 - x[3l * j + 2];
+        const double4 xj = cpos[p];
+        const double d0 = xi0 - xj.x, d1 = xi1 - xj.y, d2 = xi2 - xj.z;
         if (d0 * d0 + d1 * d1 + d2 * d2 <= r2) f(j);
       }
     }
 }
 
 __global__ __launch_bounds__(kThreads) void k_g_count(int N, const double* x, CellGrid g, double r2,
-                                                      const int* cell_of, const int* cstart,
-                                                      const int* cnodes, int* deg) {
+                                                      const int* cell_of, const int* cstart, const int* cnodes,
+                                                      const double4* cpos, int* deg, int* rows) {
+  __shared__ int sb[kThreads * (kFillLds + 1)]; // thread t's row at t (kFillLds + 1): no bank conflicts
   const int i = blockIdx.x * kThreads + threadIdx.x;
   if (i >= N) return;
+  int* b = sb + threadIdx.x * (kFillLds + 1);
   int n = 0;
-  for_each_neighbor(i, x, g, r2, cell_of, cstart, cnodes, [&](int) { ++n; });
+  for_each_neighbor(i, make_double4(x[3l * i], x[3l * i + 1], x[3l * i + 2], 0.), cell_of[i], g, r2, cstart,
+                    cnodes, cpos, [&](int j) {
+                      if (n < kFillLds) { // insertion into the sorted prefix b[0..n)
+                        int p = n;
+                        while (p > 0 && b[p - 1] > j) {
+                          b[p] = b[p - 1];
+                          --p;
+                        }
+                        b[p] = j;
+                      }
+                      ++n;
+                    });
   deg[i] = n;
+  if (n <= kFillLds)
+    for (int k = 0; k < n; ++k) rows[(long)i * kFillLds + k] = b[k];
 }
 
 __global__ __launch_bounds__(kThreads) void k_g_fill(int N, const double* x, CellGrid g, double r2,
-                                                     const int* cell_of, const int* cstart,
-                                                     const int* cnodes, const int* row, int E,
+                                                     const int* cell_of, const int* cstart, const int* cnodes,
+                                                     const double4* cpos, const int* rows, const int* row, int E,
                                                      int* send, int* recv) {
   const int i = blockIdx.x * kThreads + threadIdx.x;
   if (i >= N) return;
   const int e0 = row[i], e1 = row[i + 1] < E ? row[i + 1] : E;
   if (e0 < 0) return;
-  int n = e0;
-  for_each_neighbor(i, x, g, r2, cell_of, cstart, cnodes, [&](int j) {
-    if (n < e1) { // the count pass saw the same set; the bound only guards memory
-      // insertion into the sorted prefix send[e0..n)
-      int p = n;
-      while (p > e0 && send[p - 1] > j) {
-        send[p] = send[p - 1];
-        --p;
-      }
-      send[p] = j;
-      recv[n] = i;
-      ++n;
+  if (row[i + 1] - e0 <= kFillLds) { // the count kept the sorted row (degree from the count's scan)
+    for (int k = 0; k < e1 - e0; ++k) {
+      send[e0 + k] = rows[(long)i * kFillLds + k];
+      recv[e0 + k] = i;
     }
-  });
+    return;
+  }
+  int n = e0;
+  for_each_neighbor(i, make_double4(x[3l * i], x[3l * i + 1], x[3l * i + 2], 0.), cell_of[i], g, r2, cstart,
+                    cnodes, cpos, [&](int j) {
+                      if (n < e1) { // the count pass saw the same set; the bound only guards memory
+                        // insertion into the sorted prefix send[e0..n)
+                        int p = n;
+                        while (p > e0 && send[p - 1] > j) {
+                          send[p] = send[p - 1];
+                          --p;
+                        }
+                        send[p] = j;
+                        recv[n] = i;
+                        ++n;
+                      }
+                    });
 }
 
 // rev[e] for edge j -> i: the position of i among node j's senders
@@ -1392,6 +1425,8 @@ long scan_blocks(long n) { return (n + kScanPerBlock - 1) / kScanPerBlock; }
 struct GraphWork {
   int *ccount, *cstart, *cursor, *cell_of, *cnodes, *deg, *bsum;
   long long* total64;
+  double4* cpos; // positions by cell slot (cnodes order)
+  int* rows;     // [N][kFillLds]: each node's sorted senders, from the count
   long ints;
 };
 
@@ -1412,6 +1447,8 @@ GraphWork carve(void* base, int N, int ncell) {
   w.deg = take(N);
   w.bsum = take(scan_blocks((long)(N > ncell ? N : ncell) + 1) + 1);
   w.total64 = reinterpret_cast<long long*>(take(2)); // offsets are 256-byte aligned
+  w.cpos = reinterpret_cast<double4*>(take(8l * N));
+  w.rows = take((long)kFillLds * N);
   w.ints = o;
   return w;
 }
@@ -1745,10 +1782,10 @@ int gnn_radius_graph_count(int N, const double* x, double radius, const double l
   if (N > 0) {
     hipLaunchKernelGGL(k_g_bin, dim3(nblk(N)), dim3(kThreads), 0, s, N, x, g, w.cell_of, w.ccount);
     scan(w.ccount, g.ncell, w.cstart, w.bsum, nullptr, s);
-    hipLaunchKernelGGL(k_g_scatter, dim3(nblk(N)), dim3(kThreads), 0, s, N, w.cell_of, w.cstart, w.cursor,
-                       w.cnodes);
+    hipLaunchKernelGGL(k_g_scatter, dim3(nblk(N)), dim3(kThreads), 0, s, N, x, w.cell_of, w.cstart,
+                       w.cursor, w.cnodes, w.cpos);
     hipLaunchKernelGGL(k_g_count, dim3(nblk(N)), dim3(kThreads), 0, s, N, x, g, radius * radius, w.cell_of,
-                       w.cstart, w.cnodes, w.deg);
+                       w.cstart, w.cnodes, w.cpos, w.deg, w.rows);
   }
   scan(w.deg, N, row, w.bsum, w.total64, s);
   long long total = 0;
@@ -1770,7 +1807,7 @@ int gnn_radius_graph_fill(int N, const double* x, double radius, const double lo
   if (N == 0 || n_edges == 0) return 0;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   hipLaunchKernelGGL(k_g_fill, dim3(nblk(N)), dim3(kThreads), 0, s, N, x, g, radius * radius, w.cell_of,
-                     w.cstart, w.cnodes, row, (int)n_edges, send, recv);
+                     w.cstart, w.cnodes, w.cpos, w.rows, row, (int)n_edges, send, recv);
   hipLaunchKernelGGL(k_g_rev, dim3(nblk(N)), dim3(kThreads), 0, s, N, row, (int)n_edges, send, rev);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }

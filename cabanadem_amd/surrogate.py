@@ -333,10 +333,17 @@ class Network:
 
     # -- buffers -----------------------------------------------------------
     def _ensure(self, N, E, train):
+        """Buffers for N nodes and E edges.  They grow with 25 % headroom, since
+        the graph's edge count drifts from step to step; the hidden layers and
+        the gradient buffers exist only once the network has trained."""
         torch = _torch()
         if N <= self.cap_N and E <= self.cap_E and (not train or self.part is not None):
             return
-        N, E = max(N, self.cap_N), max(E, self.cap_E)
+        if N > self.cap_N or E > self.cap_E:
+            N, E = max(N * 5 // 4, self.cap_N), max(E * 5 // 4, self.cap_E)
+        else:
+            N, E = self.cap_N, self.cap_E
+        train = train or self.part is not None
         bf, f32, dev = torch.bfloat16, torch.float32, self.dev
         self.cap_N, self.cap_E = N, E
         L = self.L
@@ -344,12 +351,14 @@ class Network:
         self.h = [z(N, LAT) for _ in range(L + 1)]
         self.e = [z(E, LAT) for _ in range(L + 1)]
         self.agg = [z(N, LAT) for _ in range(L)]
-        self.hid_e = [z(E, LAT) for _ in range(L)]
-        self.hid_n = [z(N, LAT) for _ in range(L)]
-        self.hid_enc_n, self.hid_enc_e, self.hid_dec = z(N, LAT), z(E, LAT), z(N, LAT)
         self.out = z(N, 3, dt=f32)
         self.P = z(N, LAT, dt=f32)
+        self.hid_e, self.hid_n = [None] * L, [None] * L
+        self.hid_enc_n = self.hid_enc_e = self.hid_dec = None
         if train:
+            self.hid_e = [z(E, LAT) for _ in range(L)]
+            self.hid_n = [z(N, LAT) for _ in range(L)]
+            self.hid_enc_n, self.hid_enc_e, self.hid_dec = z(N, LAT), z(E, LAT), z(N, LAT)
             self.d_h = z(N, LAT, dt=f32)
             self.d_agg = z(N, LAT, dt=f32)
             self.d_e = z(E, LAT, dt=f32)
