@@ -323,19 +323,35 @@ def run_surrogate(args):
     ro.dem.close()
 
     # judged roll-out against a DEM reference over the same horizon
+    # The plain GPU DEM runs the autotuner's configuration (the fastest
+    # rebuild cadence the skin check allows), and so do the judge's checks;
+    # the workload's own cadence is timed beside it.
     n_macro = args.surrogate_macro
-    ro = S.Rollout(net, wl, sc, check_every=10, tol=0.02)
+    ro = S.Rollout(net, wl, sc, check_every=10, tol=0.02, tune_dem=True)
     res = ro.run(n_macro)
     x_sur = ro.x.cpu().numpy()
-    ref = wl.system()
-    tt = time.perf_counter()
-    ref.step(S.MACRO * (S.HIST + n_macro))
-    ref.sync()
-    dem_wall = time.perf_counter() - tt
-    st = ref.download(fields=("x",))
-    ref.close()
-    horizon_err = float(np.linalg.norm(x_sur - st.x, axis=1).mean() / wl.state.rad.mean())
-    dem_ms_macro = dem_wall * 1e3 / (S.HIST + n_macro)
+
+    def dem_reference(tuned):
+        ref = wl.system()
+        tune = ref.autotune(trial_steps=80) if tuned else None
+        tt = time.perf_counter()
+        ref.step(S.MACRO * (S.HIST + n_macro))
+        ref.sync()
+        ms = (time.perf_counter() - tt) * 1e3 / (S.HIST + n_macro)
+        st = ref.download(fields=("x",))
+        ref.close()
+        return ms, st.x, tune
+
+    dem_ms_fixed, _, _ = dem_reference(False)
+    try:
+        dem_ms_macro, x_ref, tune = dem_reference(True)
+        ref_cfg = {"rebuild_every": tune.rebuild_every, "grid_blocks": tune.grid_blocks,
+                   "graph": bool(tune.use_graph)}
+    except Exception as exc:  # the tuned cadence broke the skin check later on
+        log(f"[surrogate] tuned DEM reference failed ({exc}); using the workload's cadence")
+        dem_ms_macro, x_ref, _ = dem_reference(False)
+        ref_cfg = {"rebuild_every": wl.rebuild_every, "grid_blocks": 0, "graph": False}
+    horizon_err = float(np.linalg.norm(x_sur - x_ref, axis=1).mean() / wl.state.rad.mean())
     sur_ms = res["surrogate_ms_per_macro"]
     speed_no_checks = dem_ms_macro / sur_ms
     k = ro.check_every
@@ -355,7 +371,15 @@ def run_surrogate(args):
                    "parallelism": "single GPU"},
         "speedup_without_checks": speed_no_checks,
         "rollout": dict(res, horizon_mean_position_error_over_r=horizon_err,
-                        dem_ms_per_macro_reference_run=dem_ms_macro),
+                        dem_ms_per_macro_reference_run=dem_ms_macro,
+                        dem_reference_config=ref_cfg,
+                        dem_check_config=({"rebuild_every": ro.dem_tune.rebuild_every,
+                                           "grid_blocks": ro.dem_tune.grid_blocks,
+                                           "graph": bool(ro.dem_tune.use_graph)} if ro.dem_tune else None),
+                        workload_rebuild_every=wl.rebuild_every,
+                        dem_ms_per_macro_workload_cadence=dem_ms_fixed,
+                        speedup_vs_workload_cadence=dem_ms_fixed * ro.check_every
+                        / (res["surrogate_ms_per_macro"] * ro.check_every + res["dem_ms_per_macro"])),
         "one_step_mse_held_out": {"untrained": before, "trained": after, "no_network": baseline},
         "mfma": {"forward_ms": fwd_ms, "nodes": g.N, "edges": g.E, "flops": flops,
                  "achieved_tflops": mfma_tf, "peak_tflops": 2500.0, "frac": mfma_tf / 2500.0},

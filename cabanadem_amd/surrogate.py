@@ -626,12 +626,15 @@ class Rollout:
     re-simulates the same macro-step from the same state; the mean position
     error |x_surrogate - x_DEM| / r_mean and the kinetic and mechanical
     (kinetic + gravitational) energy errors are recorded and, above `tol`
-    in position, the run continues from the DEM state."""
+    in position, the run continues from the DEM state.  tune_dem: the judge's
+    DEM context runs the autotuner's configuration (rebuild cadence, launch
+    shape), picked from the warm-up state."""
 
-    def __init__(self, net: Network, wl, scales: Scales, check_every=10, tol=0.02):
+    def __init__(self, net: Network, wl, scales: Scales, check_every=10, tol=0.02, tune_dem=False):
         torch = _torch()
         self.net, self.wl, self.sc = net, wl, scales
         self.check_every, self.tol = check_every, tol
+        self.tune_dem, self.dem_tune = tune_dem, None
         n = wl.state.n
         self.n = n
         self.dem = wl.system()
@@ -665,6 +668,8 @@ class Rollout:
     def warm_up(self):
         """HIST macro-steps of DEM to fill the velocity history."""
         torch = _torch()
+        if self.tune_dem and self.dem_tune is None:
+            self.dem_tune = self.dem.autotune(trial_steps=80)  # restores the state
         vh = []
         for _ in range(HIST):
             self.dem.step(MACRO)
