@@ -442,17 +442,23 @@ struct EdgeFwdArgs {
   const float* b2;    // [128]
   bf16* hid;          // [E][128] post-ReLU hidden, or null
   bf16* out;          // [E][128]: e + MLP
+  // with the encoder fused in (e not read): e = We2 relu(We1 xe + be1) + be2
+  const bf16* xe;     // [E][32]
+  const bf16* We1t;   // [128][32]
+  const float* be1;
+  const bf16* We2t;   // [128][128]
+  const float* be2;
 };
 
-// A wave tile of 16 EB edges: their rows, receivers, and the e^T and
-// h[send]^T fragments.
+// A wave tile of 16 EB edges: their rows, receivers, and the e^T (or, with
+// the encoder fused in, xe^T) and h[send]^T fragments.
 template <int EB>
 struct EdgeTile {
   int ed[EB], ri[EB];
-  bf16x8 Be[EB][4], Bh[EB][4];
+  bf16x8 Be[EB][4], Bh[EB][4], Bx[EB];
 };
 
-template <int EB>
+template <int EB, bool ENC>
 __device__ inline void edge_tile_load(const EdgeFwdArgs& a, int e0, int lr, int lk, EdgeTile<EB>& T) {
 #pragma unroll
   for (int eb = 0; eb < EB; ++eb) {
@@ -460,9 +466,10 @@ __device__ inline void edge_tile_load(const EdgeFwdArgs& a, int e0, int lr, int 
     T.ed[eb] = ee < a.E ? ee : a.E - 1; // clamped loads; stores are guarded
     T.ri[eb] = a.recv[T.ed[eb]];
     const int sj = a.send[T.ed[eb]];
+    if (ENC) T.Bx[eb] = ld8(a.xe + (long)T.ed[eb] * 32 + lk);
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) {
-      T.Be[eb][ks] = ld8(a.e + (long)T.ed[eb] * kLat + ks * 32 + lk);
+      if (!ENC) T.Be[eb][ks] = ld8(a.e + (long)T.ed[eb] * kLat + ks * 32 + lk);
       T.Bh[eb][ks] = ld8(a.h + (long)sj * kLat + ks * 32 + lk);
     }
   }
@@ -503,17 +510,25 @@ __device__ inline bf16x8 pack8(f32x4 lo, f32x4 hi) {
 // per tile and applied to its EB edge blocks; nothing else goes through LDS.
 // With a grid that is a multiple of 8, XCD (blockIdx % 8) takes a contiguous
 // eighth of the tiles, so the rows its edges gather stay in that XCD's L2.
-template <int EB, int WAVES>
+// ENC: the edge encoder runs first on each tile (its weights in LDS too),
+// in the dense kernel's arithmetic, so e is never stored or read.
+template <int EB, int WAVES, bool ENC>
 __global__ __launch_bounds__(64 * WAVES, 1) void k_edge_fwd(EdgeFwdArgs a) {
   constexpr int TE = 16 * EB;
   __shared__ bf16 swa[kLat][kWPitch]; // W1a^T: [row][k]
   __shared__ bf16 swc[kLat][kWPitch]; // W1c^T: [row][k]
   __shared__ bf16 sw2[kLat][kWPitch]; // W2^T:  [row][k]
+  __shared__ bf16 se1[ENC ? kLat : 1][32 + 8];   // encoder W1^T: [row][k]
+  __shared__ bf16 se2[ENC ? kLat : 1][kWPitch];  // encoder W2^T: [row][k]
   for (int c = threadIdx.x; c < kLat * 16; c += 64 * WAVES) {
     const int r = c >> 4, k8 = (c & 15) * 8, u = unit_of_row(r);
     *reinterpret_cast<bf16x8*>(&swa[r][k8]) = ld8(a.W1t + (long)u * (3 * kLat) + k8);
     *reinterpret_cast<bf16x8*>(&swc[r][k8]) = ld8(a.W1t + (long)u * (3 * kLat) + 2 * kLat + k8);
     *reinterpret_cast<bf16x8*>(&sw2[r][k8]) = ld8(a.W2t + (long)u * kLat + k8);
+    if (ENC) {
+      *reinterpret_cast<bf16x8*>(&se2[r][k8]) = ld8(a.We2t + (long)u * kLat + k8);
+      if (k8 < 32) *reinterpret_cast<bf16x8*>(&se1[r][k8]) = ld8(a.We1t + (long)u * 32 + k8);
+    }
   }
   __syncthreads(); // the only block barrier: the loop below is per wave
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -533,14 +548,42 @@ __global__ __launch_bounds__(64 * WAVES, 1) void k_edge_fwd(EdgeFwdArgs a) {
   }
   if (t >= t_end) return;
   EdgeTile<EB> cur, nxt;
-  edge_tile_load<EB>(a, t * TE, lr, lk, cur);
+  edge_tile_load<EB, ENC>(a, t * TE, lr, lk, cur);
   for (; t < t_end; t += stride) {
     const int e0 = t * TE;
     const int tn = t + stride;
-    if (tn < t_end) edge_tile_load<EB>(a, tn * TE, lr, lk, nxt);
+    if (tn < t_end) edge_tile_load<EB, ENC>(a, tn * TE, lr, lk, nxt);
     // keeps the weight fragments' LDS reads inside the loop: hoisted out of
     // it as loop-invariant they would need 384 registers
     __asm__ volatile("" ::: "memory");
+    if (ENC) {
+      // e^T fragments = encoder(xe^T), as k_dense_fwd<32> computes them
+#pragma unroll
+      for (int eb = 0; eb < EB; ++eb) {
+        f32x4 q[8];
+#pragma unroll
+        for (int ub = 0; ub < 8; ++ub) {
+          const float4 bb = *reinterpret_cast<const float4*>(a.be1 + 32 * (ub >> 1) + lk + 4 * (ub & 1));
+          q[ub] = mfma(*reinterpret_cast<const bf16x8*>(&se1[ub * 16 + lr][lk]), cur.Bx[eb],
+                       f32x4{bb.x, bb.y, bb.z, bb.w});
+        }
+        bf16x8 qb[4];
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) qb[s4] = pack8<true>(q[2 * s4], q[2 * s4 + 1]);
+#pragma unroll
+        for (int ub = 0; ub < 8; ++ub) {
+          const float4 bb = *reinterpret_cast<const float4*>(a.be2 + 32 * (ub >> 1) + lk + 4 * (ub & 1));
+          q[ub] = f32x4{bb.x, bb.y, bb.z, bb.w};
+        }
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks)
+#pragma unroll
+          for (int ub = 0; ub < 8; ++ub)
+            q[ub] = mfma(*reinterpret_cast<const bf16x8*>(&se2[ub * 16 + lr][ks * 32 + lk]), qb[ks], q[ub]);
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) cur.Be[eb][s4] = pack8<false>(q[2 * s4], q[2 * s4 + 1]);
+      }
+    }
     // layer 1 on top of b1 + P[recv]: block 2 s + t holds units 32 s + 8 lg + 4 t .. + 3
     f32x4 acc[8][EB];
 #pragma unroll
@@ -630,9 +673,9 @@ constexpr int kEdgeEB = GNN_EDGE_EB, kEdgeWaves = GNN_EDGE_WAVES;
 // Dense MLP y = res + W2 relu(W1 [x0, x1] + b1) + b2 with 128 outputs, the
 // inputs ungathered rows (K = 32, 128 or 256: x1 holds columns 128..255).
 // The edge kernel's scheme: persistent, wave-granular, weights in LDS for
-// the whole block, transposed products (rows are the 16 MFMA columns), the
-// next tile's rows loaded while the current one multiplies, and the output
-// through LDS so that whole rows are stored.
+// the whole block (rows in unit_of_row order), transposed products (rows are
+// the 16 MFMA columns), the next tile's rows loaded while the current one
+// multiplies, and the hidden layer and output in registers.
 // ---------------------------------------------------------------------------
 
 struct DenseFwdArgs {
@@ -652,16 +695,15 @@ template <int K, int WAVES>
 __global__ __launch_bounds__(64 * WAVES, 1) void k_dense_fwd(DenseFwdArgs a) {
   constexpr int KS = K / 32;          // k steps of the first layer
   constexpr int W0 = K < kLat ? K : kLat; // width of x0
-  __shared__ bf16 sw1[kLat][K + 8];   // W1^T: [unit][k]
-  __shared__ bf16 sw2[kLat][kWPitch]; // W2^T: [out][k]
-  __shared__ bf16 sh[WAVES][16][kWPitch];
+  __shared__ bf16 sw1[kLat][K + 8];   // W1^T: [row][k], rows in unit_of_row order
+  __shared__ bf16 sw2[kLat][kWPitch]; // W2^T: [row][k], likewise
   for (int c = threadIdx.x; c < kLat * (K / 8); c += 64 * WAVES) {
     const int r = c / (K / 8), k8 = (c % (K / 8)) * 8;
-    *reinterpret_cast<bf16x8*>(&sw1[r][k8]) = ld8(a.W1t + (long)r * K + k8);
+    *reinterpret_cast<bf16x8*>(&sw1[r][k8]) = ld8(a.W1t + (long)unit_of_row(r) * K + k8);
   }
   for (int c = threadIdx.x; c < kLat * 16; c += 64 * WAVES) {
     const int r = c >> 4, k8 = (c & 15) * 8;
-    *reinterpret_cast<bf16x8*>(&sw2[r][k8]) = ld8(a.W2t + (long)r * kLat + k8);
+    *reinterpret_cast<bf16x8*>(&sw2[r][k8]) = ld8(a.W2t + (long)unit_of_row(r) * kLat + k8);
   }
   __syncthreads(); // the only block barrier
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -681,14 +723,18 @@ __global__ __launch_bounds__(64 * WAVES, 1) void k_dense_fwd(DenseFwdArgs a) {
   bf16x8 cur[KS], nxt[KS];
   load_rows(t * 16, cur);
   for (; t < ntiles; t += stride) {
+    // keeps the weight fragments' LDS reads inside the loop
+    __asm__ volatile("" ::: "memory");
     const int m0 = t * 16;
     const int tn = t + stride;
     if (tn < ntiles) load_rows(tn * 16, nxt);
-    const int mrow = m0 + lr < a.M ? m0 + lr : a.M - 1;
+    const bool live = m0 + lr < a.M;
+    const int mrow = live ? m0 + lr : a.M - 1;
+    // layer 1: block 2 s + t holds units 32 s + 8 lg + 4 t .. + 3
     f32x4 acc[8];
 #pragma unroll
     for (int ub = 0; ub < 8; ++ub) {
-      const float4 bb = *reinterpret_cast<const float4*>(a.b1 + ub * 16 + 4 * lg);
+      const float4 bb = *reinterpret_cast<const float4*>(a.b1 + 32 * (ub >> 1) + lk + 4 * (ub & 1));
       acc[ub] = f32x4{bb.x, bb.y, bb.z, bb.w};
     }
 #pragma unroll
@@ -696,44 +742,37 @@ __global__ __launch_bounds__(64 * WAVES, 1) void k_dense_fwd(DenseFwdArgs a) {
 #pragma unroll
       for (int ub = 0; ub < 8; ++ub)
         acc[ub] = mfma(*reinterpret_cast<const bf16x8*>(&sw1[ub * 16 + lr][ks * 32 + lk]), cur[ks], acc[ub]);
+    bf16x8 hb[4];
 #pragma unroll
-    for (int ub = 0; ub < 8; ++ub) {
-      bf16x4 hv;
+    for (int s4 = 0; s4 < 4; ++s4) hb[s4] = pack8<true>(acc[2 * s4], acc[2 * s4 + 1]);
+    if (a.hid && live)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) hv[r] = (bf16)(acc[ub][r] > 0.f ? acc[ub][r] : 0.f);
-      *reinterpret_cast<bf16x4*>(&sh[w][lr][ub * 16 + 4 * lg]) = hv;
-    }
-    wave_lds_sync();
-    if (a.hid) wave_store_tile<16>(sh[w], a.hid, m0, a.M, lane);
+      for (int s4 = 0; s4 < 4; ++s4) *reinterpret_cast<bf16x8*>(a.hid + (long)mrow * kLat + 32 * s4 + lk) = hb[s4];
+    // layer 2 on top of b2 (+ the residual row)
     f32x4 acc2[8];
 #pragma unroll
-    for (int ub = 0; ub < 8; ++ub) {
-      const int u = ub * 16 + 4 * lg;
-      const float4 bb = *reinterpret_cast<const float4*>(a.b2 + u);
-      acc2[ub] = f32x4{bb.x, bb.y, bb.z, bb.w};
+    for (int s4 = 0; s4 < 4; ++s4) {
+      const int u = 32 * s4 + lk;
+      const float4 b0 = *reinterpret_cast<const float4*>(a.b2 + u);
+      const float4 b1 = *reinterpret_cast<const float4*>(a.b2 + u + 4);
+      acc2[2 * s4] = f32x4{b0.x, b0.y, b0.z, b0.w};
+      acc2[2 * s4 + 1] = f32x4{b1.x, b1.y, b1.z, b1.w};
       if (a.res) {
-        const bf16x4 rv = *reinterpret_cast<const bf16x4*>(a.res + (long)mrow * kLat + u);
-        acc2[ub] += f32x4{(float)rv[0], (float)rv[1], (float)rv[2], (float)rv[3]};
+        const bf16x8 rv = ld8(a.res + (long)mrow * kLat + u);
+        acc2[2 * s4] += f32x4{(float)rv[0], (float)rv[1], (float)rv[2], (float)rv[3]};
+        acc2[2 * s4 + 1] += f32x4{(float)rv[4], (float)rv[5], (float)rv[6], (float)rv[7]};
       }
     }
 #pragma unroll
-    for (int ks = 0; ks < 4; ++ks) {
-      const bf16x8 hb = *reinterpret_cast<const bf16x8*>(&sh[w][lr][ks * 32 + lk]);
+    for (int ks = 0; ks < 4; ++ks)
 #pragma unroll
       for (int ub = 0; ub < 8; ++ub)
-        acc2[ub] = mfma(*reinterpret_cast<const bf16x8*>(&sw2[ub * 16 + lr][ks * 32 + lk]), hb, acc2[ub]);
-    }
-    wave_lds_sync();
+        acc2[ub] = mfma(*reinterpret_cast<const bf16x8*>(&sw2[ub * 16 + lr][ks * 32 + lk]), hb[ks], acc2[ub]);
+    if (live)
 #pragma unroll
-    for (int ub = 0; ub < 8; ++ub) {
-      bf16x4 ov;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) ov[r] = (bf16)acc2[ub][r];
-      *reinterpret_cast<bf16x4*>(&sh[w][lr][ub * 16 + 4 * lg]) = ov;
-    }
-    wave_lds_sync();
-    wave_store_tile<16>(sh[w], a.out, m0, a.M, lane);
-    wave_lds_sync(); // the next tile rewrites the tile
+      for (int s4 = 0; s4 < 4; ++s4)
+        *reinterpret_cast<bf16x8*>(a.out + (long)mrow * kLat + 32 * s4 + lk) =
+            pack8<false>(acc2[2 * s4], acc2[2 * s4 + 1]);
     if (tn < ntiles) {
 #pragma unroll
       for (int ks = 0; ks < KS; ++ks) cur[ks] = nxt[ks];
@@ -746,8 +785,9 @@ __global__ __launch_bounds__(64 * WAVES, 1) void k_dense_fwd(DenseFwdArgs a) {
 //   agg = sum of the node's edge rows (gnn_segment_sum, same order),
 //   h'  = h + W2 relu(W1 [h, agg] + b1) + b2,
 //   P'  = h' W1b of the next layer's edge MLP (gnn_node_p),
-// with the dense kernel's scheme (weights in LDS, transposed products, rows
-// through LDS).  The next layer's W1b is read from L2 as the A operand.
+// with the dense kernel's scheme: weights in LDS (rows in unit_of_row
+// order, the next layer's W1b beside them), transposed products, and the
+// layers chained in registers.
 // ---------------------------------------------------------------------------
 
 struct NodeFwdArgs {
@@ -1525,6 +1565,25 @@ int check_io(const gnn_mlp_io* io) {
   return 0;
 }
 
+// one block per CU (the weights in LDS, shared by its waves), persistent
+template <bool ENC>
+int launch_edge(const EdgeFwdArgs& a, int grid_cap, void* stream) {
+  const int cus = device_cus();
+  if (cus <= 0) return -2;
+  const long per_block = 16l * kEdgeEB * kEdgeWaves;
+  const long nblocks = ((long)a.E + per_block - 1) / per_block;
+  int grid = grid_cap > 0 ? grid_cap : cus;
+  if (grid > nblocks) grid = (int)nblocks;
+  hipLaunchKernelGGL((k_edge_fwd<kEdgeEB, kEdgeWaves, ENC>), dim3(grid), dim3(64 * kEdgeWaves), 0,
+                     reinterpret_cast<hipStream_t>(stream), a);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+bool edge_weights_ok(const gnn_mlp_weights* wt) {
+  return wt && wt->K == 3 * kLat && wt->nout == kLat && wt->nout_pad == kLat && wt->W1t && wt->W2t && wt->b1 &&
+         wt->b2;
+}
+
 } // namespace
 
 extern "C" {
@@ -1704,10 +1763,9 @@ int gnn_node_p(int N, const void* h, const gnn_mlp_weights* wt, float* P, void* 
 int gnn_edge_mlp_forward(long E, const void* e, const int* recv, const int* send, const float* P,
                          const void* h, const gnn_mlp_weights* wt, void* hidden, void* out, int grid_cap,
                          void* stream) {
-  if (E < 0 || E > 0x7fffffffl || !wt || wt->K != 3 * kLat || wt->nout != kLat || wt->nout_pad != kLat)
-    return -1;
+  if (E < 0 || E > 0x7fffffffl || !edge_weights_ok(wt)) return -1;
   if (E == 0) return 0;
-  if (!e || !recv || !send || !P || !h || !out || !wt->W1t || !wt->W2t || !wt->b1 || !wt->b2) return -1;
+  if (!e || !recv || !send || !P || !h || !out) return -1;
   EdgeFwdArgs a{};
   a.E = (int)E;
   a.e = reinterpret_cast<const bf16*>(e);
@@ -1721,16 +1779,34 @@ int gnn_edge_mlp_forward(long E, const void* e, const int* recv, const int* send
   a.b2 = wt->b2;
   a.hid = reinterpret_cast<bf16*>(hidden);
   a.out = reinterpret_cast<bf16*>(out);
-  // one block per CU (the weights in LDS, shared by its waves), persistent
-  const int cus = device_cus();
-  if (cus <= 0) return -2;
-  const long per_block = 16l * kEdgeEB * kEdgeWaves;
-  const long nblocks = (E + per_block - 1) / per_block;
-  int grid = grid_cap > 0 ? grid_cap : cus;
-  if (grid > nblocks) grid = (int)nblocks;
-  hipLaunchKernelGGL((k_edge_fwd<kEdgeEB, kEdgeWaves>), dim3(grid), dim3(64 * kEdgeWaves), 0,
-                     reinterpret_cast<hipStream_t>(stream), a);
-  return hipGetLastError() == hipSuccess ? 0 : -2;
+  return launch_edge<false>(a, grid_cap, stream);
+}
+
+int gnn_edge_mlp_forward_encoded(long E, const void* xe, const gnn_mlp_weights* enc, const int* recv,
+                                 const int* send, const float* P, const void* h, const gnn_mlp_weights* wt,
+                                 void* out, int grid_cap, void* stream) {
+  if (E < 0 || E > 0x7fffffffl || !edge_weights_ok(wt) || !enc || enc->K != 32 || enc->nout != kLat ||
+      enc->nout_pad != kLat || !enc->W1t || !enc->W2t || !enc->b1 || !enc->b2)
+    return -1;
+  if (E == 0) return 0;
+  if (!xe || !recv || !send || !P || !h || !out) return -1;
+  EdgeFwdArgs a{};
+  a.E = (int)E;
+  a.recv = recv;
+  a.send = send;
+  a.P = P;
+  a.h = reinterpret_cast<const bf16*>(h);
+  a.W1t = reinterpret_cast<const bf16*>(wt->W1t);
+  a.b1 = wt->b1;
+  a.W2t = reinterpret_cast<const bf16*>(wt->W2t);
+  a.b2 = wt->b2;
+  a.out = reinterpret_cast<bf16*>(out);
+  a.xe = reinterpret_cast<const bf16*>(xe);
+  a.We1t = reinterpret_cast<const bf16*>(enc->W1t);
+  a.be1 = enc->b1;
+  a.We2t = reinterpret_cast<const bf16*>(enc->W2t);
+  a.be2 = enc->b2;
+  return launch_edge<true>(a, grid_cap, stream);
 }
 
 int gnn_node_layer_forward(int N, const void* h, const void* e, const int* row, const gnn_mlp_weights* wt,

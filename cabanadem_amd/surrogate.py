@@ -37,7 +37,7 @@ GNN_LIB_PATH = Path(os.environ.get("CABANADEM_GNN_LIB") or
 LAT = 128
 FEAT = 32
 HIST = 5
-GNN_ABI_VERSION = 3
+GNN_ABI_VERSION = 4
 
 _vp = C.c_void_p
 _ip = C.POINTER(C.c_int)
@@ -77,6 +77,8 @@ _SIG = {
     "gnn_node_p": (C.c_int, [C.c_int, _vp, C.POINTER(GnnMlpWeights), _vp, _vp]),
     "gnn_edge_mlp_forward": (C.c_int, [C.c_long, _vp, _vp, _vp, _vp, _vp, C.POINTER(GnnMlpWeights), _vp,
                                        _vp, C.c_int, _vp]),
+    "gnn_edge_mlp_forward_encoded": (C.c_int, [C.c_long, _vp, C.POINTER(GnnMlpWeights), _vp, _vp, _vp, _vp,
+                                               C.POINTER(GnnMlpWeights), _vp, C.c_int, _vp]),
     "gnn_node_layer_forward": (C.c_int, [C.c_int, _vp, _vp, _vp, C.POINTER(GnnMlpWeights), _vp, _vp, _vp,
                                          C.POINTER(GnnMlpWeights), _vp, _vp]),
     "gnn_radius_graph_workspace": (C.c_long, [C.c_int, _dp, _dp, C.c_double]),
@@ -380,7 +382,10 @@ class Network:
             Lb.gnn_mlp_forward(C.byref(inp), C.byref(mlp.weights(F)), ptr(res), ptr(hid) if train else None,
                                ptr(out), ptr(outf), 0, s), "mlp forward")
         fwd(self.enc_n, io(N, (xn, None)), None, self.hid_enc_n, self.h[0], None)
-        if E > 0:
+        # inference fuses the edge encoder into the first edge MLP (e[0] is
+        # then never formed); training keeps it for the backward pass
+        fuse_enc = E > 0 and self.factorised and not train
+        if E > 0 and not fuse_enc:
             fwd(self.enc_e, io(E, (xe, None)), None, self.hid_enc_e, self.e[0], None)
         if E > 0 and self.factorised:
             # per layer: the edge MLP, then the node side in one pass (the
@@ -388,10 +393,16 @@ class Network:
             _ok(Lb.gnn_node_p(N, ptr(self.h[0]), C.byref(self.edge[0].weights(F)), ptr(self.P), s), "node P")
             for l in range(self.L):
                 h, e = self.h[l], self.e[l]
-                _ok(Lb.gnn_edge_mlp_forward(E, ptr(e), ptr(g.recv), ptr(g.send), ptr(self.P), ptr(h),
-                                            C.byref(self.edge[l].weights(F)),
-                                            ptr(self.hid_e[l]) if train else None, ptr(self.e[l + 1]), 0, s),
-                    "edge MLP forward")
+                if l == 0 and fuse_enc:
+                    _ok(Lb.gnn_edge_mlp_forward_encoded(E, ptr(xe), C.byref(self.enc_e.weights(F)), ptr(g.recv),
+                                                        ptr(g.send), ptr(self.P), ptr(h),
+                                                        C.byref(self.edge[0].weights(F)), ptr(self.e[1]), 0, s),
+                        "encoded edge MLP forward")
+                else:
+                    _ok(Lb.gnn_edge_mlp_forward(E, ptr(e), ptr(g.recv), ptr(g.send), ptr(self.P), ptr(h),
+                                                C.byref(self.edge[l].weights(F)),
+                                                ptr(self.hid_e[l]) if train else None, ptr(self.e[l + 1]), 0, s),
+                        "edge MLP forward")
                 nxt = C.byref(self.edge[l + 1].weights(F)) if l + 1 < self.L else None
                 _ok(Lb.gnn_node_layer_forward(N, ptr(h), ptr(self.e[l + 1]), ptr(g.row),
                                               C.byref(self.node[l].weights(F)),

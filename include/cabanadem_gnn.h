@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define GNN_ABI_VERSION 3
+#define GNN_ABI_VERSION 4
 #define GNN_LATENT 128
 
 /* Input rows of an MLP: the concatenation of up to three BF16 sources;
@@ -123,6 +123,14 @@ int gnn_node_p(int N, const void* h, const gnn_mlp_weights* w, float* P, void* s
 int gnn_edge_mlp_forward(long E, const void* e, const int* recv, const int* send, const float* P,
                          const void* h, const gnn_mlp_weights* w, void* hidden, void* out,
                          int grid_cap, void* stream);
+/* gnn_edge_mlp_forward_encoded: gnn_edge_mlp_forward with e = the edge
+ * encoder enc (K = 32, 128 outputs, no residual) applied to xe (bf16
+ * [E][32]) inside the same kernel, so the encoded rows are never stored:
+ * bitwise the result of gnn_mlp_forward(xe, enc) followed by
+ * gnn_edge_mlp_forward.  Inference only (no hidden rows). */
+int gnn_edge_mlp_forward_encoded(long E, const void* xe, const gnn_mlp_weights* enc, const int* recv,
+                                 const int* send, const float* P, const void* h,
+                                 const gnn_mlp_weights* w, void* out, int grid_cap, void* stream);
 
 /* The node side of a message-passing layer in one pass:
  *   agg (bf16 [N][128], optional output) = the sum of e (bf16 [E][128]) over

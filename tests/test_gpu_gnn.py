@@ -303,6 +303,57 @@ def radius_graph(n, seed, radius=2.5e-3):
     return gb.build(x), x
 
 
+@pytest.mark.parametrize("M", [1, 77, 4133, 70001])
+def test_encoded_edge_mlp_equals_encoder_then_edge_mlp(gpu_lib, M):
+    """gnn_edge_mlp_forward_encoded (the edge encoder inside the edge kernel)
+    against gnn_mlp_forward(xe, encoder) followed by gnn_edge_mlp_forward:
+    the same arithmetic, so the same bits, for any grid."""
+    rng = np.random.default_rng(M + 11)
+    Nn = 97 if M < 10000 else 5000
+    xe = np.zeros((M, 32)); xe[:, :4] = rng.normal(size=(M, 4))
+    xe = bf(xe).cuda()
+    h = bf(rng.normal(size=(Nn, 128))).cuda()
+    ia = torch.from_numpy(rng.integers(0, Nn, size=M).astype(np.int32)).cuda()
+    ib = torch.from_numpy(rng.integers(0, Nn, size=M).astype(np.int32)).cuda()
+    enc, fe, _ = make_mlp(32, 128, seed=M + 12)
+    m, fl, _ = make_mlp(384, 128, seed=M + 13)
+    Lb, s = S.glib(), S.stream()
+    we, w = enc.weights(fe), m.weights(fl)
+    P = torch.empty((Nn, 128), dtype=torch.float32, device="cuda")
+    assert Lb.gnn_node_p(Nn, S.ptr(h), w, S.ptr(P), s) == 0
+    e0 = torch.empty((M, 128), dtype=torch.bfloat16, device="cuda")
+    assert Lb.gnn_mlp_forward(S.io(M, (xe, None)), we, None, None, S.ptr(e0), None, 0, s) == 0
+    ref = torch.empty((M, 128), dtype=torch.bfloat16, device="cuda")
+    assert Lb.gnn_edge_mlp_forward(M, S.ptr(e0), S.ptr(ia), S.ptr(ib), S.ptr(P), S.ptr(h), w, None,
+                                   S.ptr(ref), 0, s) == 0
+    for cap in (0, 3):
+        out = torch.full((M, 128), float("nan"), dtype=torch.bfloat16, device="cuda")
+        assert Lb.gnn_edge_mlp_forward_encoded(M, S.ptr(xe), we, S.ptr(ia), S.ptr(ib), S.ptr(P), S.ptr(h), w,
+                                               S.ptr(out), cap, s) == 0
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref), cap
+    # the encoder must be a K = 32 MLP with 128 outputs
+    assert Lb.gnn_edge_mlp_forward_encoded(M, S.ptr(xe), w, S.ptr(ia), S.ptr(ib), S.ptr(P), S.ptr(h), w,
+                                           S.ptr(out), 0, s) == -1
+
+
+def test_network_inference_forward_equals_training_forward(gpu_lib):
+    """Inference fuses the edge encoder into the first edge MLP and skips the
+    stored hidden layers; its output is bitwise the training forward's."""
+    g, _ = radius_graph(2000, seed=14)
+    rng = np.random.default_rng(15)
+    xn = np.zeros((g.N, 32)); xn[:, :20] = rng.normal(size=(g.N, 20))
+    xe = np.zeros((g.E, 32)); xe[:, :4] = rng.normal(size=(g.E, 4))
+    xn_d, xe_d = bf(xn).cuda(), bf(xe).cuda()
+    net = S.Network(n_layers=3, seed=16, residual_scale=0.5)
+    y_inf = net.forward(g, xn_d, xe_d).clone()
+    y_train = net.forward(g, xn_d, xe_d, train=True).clone()
+    y_inf2 = net.forward(g, xn_d, xe_d).clone()
+    torch.cuda.synchronize()
+    assert torch.equal(y_inf, y_train)
+    assert torch.equal(y_inf2, y_train)
+
+
 def test_network_factorised_and_gathered_edge_forward_agree(gpu_lib):
     g, _ = radius_graph(2000, seed=4)
     rng = np.random.default_rng(5)
