@@ -280,6 +280,7 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(BuildArgs a) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const bool active = p < a.n && !a.flags->cell_overflow;
   int cnt = 0;
+  long off = p; // a.nbr offset of the row's next entry: cnt * ld + p
   int rs[9], re[9];
   double4 xi = make_double4(0., 0., 0., 0.);
 #pragma unroll
@@ -328,26 +329,30 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(BuildArgs a) {
     const int lo = wave_min_int(has ? s : INT_MAX);
     const int hi = wave_max_int(has ? e : INT_MIN);
     if (lo == INT_MAX) break; // no lane has a q-th range, nor a later one
-    const bool staged = hi - lo <= kNbrStage; // the same for the whole wave
+    // the same for the whole wave; readfirstlane lets the compiler branch on it
+    const bool staged = __builtin_amdgcn_readfirstlane(hi - lo <= kNbrStage ? 1 : 0) != 0;
+    auto test = [&](int j, const double4& xj) {
+      const double d0 = xi.x - xj.x;
+      const double d1 = xi.y - xj.y;
+      const double d2 = xi.z - xj.z;
+      const double dsqr = d0 * d0 + d1 * d1 + d2 * d2;
+      if (dsqr <= a.rc2) {
+        if (cnt < a.max_n) a.nbr[off] = j;
+        ++cnt;
+        off += a.ld;
+      }
+    };
     if (staged) {
       for (int t = lane; t < hi - lo; t += 64) sx[w][t] = a.xr_out[lo + t];
       wave_lds_sync();
+      if (has)
+        for (int j = s; j < e; ++j)
+          if (j != p) test(j, sx[w][j - lo]);
+      wave_lds_sync(); // the next round rewrites the tile
+    } else if (has) {
+      for (int j = s; j < e; ++j)
+        if (j != p) test(j, a.xr_out[j]);
     }
-    if (has) {
-      for (int j = s; j < e; ++j) {
-        if (j == p) continue;
-        const double4 xj = staged ? sx[w][j - lo] : a.xr_out[j];
-        const double d0 = xi.x - xj.x;
-        const double d1 = xi.y - xj.y;
-        const double d2 = xi.z - xj.z;
-        const double dsqr = d0 * d0 + d1 * d1 + d2 * d2;
-        if (dsqr <= a.rc2) {
-          if (cnt < a.max_n) a.nbr[(long)cnt * a.ld + p] = j;
-          ++cnt;
-        }
-      }
-    }
-    if (staged) wave_lds_sync(); // the next round rewrites the tile
   }
   if (active) a.nn[p] = cnt < a.max_n ? cnt : a.max_n;
   // one global atomic per wave, and only when it can raise the maximum
