@@ -43,21 +43,45 @@ __global__ __launch_bounds__(kBlock) void k_fill_int(int* p, long count, int val
   for (; i < count; i += stride) p[i] = value;
 }
 
+// Per-cell arrival ranks with one atomic per run of equal keys in a wave:
+// the slots are in the previous rebuild's cell order, so a wave's keys come
+// in runs.  The run's first lane reserves the run's length and the others
+// take consecutive ranks after it (any ranks that number each cell's
+// particles 0 .. count - 1 will do: k_cell_order fixes the order).
 __global__ __launch_bounds__(kBlock) void k_cell_keys(BuildArgs a) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= a.n) return;
-  const double4 p = a.xr_in[i];
-  if (!(isfinite(p.x) && isfinite(p.y) && isfinite(p.z))) atomicOr(&a.flags->nonfinite, 1);
-  // ensembles: the slot's system gives the grid and the first key
-  const int sys = a.sysid ? a.sysid[i] : 0;
-  const Grid& g = a.sysid ? a.grids[sys] : a.grid;
-  const long base = a.sysid ? a.kbase[sys] : 0;
-  const int cx = cell_coord(p.x, g, 0);
-  const int cy = cell_coord(p.y, g, 1);
-  const int cz = cell_coord(p.z, g, 2);
-  const int key = (int)(base + cell_key(g, cx, cy, cz));
+  const int lane = threadIdx.x & 63;
+  const bool active = i < a.n; // the active lanes of a wave are a prefix
+  int key = -1;
+  double4 p = make_double4(0., 0., 0., 0.);
+  if (active) {
+    p = a.xr_in[i];
+    if (!(isfinite(p.x) && isfinite(p.y) && isfinite(p.z))) atomicOr(&a.flags->nonfinite, 1);
+    // ensembles: the slot's system gives the grid and the first key
+    const int sys = a.sysid ? a.sysid[i] : 0;
+    const Grid& g = a.sysid ? a.grids[sys] : a.grid;
+    const long base = a.sysid ? a.kbase[sys] : 0;
+    const int cx = cell_coord(p.x, g, 0);
+    const int cy = cell_coord(p.y, g, 1);
+    const int cz = cell_coord(p.z, g, 2);
+    key = (int)(base + cell_key(g, cx, cy, cz));
+  }
+  const int prev = __shfl_up(key, 1);
+  const bool head = active && (lane == 0 || prev != key);
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long below = lane == 63 ? ~0ull : ((2ull << lane) - 1); // lanes 0 .. lane
+  const int hl = 63 - __clzll(heads & below); // this lane's run head (active lanes)
+  const int nactive = __popcll(__ballot(active)); // with every lane of the wave executing
+  int base_rank = 0;
+  if (head) {
+    const unsigned long long above = heads & ~below;
+    const int next = above ? __ffsll((long long)above) - 1 : nactive;
+    base_rank = atomicAdd(&a.cell_count[key], next - lane);
+  }
+  base_rank = __shfl(base_rank, hl >= 0 ? hl : 0);
+  if (!active) return;
   a.key[i] = key;
-  a.rank[i] = atomicAdd(&a.cell_count[key], 1);
+  a.rank[i] = base_rank + (lane - hl);
   if (a.check_skin) {
     const double d0 = p.x - a.xb[3 * i + 0];
     const double d1 = p.y - a.xb[3 * i + 1];
