@@ -330,6 +330,11 @@ def run_surrogate(args):
     ro = S.Rollout(net, wl, sc, check_every=10, tol=0.02, tune_dem=True)
     res = ro.run(n_macro)
     x_sur = ro.x.cpu().numpy()
+    ro.dem.close()
+    # the same judged roll-out without the network (a = g), for its errors
+    ro0 = S.Rollout(net, wl, sc, check_every=10, tol=0.02, tune_dem=True, no_network=True)
+    res0 = ro0.run(n_macro)
+    ro0.dem.close()
 
     def dem_reference(tuned):
         ref = wl.system()
@@ -380,6 +385,10 @@ def run_surrogate(args):
                         dem_ms_per_macro_workload_cadence=dem_ms_fixed,
                         speedup_vs_workload_cadence=dem_ms_fixed * ro.check_every
                         / (res["surrogate_ms_per_macro"] * ro.check_every + res["dem_ms_per_macro"])),
+        "rollout_no_network": {k: res0[k] for k in (
+            "fallbacks", "mean_position_error_over_r", "max_position_error_over_r",
+            "mean_kinetic_energy_error_rel", "max_kinetic_energy_error_rel",
+            "mean_mechanical_energy_error_rel")},
         "one_step_mse_held_out": {"untrained": before, "trained": after, "no_network": baseline},
         "mfma": {"forward_ms": fwd_ms, "nodes": g.N, "edges": g.E, "flops": flops,
                  "achieved_tflops": mfma_tf, "peak_tflops": 2500.0, "frac": mfma_tf / 2500.0},

@@ -639,13 +639,16 @@ class Rollout:
     (kinetic + gravitational) energy errors are recorded and, above `tol`
     in position, the run continues from the DEM state.  tune_dem: the judge's
     DEM context runs the autotuner's configuration (rebuild cadence, launch
-    shape), picked from the warm-up state."""
+    shape), picked from the warm-up state.  no_network: the method without
+    the network (a = g over every macro-step), judged the same way."""
 
-    def __init__(self, net: Network, wl, scales: Scales, check_every=10, tol=0.02, tune_dem=False):
+    def __init__(self, net: Network, wl, scales: Scales, check_every=10, tol=0.02, tune_dem=False,
+                 no_network=False):
         torch = _torch()
         self.net, self.wl, self.sc = net, wl, scales
         self.check_every, self.tol = check_every, tol
         self.tune_dem, self.dem_tune = tune_dem, None
+        self.no_network = no_network
         n = wl.state.n
         self.n = n
         self.dem = wl.system()
@@ -701,6 +704,11 @@ class Rollout:
 
     def surrogate_step(self):
         torch = _torch()
+        if self.no_network:
+            out = torch.zeros((self.n, 3), dtype=torch.float32, device="cuda")
+            _ok(glib().gnn_integrate(self.n, ptr(out), self.sc.a, ptr(self.g_dev), self.Dt, ptr(self.x),
+                                     ptr(self.vhist), HIST, stream()), "integrate")
+            return
         g = self.builder.build(self.x)
         if self.xe is None or self.xe.shape[0] < max(g.E, 1):
             self.xe = torch.empty((max(g.E, 1) * 5 // 4, FEAT), dtype=torch.bfloat16, device="cuda")
