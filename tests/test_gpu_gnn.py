@@ -462,3 +462,11 @@ def test_judged_rollout_records_error_and_falls_back(gpu_lib):
     assert res["max_kinetic_energy_error_rel"] >= res["mean_kinetic_energy_error_rel"] >= 0.
     assert res["fallbacks"] == 3
     assert res["surrogate_ms_per_macro"] > 0 and res["dem_ms_per_macro"] > 0
+    ro.dem.close()
+    # the method without the network (a = g), judged the same way, with the
+    # judge's DEM on the autotuner's configuration
+    ro0 = S.Rollout(net, wl, sc, check_every=3, tol=0.02, tune_dem=True, no_network=True)
+    res0 = ro0.run(6)
+    assert res0["checks"] == 2 and ro0.dem_tune is not None
+    assert np.isfinite(res0["mean_position_error_over_r"]) and res0["mean_position_error_over_r"] > 0.
+    ro0.dem.close()
