@@ -416,8 +416,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(the design for sharding is described in DESIGN.md).", file=sys.stderr)
+        print("bench.py runs on one GPU only: the library has no multi-GPU path "
+              "(slabs along z with ghost layers are the planned sharding).", file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "bonded_block":
