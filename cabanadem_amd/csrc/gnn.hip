@@ -451,14 +451,15 @@ struct EdgeFwdArgs {
 };
 
 // A wave tile of 16 EB edges: their rows, receivers, and the e^T (or, with
-// the encoder fused in, xe^T) and h[send]^T fragments.
+// the encoder fused in, xe^T), h[send]^T and (receiver term in the kernel)
+// h[recv]^T fragments.
 template <int EB>
 struct EdgeTile {
   int ed[EB], ri[EB];
-  bf16x8 Be[EB][4], Bh[EB][4], Bx[EB];
+  bf16x8 Be[EB][4], Bh[EB][4], Br[EB][4], Bx[EB];
 };
 
-template <int EB, bool ENC>
+template <int EB, bool ENC, bool PIN>
 __device__ inline void edge_tile_load(const EdgeFwdArgs& a, int e0, int lr, int lk, EdgeTile<EB>& T) {
 #pragma unroll
   for (int eb = 0; eb < EB; ++eb) {
@@ -471,6 +472,7 @@ __device__ inline void edge_tile_load(const EdgeFwdArgs& a, int e0, int lr, int 
     for (int ks = 0; ks < 4; ++ks) {
       if (!ENC) T.Be[eb][ks] = ld8(a.e + (long)T.ed[eb] * kLat + ks * 32 + lk);
       T.Bh[eb][ks] = ld8(a.h + (long)sj * kLat + ks * 32 + lk);
+      if (PIN) T.Br[eb][ks] = ld8(a.h + (long)T.ri[eb] * kLat + ks * 32 + lk);
     }
   }
 }
@@ -512,19 +514,25 @@ __device__ inline bf16x8 pack8(f32x4 lo, f32x4 hi) {
 // eighth of the tiles, so the rows its edges gather stay in that XCD's L2.
 // ENC: the edge encoder runs first on each tile (its weights in LDS too),
 // in the dense kernel's arithmetic, so e is never stored or read.
-template <int EB, int WAVES, bool ENC>
+// PIN: the receiver term h[recv] W1b is formed on the MFMA units from the
+// gathered row h[recv] (W1b in LDS too) instead of read from P, so P is
+// neither stored by the node pass nor read here.
+template <int EB, int WAVES, bool ENC, bool PIN>
 __global__ __launch_bounds__(64 * WAVES, 1) void k_edge_fwd(EdgeFwdArgs a) {
+  static_assert(!(ENC && PIN), "the encoder's weights and W1b do not fit in LDS together");
   constexpr int TE = 16 * EB;
   __shared__ bf16 swa[kLat][kWPitch]; // W1a^T: [row][k]
   __shared__ bf16 swc[kLat][kWPitch]; // W1c^T: [row][k]
   __shared__ bf16 sw2[kLat][kWPitch]; // W2^T:  [row][k]
   __shared__ bf16 se1[ENC ? kLat : 1][32 + 8];   // encoder W1^T: [row][k]
   __shared__ bf16 se2[ENC ? kLat : 1][kWPitch];  // encoder W2^T: [row][k]
+  __shared__ bf16 swb[PIN ? kLat : 1][kWPitch];  // W1b^T: [row][k]
   for (int c = threadIdx.x; c < kLat * 16; c += 64 * WAVES) {
     const int r = c >> 4, k8 = (c & 15) * 8, u = unit_of_row(r);
     *reinterpret_cast<bf16x8*>(&swa[r][k8]) = ld8(a.W1t + (long)u * (3 * kLat) + k8);
     *reinterpret_cast<bf16x8*>(&swc[r][k8]) = ld8(a.W1t + (long)u * (3 * kLat) + 2 * kLat + k8);
     *reinterpret_cast<bf16x8*>(&sw2[r][k8]) = ld8(a.W2t + (long)u * kLat + k8);
+    if (PIN) *reinterpret_cast<bf16x8*>(&swb[r][k8]) = ld8(a.W1t + (long)u * (3 * kLat) + kLat + k8);
     if (ENC) {
       *reinterpret_cast<bf16x8*>(&se2[r][k8]) = ld8(a.We2t + (long)u * kLat + k8);
       if (k8 < 32) *reinterpret_cast<bf16x8*>(&se1[r][k8]) = ld8(a.We1t + (long)u * 32 + k8);
@@ -548,11 +556,11 @@ __global__ __launch_bounds__(64 * WAVES, 1) void k_edge_fwd(EdgeFwdArgs a) {
   }
   if (t >= t_end) return;
   EdgeTile<EB> cur, nxt;
-  edge_tile_load<EB, ENC>(a, t * TE, lr, lk, cur);
+  edge_tile_load<EB, ENC, PIN>(a, t * TE, lr, lk, cur);
   for (; t < t_end; t += stride) {
     const int e0 = t * TE;
     const int tn = t + stride;
-    if (tn < t_end) edge_tile_load<EB, ENC>(a, tn * TE, lr, lk, nxt);
+    if (tn < t_end) edge_tile_load<EB, ENC, PIN>(a, tn * TE, lr, lk, nxt);
     // keeps the weight fragments' LDS reads inside the loop: hoisted out of
     // it as loop-invariant they would need 384 registers
     __asm__ volatile("" ::: "memory");
@@ -592,8 +600,12 @@ __global__ __launch_bounds__(64 * WAVES, 1) void k_edge_fwd(EdgeFwdArgs a) {
       const float4 bb = *reinterpret_cast<const float4*>(a.b1 + u);
 #pragma unroll
       for (int eb = 0; eb < EB; ++eb) {
-        const float4 p = *reinterpret_cast<const float4*>(a.P + (long)cur.ri[eb] * kLat + u);
-        acc[ub][eb] = f32x4{bb.x + p.x, bb.y + p.y, bb.z + p.z, bb.w + p.w};
+        if (PIN) {
+          acc[ub][eb] = f32x4{bb.x, bb.y, bb.z, bb.w};
+        } else {
+          const float4 p = *reinterpret_cast<const float4*>(a.P + (long)cur.ri[eb] * kLat + u);
+          acc[ub][eb] = f32x4{bb.x + p.x, bb.y + p.y, bb.z + p.z, bb.w + p.w};
+        }
       }
     }
 #pragma unroll
@@ -606,6 +618,11 @@ __global__ __launch_bounds__(64 * WAVES, 1) void k_edge_fwd(EdgeFwdArgs a) {
         for (int eb = 0; eb < EB; ++eb) {
           acc[ub][eb] = mfma(Aa, cur.Be[eb][ks], acc[ub][eb]);
           acc[ub][eb] = mfma(Ac, cur.Bh[eb][ks], acc[ub][eb]);
+        }
+        if (PIN) {
+          const bf16x8 Ab = *reinterpret_cast<const bf16x8*>(&swb[ub * 16 + lr][ks * 32 + lk]);
+#pragma unroll
+          for (int eb = 0; eb < EB; ++eb) acc[ub][eb] = mfma(Ab, cur.Br[eb][ks], acc[ub][eb]);
         }
       }
     // relu -> BF16: the hidden^T fragment of k step s is blocks 2 s, 2 s + 1
@@ -1566,7 +1583,7 @@ int check_io(const gnn_mlp_io* io) {
 }
 
 // one block per CU (the weights in LDS, shared by its waves), persistent
-template <bool ENC>
+template <bool ENC, bool PIN>
 int launch_edge(const EdgeFwdArgs& a, int grid_cap, void* stream) {
   const int cus = device_cus();
   if (cus <= 0) return -2;
@@ -1574,7 +1591,7 @@ int launch_edge(const EdgeFwdArgs& a, int grid_cap, void* stream) {
   const long nblocks = ((long)a.E + per_block - 1) / per_block;
   int grid = grid_cap > 0 ? grid_cap : cus;
   if (grid > nblocks) grid = (int)nblocks;
-  hipLaunchKernelGGL((k_edge_fwd<kEdgeEB, kEdgeWaves, ENC>), dim3(grid), dim3(64 * kEdgeWaves), 0,
+  hipLaunchKernelGGL((k_edge_fwd<kEdgeEB, kEdgeWaves, ENC, PIN>), dim3(grid), dim3(64 * kEdgeWaves), 0,
                      reinterpret_cast<hipStream_t>(stream), a);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
@@ -1765,7 +1782,7 @@ int gnn_edge_mlp_forward(long E, const void* e, const int* recv, const int* send
                          void* stream) {
   if (E < 0 || E > 0x7fffffffl || !edge_weights_ok(wt)) return -1;
   if (E == 0) return 0;
-  if (!e || !recv || !send || !P || !h || !out) return -1;
+  if (!e || !recv || !send || !h || !out) return -1;
   EdgeFwdArgs a{};
   a.E = (int)E;
   a.e = reinterpret_cast<const bf16*>(e);
@@ -1779,7 +1796,7 @@ int gnn_edge_mlp_forward(long E, const void* e, const int* recv, const int* send
   a.b2 = wt->b2;
   a.hid = reinterpret_cast<bf16*>(hidden);
   a.out = reinterpret_cast<bf16*>(out);
-  return launch_edge<false>(a, grid_cap, stream);
+  return P ? launch_edge<false, false>(a, grid_cap, stream) : launch_edge<false, true>(a, grid_cap, stream);
 }
 
 int gnn_edge_mlp_forward_encoded(long E, const void* xe, const gnn_mlp_weights* enc, const int* recv,
@@ -1806,7 +1823,7 @@ int gnn_edge_mlp_forward_encoded(long E, const void* xe, const gnn_mlp_weights* 
   a.be1 = enc->b1;
   a.We2t = reinterpret_cast<const bf16*>(enc->W2t);
   a.be2 = enc->b2;
-  return launch_edge<true>(a, grid_cap, stream);
+  return launch_edge<true, false>(a, grid_cap, stream);
 }
 
 int gnn_node_layer_forward(int N, const void* h, const void* e, const int* row, const gnn_mlp_weights* wt,

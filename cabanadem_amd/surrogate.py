@@ -260,15 +260,18 @@ def io(M, *segs):
 
 
 class Network:
-    def __init__(self, n_layers=10, seed=0, device="cuda", residual_scale=0.1, factorised=True):
-        """factorised: run the edge MLPs' forward through gnn_node_p +
-        gnn_edge_mlp_forward (the receiver's share of the first layer
-        precomputed per node) rather
-        than gnn_mlp_forward over the three gathered sources; the same
-        function up to FP32 summation order."""
+    def __init__(self, n_layers=10, seed=0, device="cuda", residual_scale=0.1, factorised=True,
+                 receiver_in_kernel=True):
+        """factorised: run the edge MLPs' forward through gnn_edge_mlp_forward
+        (first layer split into e W1a + h[recv] W1b + h[send] W1c) rather than
+        gnn_mlp_forward over the three gathered sources; the same function up
+        to FP32 summation order.  receiver_in_kernel: layers after the first
+        form h[recv] W1b in the edge kernel; otherwise the node pass stores
+        P = h W1b per node for the next layer's edge kernel to read."""
         torch = _torch()
         self.L = n_layers
         self.factorised = factorised
+        self.receiver_in_kernel = receiver_in_kernel
         self.dev = device
         off = 0
         self.enc_n = MLP(FEAT, LAT, off); off = self.enc_n.end
@@ -389,21 +392,26 @@ class Network:
             fwd(self.enc_e, io(E, (xe, None)), None, self.hid_enc_e, self.e[0], None)
         if E > 0 and self.factorised:
             # per layer: the edge MLP, then the node side in one pass (the
-            # aggregation, the node MLP and the next layer's P = h W1b)
+            # aggregation, the node MLP and, without receiver_in_kernel, the
+            # next layer's P = h W1b).  Layer 0 reads P from gnn_node_p (its
+            # kernel holds the encoder's weights instead of W1b); the later
+            # layers form h[recv] W1b in the edge kernel unless told not to.
             _ok(Lb.gnn_node_p(N, ptr(self.h[0]), C.byref(self.edge[0].weights(F)), ptr(self.P), s), "node P")
             for l in range(self.L):
                 h, e = self.h[l], self.e[l]
+                P = self.P if (l == 0 or not self.receiver_in_kernel) else None
                 if l == 0 and fuse_enc:
                     _ok(Lb.gnn_edge_mlp_forward_encoded(E, ptr(xe), C.byref(self.enc_e.weights(F)), ptr(g.recv),
                                                         ptr(g.send), ptr(self.P), ptr(h),
                                                         C.byref(self.edge[0].weights(F)), ptr(self.e[1]), 0, s),
                         "encoded edge MLP forward")
                 else:
-                    _ok(Lb.gnn_edge_mlp_forward(E, ptr(e), ptr(g.recv), ptr(g.send), ptr(self.P), ptr(h),
+                    _ok(Lb.gnn_edge_mlp_forward(E, ptr(e), ptr(g.recv), ptr(g.send), ptr(P), ptr(h),
                                                 C.byref(self.edge[l].weights(F)),
                                                 ptr(self.hid_e[l]) if train else None, ptr(self.e[l + 1]), 0, s),
                         "edge MLP forward")
-                nxt = C.byref(self.edge[l + 1].weights(F)) if l + 1 < self.L else None
+                nxt = (C.byref(self.edge[l + 1].weights(F))
+                       if l + 1 < self.L and not self.receiver_in_kernel else None)
                 _ok(Lb.gnn_node_layer_forward(N, ptr(h), ptr(self.e[l + 1]), ptr(g.row),
                                               C.byref(self.node[l].weights(F)),
                                               ptr(self.agg[l]) if train else None,

@@ -116,9 +116,11 @@ int gnn_mse(const float* y, const float* t, long count, float* dY, float* part, 
  * gnn_node_p: P (float [N][128]) = h W1b for the node latents h (bf16 [N][128]).
  * gnn_edge_mlp_forward: out[e] = e[e] + W2 relu(e[e] W1a + P[recv[e]]
  * + h[send[e]] W1c + b1) + b2 (bf16 [E][128]); hidden (bf16 [E][128],
- * optional) as in gnn_mlp_forward.  The same function as gnn_mlp_forward
- * over the three sources, up to FP32 summation order.  grid_cap > 0 limits
- * the persistent workgroups (default one per CU). */
+ * optional) as in gnn_mlp_forward.  P may be null: the kernel then forms
+ * h[recv[e]] W1b itself from the gathered row, and P need not exist.  The
+ * same function as gnn_mlp_forward over the three sources, up to FP32
+ * summation order.  grid_cap > 0 limits the persistent workgroups (default
+ * one per CU). */
 int gnn_node_p(int N, const void* h, const gnn_mlp_weights* w, float* P, void* stream);
 int gnn_edge_mlp_forward(long E, const void* e, const int* recv, const int* send, const float* P,
                          const void* h, const gnn_mlp_weights* w, void* hidden, void* out,

@@ -118,6 +118,18 @@ def test_factorised_edge_mlp_matches_gathered_form(gpu_lib, M):
                                    S.ptr(out2), 3, s) == 0
     torch.cuda.synchronize()
     assert torch.equal(out2, out)
+    # P = null: the kernel forms h[recv] W1b on the MFMA units itself
+    outr, hidr = torch.empty_like(out), torch.empty_like(hid)
+    assert Lb.gnn_edge_mlp_forward(M, S.ptr(ec), S.ptr(iac), S.ptr(ibc), None, S.ptr(hc), w, S.ptr(hidr),
+                                   S.ptr(outr), 0, s) == 0
+    outr3 = torch.empty_like(out)
+    assert Lb.gnn_edge_mlp_forward(M, S.ptr(ec), S.ptr(iac), S.ptr(ibc), None, S.ptr(hc), w, None,
+                                   S.ptr(outr3), 3, s) == 0
+    torch.cuda.synchronize()
+    assert rel(outr.float().cpu(), out_g.float().cpu()) < 5e-3
+    assert rel(hidr.float().cpu(), hid_g.float().cpu()) < 5e-3
+    assert rel(outr.float().cpu(), y) < 1e-2
+    assert torch.equal(outr3, outr)
 
 
 @pytest.mark.parametrize("N,last", [(1, False), (77, True), (5000, False), (64001, False)])
@@ -367,9 +379,13 @@ def test_network_factorised_and_gathered_edge_forward_agree(gpu_lib):
     xn_d, xe_d = bf(xn).cuda(), bf(xe).cuda()
     a = S.Network(n_layers=4, seed=6, residual_scale=0.5, factorised=True)
     b = S.Network(n_layers=4, seed=6, residual_scale=0.5, factorised=False)
+    c = S.Network(n_layers=4, seed=6, residual_scale=0.5, factorised=True, receiver_in_kernel=False)
     ya = a.forward(g, xn_d, xe_d).cpu().numpy()
     yb = b.forward(g, xn_d, xe_d).cpu().numpy()
+    yc = c.forward(g, xn_d, xe_d).cpu().numpy()
     assert rel(ya, yb) < 1e-2
+    assert rel(yc, yb) < 1e-2
+    assert rel(ya, yc) < 1e-2
 
 
 def test_network_forward_backward_match_oracle(gpu_lib):
