@@ -253,17 +253,20 @@ def run_ensemble(args):
     print(json.dumps(out), flush=True)
 
 
-def forward_flops(L, N, E):
+def forward_flops(L, N, E, receiver_in_kernel=True):
     """FLOPs the forward pass executes on the MFMA units (2 per MAC): per
     layer, each edge multiplies e and h[send] by W1a and W1c and its hidden
-    layer by W2 (3 x 128^2), each node forms P = h W1b (128^2) and runs its
-    node MLP ((256 + 128) x 128); plus the encoders and the decoder (output
-    padded to 16)."""
+    layer by W2 (3 x 128^2) and each node runs its node MLP ((256 + 128) x
+    128); the receiver term h W1b (128^2) is formed once per node for the
+    first layer and, with receiver_in_kernel, once per edge for the others;
+    plus the encoders and the decoder (output padded to 16).  With
+    receiver_in_kernel off this is the network's minimal count."""
     lat = 128
     enc = N * (32 * lat + lat * lat) + E * (32 * lat + lat * lat)
-    layers = L * (E * 3 * lat * lat + N * (lat * lat + 2 * lat * lat + lat * lat))
+    layers = L * (E * 3 * lat * lat + N * 3 * lat * lat)
+    recv = N * lat * lat + (L - 1) * (E if receiver_in_kernel else N) * lat * lat
     dec = N * (lat * lat + lat * 16)
-    return 2.0 * (enc + layers + dec)
+    return 2.0 * (enc + layers + recv + dec)
 
 
 def run_surrogate(args):
@@ -318,7 +321,7 @@ def run_surrogate(args):
     e1.record()
     torch.cuda.synchronize()
     fwd_ms = e0.elapsed_time(e1) / reps
-    flops = forward_flops(10, g.N, g.E)
+    flops = forward_flops(10, g.N, g.E, net.receiver_in_kernel)
     mfma_tf = flops / (fwd_ms * 1e-3) / 1e12
     ro.dem.close()
 
@@ -391,6 +394,7 @@ def run_surrogate(args):
             "mean_mechanical_energy_error_rel")},
         "one_step_mse_held_out": {"untrained": before, "trained": after, "no_network": baseline},
         "mfma": {"forward_ms": fwd_ms, "nodes": g.N, "edges": g.E, "flops": flops,
+                 "flops_minimal": forward_flops(10, g.N, g.E, False),
                  "achieved_tflops": mfma_tf, "peak_tflops": 2500.0, "frac": mfma_tf / 2500.0},
         "times_s": {"data_generation": t_gen, "training": t_train},
     }
