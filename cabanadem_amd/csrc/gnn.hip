@@ -592,7 +592,8 @@ __global__ __launch_bounds__(64 * WAVES, 1) void k_edge_fwd(EdgeFwdArgs a) {
         for (int s4 = 0; s4 < 4; ++s4) cur.Be[eb][s4] = pack8<false>(q[2 * s4], q[2 * s4 + 1]);
       }
     }
-    // layer 1 on top of b1 + P[recv]: block 2 s + t holds units 32 s + 8 lg + 4 t .. + 3
+    // layer 1 on top of b1 + P[recv] (PIN: b1 alone, the receiver term comes
+    // from h[recv] below): block 2 s + t holds units 32 s + 8 lg + 4 t .. + 3
     f32x4 acc[8][EB];
 #pragma unroll
     for (int ub = 0; ub < 8; ++ub) {
@@ -677,7 +678,8 @@ __global__ __launch_bounds__(64 * WAVES, 1) void k_edge_fwd(EdgeFwdArgs a) {
 }
 
 // Launch shape of the edge kernel: 16 EB edges per wave tile, WAVES waves per
-// block (one block per CU: the three 34 KB weight tiles fill the LDS).
+// block (one block per CU: three 34 KB weight tiles, plus W1b or the
+// encoder's weights, fill the LDS).
 #ifndef GNN_EDGE_EB
 #define GNN_EDGE_EB 1
 #endif
@@ -826,7 +828,7 @@ struct NodeFwdArgs {
 // Launch shape of the node pass: WAVES waves per block (one block per CU:
 // the weights fill 137 KB of LDS) and AGG edge rows in flight per lane in the
 // aggregation.  Four waves per SIMD with two rows in flight measured best at
-// 64k nodes (34 us; two waves with six rows in flight 35 us).
+// 64k nodes (two waves with six rows in flight were 4 % slower).
 #ifndef GNN_NODE_WAVES
 #define GNN_NODE_WAVES 16
 #endif
