@@ -1246,9 +1246,10 @@ struct CellGrid {
 
 constexpr int kScanPerBlock = 4 * kThreads; // elements per block of the scan
 
-// The count sorts each node's senders in LDS (one row per thread) and keeps
-// rows of up to kFillLds of them in the workspace, so the fill copies them;
-// longer rows are found again by the fill and sorted in place in send.
+// The count sorts each node's senders in LDS (one row per thread, threads in
+// cell-slot order) and keeps rows of up to kFillLds of them in the workspace,
+// so the fill copies them; longer rows are found again by the fill and
+// sorted in place in send.
 constexpr int kFillLds = 32;
 
 __device__ inline int cell_coord(double x, double lo, double inv, int nc) {
@@ -1371,16 +1372,17 @@ __device__ inline void for_each_neighbor(int i, double4 xi, int c, const CellGri
     }
 }
 
-__global__ __launch_bounds__(kThreads) void k_g_count(int N, const double* x, CellGrid g, double r2,
-                                                      const int* cell_of, const int* cstart, const int* cnodes,
-                                                      const double4* cpos, int* deg, int* rows) {
+__global__ __launch_bounds__(kThreads) void k_g_count(int N, CellGrid g, double r2, const int* cell_of,
+                                                      const int* cstart, const int* cnodes, const double4* cpos,
+                                                      int* deg, int* rows) {
   __shared__ int sb[kThreads * (kFillLds + 1)]; // thread t's row at t (kFillLds + 1): no bank conflicts
-  const int i = blockIdx.x * kThreads + threadIdx.x;
-  if (i >= N) return;
+  const int t = blockIdx.x * kThreads + threadIdx.x; // cell slot: a wave's nodes share cells
+  if (t >= N) return;
+  const int i = cnodes[t];
+  const double4 xi = cpos[t];
   int* b = sb + threadIdx.x * (kFillLds + 1);
   int n = 0;
-  for_each_neighbor(i, make_double4(x[3l * i], x[3l * i + 1], x[3l * i + 2], 0.), cell_of[i], g, r2, cstart,
-                    cnodes, cpos, [&](int j) {
+  for_each_neighbor(i, xi, cell_of[i], g, r2, cstart, cnodes, cpos, [&](int j) {
                       if (n < kFillLds) { // insertion into the sorted prefix b[0..n)
                         int p = n;
                         while (p > 0 && b[p - 1] > j) {
@@ -1879,7 +1881,7 @@ int gnn_radius_graph_count(int N, const double* x, double radius, const double l
     scan(w.ccount, g.ncell, w.cstart, w.bsum, nullptr, s);
     hipLaunchKernelGGL(k_g_scatter, dim3(nblk(N)), dim3(kThreads), 0, s, N, x, w.cell_of, w.cstart,
                        w.cursor, w.cnodes, w.cpos);
-    hipLaunchKernelGGL(k_g_count, dim3(nblk(N)), dim3(kThreads), 0, s, N, x, g, radius * radius, w.cell_of,
+    hipLaunchKernelGGL(k_g_count, dim3(nblk(N)), dim3(kThreads), 0, s, N, g, radius * radius, w.cell_of,
                        w.cstart, w.cnodes, w.cpos, w.deg, w.rows);
   }
   scan(w.deg, N, row, w.bsum, w.total64, s);
