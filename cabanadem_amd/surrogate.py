@@ -308,6 +308,7 @@ class Network:
             m.alloc(device)
         self.cast()
         self.cap_N = self.cap_E = 0
+        self.h = self.e = None
         self.part = self.bpart = None
         self.nchunks = 64
 
@@ -338,53 +339,78 @@ class Network:
 
     # -- buffers -----------------------------------------------------------
     def _ensure(self, N, E, train):
-        """Buffers for N nodes and E edges.  They grow with 25 % headroom, since
-        the graph's edge count drifts from step to step; the hidden layers and
-        the gradient buffers exist only once the network has trained."""
+        """Buffers for N nodes and E edges, in two groups that grow separately
+        with 25 % headroom (the graph's edge count drifts from step to step,
+        and the node side of a forward runs before the graph exists); the
+        hidden layers and the gradient buffers exist only once the network
+        has trained."""
         torch = _torch()
-        if N <= self.cap_N and E <= self.cap_E and (not train or self.part is not None):
+        first_train = train and self.part is None
+        grow_n, grow_e = N > self.cap_N or self.h is None, E > self.cap_E or self.e is None
+        if not (grow_n or grow_e or first_train):
             return
-        if N > self.cap_N or E > self.cap_E:
-            N, E = max(N * 5 // 4, self.cap_N), max(E * 5 // 4, self.cap_E)
-        else:
-            N, E = self.cap_N, self.cap_E
         train = train or self.part is not None
         bf, f32, dev = torch.bfloat16, torch.float32, self.dev
-        self.cap_N, self.cap_E = N, E
         L = self.L
         z = lambda *s, dt=bf: torch.zeros(s, dtype=dt, device=dev)
-        self.h = [z(N, LAT) for _ in range(L + 1)]
-        self.e = [z(E, LAT) for _ in range(L + 1)]
-        self.agg = [z(N, LAT) for _ in range(L)]
-        self.out = z(N, 3, dt=f32)
-        self.P = z(N, LAT, dt=f32)
-        self.hid_e, self.hid_n = [None] * L, [None] * L
-        self.hid_enc_n = self.hid_enc_e = self.hid_dec = None
-        if train:
-            self.hid_e = [z(E, LAT) for _ in range(L)]
-            self.hid_n = [z(N, LAT) for _ in range(L)]
-            self.hid_enc_n, self.hid_enc_e, self.hid_dec = z(N, LAT), z(E, LAT), z(N, LAT)
-            self.d_h = z(N, LAT, dt=f32)
-            self.d_agg = z(N, LAT, dt=f32)
-            self.d_e = z(E, LAT, dt=f32)
-            self.d_hi = z(E, LAT, dt=f32)
-            self.d_hj = z(E, LAT, dt=f32)
-            self.dH_e = z(E, LAT)
-            self.dH_n = z(N, LAT)
+        if grow_n or first_train:
+            N = max(N * 5 // 4, self.cap_N) if grow_n else self.cap_N
+            self.cap_N = N
+            self.h = [z(N, LAT) for _ in range(L + 1)]
+            self.agg = [z(N, LAT) for _ in range(L)]
+            self.out = z(N, 3, dt=f32)
+            self.P = z(N, LAT, dt=f32)
+            self.hid_n = [z(N, LAT) for _ in range(L)] if train else [None] * L
+            self.hid_enc_n = z(N, LAT) if train else None
+            self.hid_dec = z(N, LAT) if train else None
+            if train:
+                self.d_h = z(N, LAT, dt=f32)
+                self.d_agg = z(N, LAT, dt=f32)
+                self.dH_n = z(N, LAT)
+        if grow_e or first_train:
+            E = max(E * 5 // 4, self.cap_E) if grow_e else self.cap_E
+            self.cap_E = E
+            self.e = [z(E, LAT) for _ in range(L + 1)]
+            self.hid_e = [z(E, LAT) for _ in range(L)] if train else [None] * L
+            self.hid_enc_e = z(E, LAT) if train else None
+            if train:
+                self.d_e = z(E, LAT, dt=f32)
+                self.d_hi = z(E, LAT, dt=f32)
+                self.d_hj = z(E, LAT, dt=f32)
+                self.dH_e = z(E, LAT)
+        if train and self.part is None:
             self.part = z(self.nchunks * 3 * LAT * LAT, dt=f32)
             self.bpart = z(self.nchunks * LAT, dt=f32)
 
     # -- forward -----------------------------------------------------------
+    def _fwd(self, mlp, inp, res, hid, out, outf, train):
+        # the hidden layers are kept for the backward pass only
+        _ok(glib().gnn_mlp_forward(C.byref(inp), C.byref(mlp.weights(self.flat)), ptr(res),
+                                   ptr(hid) if train else None, ptr(out), ptr(outf), 0, stream()),
+            "mlp forward")
+
     def forward(self, g: Graph, xn, xe, train=False):
         """xn bf16 [N][32], xe bf16 [E][32] -> self.out float [N][3]."""
+        self.forward_nodes(g.N, xn, train)
+        return self.forward_edges(g, xe, train)
+
+    def forward_nodes(self, N, xn, train=False):
+        """The part of the forward that needs no graph: the node encoder and
+        layer 0's receiver term P = h W1b (factorised).  It can be queued
+        before the graph is built, so that it runs while the host waits for
+        the graph's edge count."""
+        self._ensure(N, 0, train)
+        self._fwd(self.enc_n, io(N, (xn, None)), None, self.hid_enc_n, self.h[0], None, train)
+        if self.factorised:
+            _ok(glib().gnn_node_p(N, ptr(self.h[0]), C.byref(self.edge[0].weights(self.flat)), ptr(self.P),
+                                  stream()), "node P")
+
+    def forward_edges(self, g: Graph, xe, train=False):
+        """The rest of the forward, after forward_nodes on the same N."""
         self._ensure(g.N, g.E, train)
         Lb, s, N, E = glib(), stream(), g.N, g.E
         F = self.flat
-        # the hidden layers are kept for the backward pass only
-        fwd = lambda mlp, inp, res, hid, out, outf: _ok(
-            Lb.gnn_mlp_forward(C.byref(inp), C.byref(mlp.weights(F)), ptr(res), ptr(hid) if train else None,
-                               ptr(out), ptr(outf), 0, s), "mlp forward")
-        fwd(self.enc_n, io(N, (xn, None)), None, self.hid_enc_n, self.h[0], None)
+        fwd = lambda mlp, inp, res, hid, out, outf: self._fwd(mlp, inp, res, hid, out, outf, train)
         # inference fuses the edge encoder into the first edge MLP (e[0] is
         # then never formed); training keeps it for the backward pass
         fuse_enc = E > 0 and self.factorised and not train
@@ -396,7 +422,6 @@ class Network:
             # next layer's P = h W1b).  Layer 0 reads P from gnn_node_p (its
             # kernel holds the encoder's weights instead of W1b); the later
             # layers form h[recv] W1b in the edge kernel unless told not to.
-            _ok(Lb.gnn_node_p(N, ptr(self.h[0]), C.byref(self.edge[0].weights(F)), ptr(self.P), s), "node P")
             for l in range(self.L):
                 h, e = self.h[l], self.e[l]
                 P = self.P if (l == 0 or not self.receiver_in_kernel) else None
@@ -538,13 +563,23 @@ class Scales:
     radius: float   # graph radius
 
 
+def node_features(N, x_dev, vhist_dev, rad_dev, planes_dev, n_planes, sc: Scales, xn):
+    """Node features on the device (bf16 [N][32]); they need no graph."""
+    _ok(glib().gnn_node_features(N, ptr(x_dev), ptr(vhist_dev), HIST, ptr(rad_dev), sc.v, sc.r_mean,
+                                 sc.r_std, ptr(planes_dev), n_planes, sc.radius, ptr(xn), stream()),
+        "node features")
+
+
+def edge_features(g: Graph, x_dev, sc: Scales, xe):
+    """Edge features on the device (bf16 [E][32])."""
+    _ok(glib().gnn_edge_features(g.E, ptr(x_dev), ptr(g.recv), ptr(g.send), sc.radius, ptr(xe), stream()),
+        "edge features")
+
+
 def features(g: Graph, x_dev, vhist_dev, rad_dev, planes_dev, n_planes, sc: Scales, xn, xe):
     """Node and edge features on the device (bf16 [N][32], [E][32])."""
-    L, s = glib(), stream()
-    _ok(L.gnn_node_features(g.N, ptr(x_dev), ptr(vhist_dev), HIST, ptr(rad_dev), sc.v, sc.r_mean,
-                            sc.r_std, ptr(planes_dev), n_planes, sc.radius, ptr(xn), s), "node features")
-    _ok(L.gnn_edge_features(g.E, ptr(x_dev), ptr(g.recv), ptr(g.send), sc.radius, ptr(xe), s),
-        "edge features")
+    node_features(g.N, x_dev, vhist_dev, rad_dev, planes_dev, n_planes, sc, xn)
+    edge_features(g, x_dev, sc, xe)
 
 
 @dataclass
@@ -717,11 +752,15 @@ class Rollout:
             _ok(glib().gnn_integrate(self.n, ptr(out), self.sc.a, ptr(self.g_dev), self.Dt, ptr(self.x),
                                      ptr(self.vhist), HIST, stream()), "integrate")
             return
+        # the node side first: it needs no graph, so the GPU runs it while the
+        # host waits for the graph's edge count
+        node_features(self.n, self.x, self.vhist, self.rad, self.planes, self.n_planes, self.sc, self.xn)
+        self.net.forward_nodes(self.n, self.xn)
         g = self.builder.build(self.x)
         if self.xe is None or self.xe.shape[0] < max(g.E, 1):
             self.xe = torch.empty((max(g.E, 1) * 5 // 4, FEAT), dtype=torch.bfloat16, device="cuda")
-        features(g, self.x, self.vhist, self.rad, self.planes, self.n_planes, self.sc, self.xn, self.xe)
-        out = self.net.forward(g, self.xn, self.xe[:g.E])
+        edge_features(g, self.x, self.sc, self.xe)
+        out = self.net.forward_edges(g, self.xe[:g.E])
         _ok(glib().gnn_integrate(self.n, ptr(out), self.sc.a, ptr(self.g_dev), self.Dt, ptr(self.x),
                                  ptr(self.vhist), HIST, stream()), "integrate")
 
