@@ -1909,4 +1909,12 @@ int gnn_radius_graph_fill(int N, const double* x, double radius, const double lo
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
+int gnn_radius_graph_build(int N, const double* x, double radius, const double lo[3], const double hi[3],
+                           void* work, long work_bytes, int* row, long capacity, int* send, int* recv, int* rev,
+                           long* n_edges, void* stream) {
+  if (int r = gnn_radius_graph_count(N, x, radius, lo, hi, work, work_bytes, row, n_edges, stream)) return r;
+  if (*n_edges > capacity) return 1; // the caller grows the edge arrays and calls the fill
+  return gnn_radius_graph_fill(N, x, radius, lo, hi, work, work_bytes, row, *n_edges, send, recv, rev, stream);
+}
+
 } // extern "C"

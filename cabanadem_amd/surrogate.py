@@ -37,7 +37,7 @@ GNN_LIB_PATH = Path(os.environ.get("CABANADEM_GNN_LIB") or
 LAT = 128
 FEAT = 32
 HIST = 5
-GNN_ABI_VERSION = 4
+GNN_ABI_VERSION = 5
 
 _vp = C.c_void_p
 _ip = C.POINTER(C.c_int)
@@ -86,6 +86,8 @@ _SIG = {
                                          C.POINTER(C.c_long), _vp]),
     "gnn_radius_graph_fill": (C.c_int, [C.c_int, _vp, C.c_double, _dp, _dp, _vp, C.c_long, _vp, C.c_long,
                                         _vp, _vp, _vp, _vp]),
+    "gnn_radius_graph_build": (C.c_int, [C.c_int, _vp, C.c_double, _dp, _dp, _vp, C.c_long, _vp, C.c_long,
+                                         _vp, _vp, _vp, C.POINTER(C.c_long), _vp]),
 }
 
 _GLIB = None
@@ -190,14 +192,17 @@ class GraphBuilder:
             raise ValueError("radius graph: x must be a contiguous float64 [n][3] device tensor")
         L, s = glib(), stream()
         E = C.c_long(0)
-        _ok(L.gnn_radius_graph_count(self.n, ptr(x), self.radius, self.lo, self.hi, ptr(self.work), self.wbytes,
-                                     ptr(self.row), C.byref(E), s), "radius graph count")
+        # count and fill in one call while the edge arrays are large enough
+        rc = L.gnn_radius_graph_build(self.n, ptr(x), self.radius, self.lo, self.hi, ptr(self.work),
+                                      self.wbytes, ptr(self.row), self.cap, ptr(self.send), ptr(self.recv),
+                                      ptr(self.rev), C.byref(E), s)
+        if rc < 0:
+            _ok(rc, "radius graph build")
         E = int(E.value)
-        if E > self.cap:
+        if rc == 1:
             self.cap = max(E * 5 // 4, 64)
             z = lambda: torch.empty(self.cap, dtype=torch.int32, device="cuda")
             self.send, self.recv, self.rev = z(), z(), z()
-        if E > 0:
             _ok(L.gnn_radius_graph_fill(self.n, ptr(x), self.radius, self.lo, self.hi, ptr(self.work),
                                         self.wbytes, ptr(self.row), E, ptr(self.send), ptr(self.recv),
                                         ptr(self.rev), s), "radius graph fill")

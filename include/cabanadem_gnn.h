@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define GNN_ABI_VERSION 4
+#define GNN_ABI_VERSION 5
 #define GNN_LATENT 128
 
 /* Input rows of an MLP: the concatenation of up to three BF16 sources;
@@ -160,7 +160,11 @@ int gnn_node_layer_forward(int N, const void* h, const void* e, const int* row,
  * gnn_radius_graph_fill: with the same arguments and the workspace as left by
  * the count, fills send, recv and rev (int [E] each; rev[e] is the edge
  * recv[e] -> send[e]).  n_edges is the count's E (the capacity of the three
- * arrays). */
+ * arrays).
+ * gnn_radius_graph_build: the count and, when E <= capacity (the length of
+ * send, recv and rev), the fill in one call, so no host round trip sits
+ * between them; returns 1, with *n_edges set and the fill not run, when E
+ * exceeds capacity (grow the arrays, then call gnn_radius_graph_fill). */
 long gnn_radius_graph_workspace(int N, const double lo[3], const double hi[3], double radius);
 int gnn_radius_graph_count(int N, const double* x, double radius, const double lo[3],
                            const double hi[3], void* work, long work_bytes, int* row,
@@ -168,6 +172,10 @@ int gnn_radius_graph_count(int N, const double* x, double radius, const double l
 int gnn_radius_graph_fill(int N, const double* x, double radius, const double lo[3],
                           const double hi[3], const void* work, long work_bytes, const int* row,
                           long n_edges, int* send, int* recv, int* rev, void* stream);
+int gnn_radius_graph_build(int N, const double* x, double radius, const double lo[3],
+                           const double hi[3], void* work, long work_bytes, int* row,
+                           long capacity, int* send, int* recv, int* rev, long* n_edges,
+                           void* stream);
 
 #ifdef __cplusplus
 }

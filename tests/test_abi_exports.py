@@ -30,7 +30,7 @@ def test_surrogate_library_exports_every_declared_symbol():
     assert not missing, missing
     assert names == set(S._SIG)
     L = S.glib()
-    assert L.gnn_abi_version() == S.GNN_ABI_VERSION == 4
+    assert L.gnn_abi_version() == S.GNN_ABI_VERSION == 5
     # host-side argument checks answer without a device
     lo, hi = (ctypes.c_double * 3)(0, 0, 0), (ctypes.c_double * 3)(1, 1, 1)
     assert L.gnn_radius_graph_workspace(100, lo, hi, 0.1) > 0
