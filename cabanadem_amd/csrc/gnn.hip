@@ -1885,11 +1885,18 @@ int gnn_radius_graph_count(int N, const double* x, double radius, const double l
                        w.cstart, w.cnodes, w.cpos, w.deg, w.rows);
   }
   scan(w.deg, N, row, w.bsum, w.total64, s);
-  long long total = 0;
-  if (hipMemcpyAsync(&total, w.total64, sizeof(total), hipMemcpyDeviceToHost, s) != hipSuccess) return -2;
+  // the total comes back through a pinned word (one per host thread, kept
+  // until the process ends): a direct copy, without the staging a pageable
+  // destination takes
+  thread_local long long* total = nullptr;
+  if (!total && hipHostMalloc(reinterpret_cast<void**>(&total), sizeof(long long), hipHostMallocDefault) != hipSuccess) {
+    total = nullptr;
+    return -2;
+  }
+  if (hipMemcpyAsync(total, w.total64, sizeof(long long), hipMemcpyDeviceToHost, s) != hipSuccess) return -2;
   if (hipStreamSynchronize(s) != hipSuccess) return -2;
-  *n_edges = (long)total;
-  return total > 0x7fffffffll ? -3 : 0;
+  *n_edges = (long)*total;
+  return *total > 0x7fffffffll ? -3 : 0;
 }
 
 int gnn_radius_graph_fill(int N, const double* x, double radius, const double lo[3], const double hi[3],
