@@ -27,7 +27,7 @@ namespace demhip {
 namespace {
 
 #ifndef DEM_PHASE1_BATCH
-#define DEM_PHASE1_BATCH 4
+#define DEM_PHASE1_BATCH 6
 #endif
 #ifndef DEM_FORCE_MIN_WAVES
 #define DEM_FORCE_MIN_WAVES 1
