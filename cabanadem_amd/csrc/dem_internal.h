@@ -32,7 +32,10 @@
 
 namespace demhip {
 
-constexpr int kBlock = 256;       // threads per block of the particle kernels
+#ifndef DEM_BLOCK
+#define DEM_BLOCK 128
+#endif
+constexpr int kBlock = DEM_BLOCK;  // threads per block of the particle kernels
 constexpr int kMaxContacts = 32;  // upper bound of max_contacts
 constexpr int kMaxNeighbors = 255;
 constexpr int kXcds = 8;          // MI355X: 8 XCDs, workgroups dealt round-robin

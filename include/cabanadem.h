@@ -80,7 +80,7 @@ typedef struct dem_config {
   double lo[3], hi[3]; /* binning box (mesh_lo / mesh_hi); outside particles are clamped in */
   int rebuild_every;   /* neighbour rebuild cadence K in steps (1 = every step, as the reference) */
   int physics_mode;    /* DEM_MODE_* */
-  int block_size;      /* threads per block for the particle kernels (0 = 256) */
+  int block_size;      /* threads per block for the particle kernels (0 or 128) */
   int reserved;
 } dem_config;
 
@@ -303,7 +303,7 @@ int dem_set_system_params(dem_ctx* ctx, int system, double cor_pp, double cor_pw
 /* out[s][5] = KE_trans, KE_rot, px, py, pz of each system, current state. */
 int dem_system_diagnostics(dem_ctx* ctx, double* out);
 /* Force-kernel launch shape: grid_blocks = 0 launches one workgroup per
- * tile of 256 particles; > 0 launches a persistent grid of that many
+ * tile of 128 particles; > 0 launches a persistent grid of that many
  * workgroups that stride over the tiles.  Results do not depend on it. */
 int dem_set_launch(dem_ctx* ctx, int grid_blocks);
 
