@@ -39,7 +39,10 @@ constexpr int kBlock = DEM_BLOCK;  // threads per block of the particle kernels
 constexpr int kMaxContacts = 32;  // upper bound of max_contacts
 constexpr int kMaxNeighbors = 255;
 constexpr int kXcds = 8;          // MI355X: 8 XCDs, workgroups dealt round-robin
-constexpr int kTileRows = 8;      // y-rows per cell tile
+#ifndef DEM_TILE_ROWS
+#define DEM_TILE_ROWS 8
+#endif
+constexpr int kTileRows = DEM_TILE_ROWS;      // y-rows per cell tile
 
 // Logical block index that gives each XCD one contiguous run of blocks, so
 // that the neighbour gathers of one XCD stay inside its own L2.  Workgroup b
