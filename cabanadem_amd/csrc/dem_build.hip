@@ -283,7 +283,10 @@ __device__ inline void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-constexpr int kNbrStage = 256; // slots of one range round staged in LDS per wave
+#ifndef DEM_NBR_STAGE
+#define DEM_NBR_STAGE 128
+#endif
+constexpr int kNbrStage = DEM_NBR_STAGE; // slots of one range round staged in LDS per wave
 
 // Cabana's full Verlet list (VerletList with FullNeighborTag, built with the
 // cell list, src/CabanaDEM_Solver.hpp:60-63): j != i with |x_i - x_j|^2 <= rc^2,
