@@ -413,8 +413,12 @@ int validate(const dem_config* c, std::string* msg) {
   if (c->physics_mode != DEM_MODE_REFERENCE && c->physics_mode != DEM_MODE_CORRECTED) {
     *msg = "physics_mode must be 0 or 1"; return DEM_ERR_ARG;
   }
-  if (c->block_size != 0 && c->block_size != kBlock) { *msg = "block_size must be 0 or 128"; return DEM_ERR_ARG; }
-  static_assert(kBlock == 128, "the block_size message and the header state 128");
+#define DEM_STR2(x) #x
+#define DEM_STR(x) DEM_STR2(x)
+  if (c->block_size != 0 && c->block_size != kBlock) {
+    *msg = "block_size must be 0 or " DEM_STR(DEM_BLOCK);
+    return DEM_ERR_ARG;
+  }
   return DEM_OK;
 }
 
