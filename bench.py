@@ -2,7 +2,8 @@
 
 Workload (plain_large): 4,000,000 polydisperse spheres on a jittered lattice in
 a six-wall box, gravity, dt = 1e-6 s, cutoff 2.6 mm, rebuild every 20 steps.
---warmup steps settle the packing (untimed; 2,000 by default), then --steps
+The packing first settles for 2,000 steps (part of the workload, untimed,
+whatever --warmup says), then --warmup further untimed steps, then --steps
 full time steps are timed (10,000 by default).  Inputs are resident in HBM
 before the timed region.  One JSON line on stdout.
 
@@ -27,10 +28,32 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+SETTLE_STEPS = 2000  # plain_large: the packing settles under gravity first
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def finite_or_none(v):
+    """JSON has no NaN: a figure that could not be formed is null."""
+    if isinstance(v, dict):
+        return {k: finite_or_none(x) for k, x in v.items()}
+    if isinstance(v, (list, tuple)):
+        return [finite_or_none(x) for x in v]
+    if isinstance(v, float) and not np.isfinite(v):
+        return None
+    return v
+
+
+def emit(out):
+    print(json.dumps(finite_or_none(out), allow_nan=False), flush=True)
+
+
+def diag_interval(steps):
+    """Sampling interval of the timed window's diagnostics: about ten rows,
+    at least one whatever the window's length."""
+    return max(1, steps // 10)
 
 
 def measured_traffic(n):
@@ -46,22 +69,21 @@ def measured_traffic(n):
     return {"bytes_per_launch": d["bytes_per_launch"], "source": str(p.relative_to(ROOT))}
 
 
-def cpu_baseline(dims, steps_cadence, steps_faithful):
-    """Times the CPU oracle (the reference restated in C++/OpenMP) on the same
-    workload's initial state: rebuild every 20 steps, and every step as the
-    reference does."""
+def cpu_baseline(wl, settled, settle_steps, steps_cadence, steps_faithful):
+    """Times the CPU oracle (the reference restated in C++/OpenMP) from the
+    same settled snapshot the GPU's timed window starts from (positions,
+    velocities, spins and contact histories downloaded after the settle):
+    rebuilding every 20 steps, and every step as the reference does."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle_py as O
-    from cabanadem_amd import workloads as W
 
-    wl = W.plain_large(dims=dims)
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
     prm = O.params(dt=wl.dt, cor_pp=wl.cor, cor_pw=wl.cor, fr_pp=wl.friction,
                    fr_pw=wl.friction, gravity=wl.gravity, mode=0)
-    st = wl.state
+    st = settled.copy()
     _, t1 = O.run(st, wl.walls, prm, nsteps=steps_cadence, rc=wl.cutoff, lo=wl.lo, hi=wl.hi,
                   rebuild_every=wl.rebuild_every, max_n=64, threads=threads)
-    st = W.plain_large(dims=dims).state
+    st = settled.copy()
     _, t2 = O.run(st, wl.walls, prm, nsteps=steps_faithful, rc=wl.cutoff, lo=wl.lo, hi=wl.hi,
                   rebuild_every=1, max_n=64, threads=threads)
     n = st.n
@@ -70,10 +92,11 @@ def cpu_baseline(dims, steps_cadence, steps_faithful):
         "unit": "particle-steps/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"{n} spheres of the same workload from its initial state: {steps_cadence} "
-                   f"steps rebuilding every {wl.rebuild_every} ({t1:.1f} s); rebuilding every "
-                   f"step as the reference does: {n * steps_faithful / t2:.4g} particle-steps/s "
-                   f"over {steps_faithful} steps ({t2:.1f} s)"),
+        "sample": (f"{n} spheres of the same workload from the GPU's state after the "
+                   f"{settle_steps}-step settle: {steps_cadence} steps rebuilding every "
+                   f"{wl.rebuild_every} ({t1:.1f} s); rebuilding every step as the reference "
+                   f"does: {n * steps_faithful / t2:.4g} particle-steps/s over {steps_faithful} "
+                   f"steps ({t2:.1f} s)"),
         "value_rebuild_every_step": n * steps_faithful / t2,
     }
 
@@ -161,7 +184,7 @@ def run_bonded(args):
         out["cpu_baseline"] = cpu_baseline_bonded(dims, 200, wl.rebuild_every)
     else:
         out["cpu_baseline"] = None
-    print(json.dumps(out), flush=True)
+    emit(out)
 
 
 def cpu_baseline_ensemble(wl, nsys_sample, steps):
@@ -250,7 +273,7 @@ def run_ensemble(args):
     sysm.close()
     del sysm
     out["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline_ensemble(wl, 64, 500)
-    print(json.dumps(out), flush=True)
+    emit(out)
 
 
 def forward_flops(L, N, E, receiver_in_kernel=True):
@@ -398,7 +421,7 @@ def run_surrogate(args):
                  "achieved_tflops": mfma_tf, "peak_tflops": 2500.0, "frac": mfma_tf / 2500.0},
         "times_s": {"data_generation": t_gen, "training": t_train},
     }
-    print(json.dumps(out), flush=True)
+    emit(out)
 
 
 def main():
@@ -407,6 +430,8 @@ def main():
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--dims", type=str, default=None)
+    ap.add_argument("--settle", type=int, default=SETTLE_STEPS,
+                    help="plain_large: settle steps before the timed window (part of the workload)")
     ap.add_argument("--workload", choices=("plain_large", "bonded_block", "ensemble", "surrogate"),
                     default="plain_large")
     ap.add_argument("--surrogate-traj", type=int, default=5)
@@ -440,7 +465,7 @@ def main():
         args.warmup = 0 if args.warmup is None else args.warmup
         return run_ensemble(args)
     args.steps = 10000 if args.steps is None else args.steps
-    args.warmup = 2000 if args.warmup is None else args.warmup
+    args.warmup = 0 if args.warmup is None else args.warmup
     args.dims = args.dims or "200,200,100"
 
     from cabanadem_amd import abi
@@ -453,12 +478,18 @@ def main():
     sysm = wl.system(mode=args.mode)
     log(f"[bench] {n} spheres generated and uploaded in {time.perf_counter() - t0:.1f} s")
 
-    every = 500
-    sysm.set_diag_every(every)
+    t0 = time.perf_counter()
+    sysm.step(args.settle)
+    log(f"[bench] settle {args.settle} steps in {time.perf_counter() - t0:.2f} s")
+    settled = None if args.no_cpu_baseline else sysm.download()
     if args.warmup > 0:
         t0 = time.perf_counter()
         sysm.step(args.warmup)
         log(f"[bench] warmup {args.warmup} steps in {time.perf_counter() - t0:.2f} s")
+    # n, c and w are sampled inside the timed window: any `steps` consecutive
+    # steps hold at least steps // every multiples of `every`
+    every = diag_interval(args.steps)
+    sysm.set_diag_every(every)
     nrows0 = sysm.diag_history().shape[0]
 
     sysm.set_timing(True)
@@ -503,7 +534,9 @@ def main():
         "data": "synthetic (jittered-lattice packing from a seeded counter-based generator)",
         "config": {
             "workload": "plain DEM, large: %d polydisperse spheres (%dx%dx%d lattice), "
-                        "six-wall box, gravity, settle %d + timed %d steps" % (n, *dims, args.warmup, args.steps),
+                        "six-wall box, gravity, settle %d + warmup %d + timed %d steps"
+                        % (n, *dims, args.settle, args.warmup, args.steps),
+            "settle_steps": args.settle, "diag_every": every, "diag_rows_timed": int(len(diag)),
             "spheres": n, "dt": wl.dt, "cutoff": wl.cutoff, "rebuild_every": wl.rebuild_every,
             "physics_mode": "reference" if args.mode == 0 else "corrected",
             "parallelism": "single GPU",
@@ -543,11 +576,11 @@ def main():
         cs, cf = (int(v) for v in args.cpu_steps.split(","))
         log(f"[bench] CPU baseline: {cs} + {cf} oracle steps on {n} spheres")
         t0 = time.perf_counter()
-        out["cpu_baseline"] = cpu_baseline(dims, cs, cf)
+        out["cpu_baseline"] = cpu_baseline(wl, settled, args.settle, cs, cf)
         log(f"[bench] CPU baseline done in {time.perf_counter() - t0:.1f} s")
     else:
         out["cpu_baseline"] = None
-    print(json.dumps(out), flush=True)
+    emit(out)
 
 
 if __name__ == "__main__":

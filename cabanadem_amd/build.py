@@ -78,6 +78,8 @@ def build_oracle() -> Path:
 EXAMPLES = {
     "dem_validation": "examples/dem_validation.cpp",
     "bonded_tensile": "examples/bonded_tensile.cpp",
+    # the GPU insertion test's driver (tests/test_gpu_insertion.py)
+    "test_insertion_gpu": "tests/cpp/test_insertion_gpu.cpp",
 }
 
 

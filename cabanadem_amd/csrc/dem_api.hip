@@ -135,6 +135,7 @@ struct dem_ctx {
 
   // host copies of the static properties, original order
   std::vector<double> h_m, h_rad, h_E, h_nu, h_G, h_I;
+  double max_rad = 0.; // largest uploaded radius (0: none uploaded yet)
 
   // timing
   bool timing = false;
@@ -320,6 +321,27 @@ struct dem_ctx {
       hipStreamSynchronize(stream);
     }
     return r;
+  }
+
+  // Particle-particle contacts are found through the neighbour rows (every
+  // plain system; a bonded one only with contacts between unbonded pairs).
+  bool pp_live() const { return !bonds_on || bcfg.unbonded_contacts; }
+
+  // The rows must hold every pair that can touch before the next rebuild:
+  // cutoff >= 2 r_max, plus the skin when the rows are reused for several
+  // steps (rebuild_every > 1, which needs a skin to be checked at all).
+  int check_list_cover() {
+    if (!pp_live() || max_rad <= 0.) return DEM_OK;
+    const int K = cfg.rebuild_every;
+    if (K > 1 && !(cfg.skin > 0.))
+      return fail(DEM_ERR_ARG, "rebuild_every > 1 needs skin > 0: without it no displacement check "
+                               "guards the contacts that form between rebuilds");
+    const double need = 2. * max_rad + (K > 1 ? cfg.skin : 0.);
+    if (cfg.cutoff < need * (1. - 1e-12))
+      return fail(DEM_ERR_ARG, "cutoff " + std::to_string(cfg.cutoff) + " is below 2 r_max" +
+                               (K > 1 ? " + skin" : "") + " = " + std::to_string(need) +
+                               ": contacts would be missed");
+    return DEM_OK;
   }
 
   // Materials and walls of every system to the device (synchronous).
@@ -687,6 +709,9 @@ int dem_set_params(dem_ctx* c, const dem_config* cfg) {
   if (!same) return c->fail(DEM_ERR_ARG, "dem_set_params cannot change n, max_contacts, cutoff or the box");
   hipStreamSynchronize(c->stream);
   dem_config keep = c->cfg;
+  // positions at the last build (xb) are kept only while skin > 0: after a
+  // change the next rebuild runs unchecked and records them afresh
+  if (want.skin != keep.skin) c->list_valid = false;
   c->cfg = want;
   c->cfg.max_neighbors = keep.max_neighbors;
   c->cfg.n_walls = keep.n_walls;
@@ -784,6 +809,7 @@ int dem_upload_particles(dem_ctx* c, const dem_particles_view* v) {
   };
   setv(c->h_m, v->m); setv(c->h_rad, v->rad); setv(c->h_E, v->E);
   setv(c->h_nu, v->nu); setv(c->h_G, v->G); setv(c->h_I, v->I);
+  if (v->rad) c->max_rad = *std::max_element(c->h_rad.begin(), c->h_rad.end());
   std::vector<double4> mat(n);
   std::vector<double2> mI(n);
   for (int p = 0; p < n; ++p) {
@@ -1018,6 +1044,8 @@ int dem_update_tangential(dem_ctx* c) {
 static int force_call(dem_ctx* c, ForceKind kind, bool needs_list) {
   if (int r = check_ctx(c)) return r;
   if (needs_list && !c->list_valid) return c->fail(DEM_ERR_STATE, "neighbour list not built for the current positions");
+  if (needs_list && c->max_rad > 0. && c->pp_live() && c->cfg.cutoff < 2. * c->max_rad * (1. - 1e-12))
+    return c->fail(DEM_ERR_ARG, "cutoff is below 2 r_max: contacts would be missed");
   hipError_t e = launch_force(kind, c->step_args(), false, c->grid_blocks, c->stream);
   if (e != hipSuccess) return c->hip_fail(e, "force launch");
   if (kind != FK_PW_ACCUM && kind != FK_BOND_ACCUM) c->cC ^= 1;
@@ -1123,6 +1151,7 @@ static int graph_cycle(dem_ctx* c) {
 int dem_step(dem_ctx* c, int nsteps) {
   if (int r = check_ctx(c)) return r;
   if (nsteps <= 0) return DEM_OK;
+  if (int r = c->check_list_cover()) return r;
   const int K = c->cfg.rebuild_every;
   // diagnostic rows of this call
   int rows = 0;

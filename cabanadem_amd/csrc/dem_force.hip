@@ -123,7 +123,10 @@ __device__ inline void pp_contact(const double4& xi, const double* ui, const dou
       if (lm > 1e-12) { h[0] = m0 / lm; h[1] = m1 / lm; h[2] = m2 / lm; }
       else { h[0] = 0.; h[1] = 0.; h[2] = 0.; }
     } else {
-      h[0] = m0; h[1] = m1; h[2] = m2;
+      // the spring keeps the elastic part of the applied force
+      h[0] = -(tf0 + eta_t * vt0) / kt;
+      h[1] = -(tf1 + eta_t * vt1) / kt;
+      h[2] = -(tf2 + eta_t * vt2) / kt;
     }
   }
   F[0] += fn0 + tf0;                                                // :324-326

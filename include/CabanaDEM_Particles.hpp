@@ -109,8 +109,11 @@ class Particles {
   // <output_folder>/particles_<output_step>.csv, with the time in the header.
   void output(const int output_step, const double output_time, const bool use_reference = true) {
     (void)use_reference;
+    // a read-only access: host edits made before this call must still be
+    // uploaded by the next device call, so only this call's mark is undone
+    const bool touched = mirror_->host_touched;
     mirror_->before_host_access();
-    mirror_->host_touched = false; // read-only access
+    mirror_->host_touched = touched;
     std::string fn = output_folder_name_ + "/particles_" + std::to_string(output_step) + ".csv";
     std::ofstream os(fn);
     if (!os) {

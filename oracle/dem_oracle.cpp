@@ -372,7 +372,11 @@ static int force_pp_impl(OracleParticles* p, const int* counts, const int* ids, 
           if (lm > 1e-12) { sx[found_at] = m0 / lm; sy[found_at] = m1 / lm; sz[found_at] = m2 / lm; }
           else { sx[found_at] = 0.; sy[found_at] = 0.; sz[found_at] = 0.; }
         } else {
-          sx[found_at] = m0; sy[found_at] = m1; sz[found_at] = m2;
+          // corrected: the spring keeps the elastic part of the applied
+          // force, -(F_t + eta_t v_t) / k_t (k_t delta_t while sticking)
+          sx[found_at] = -(tf0 + eta_t * vt0) / kt;
+          sy[found_at] = -(tf1 + eta_t * vt1) / kt;
+          sz[found_at] = -(tf2 + eta_t * vt2) / kt;
         }
       }
       p->force[3 * i + 0] += fn0 + tf0;                        // :324-326

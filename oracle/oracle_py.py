@@ -12,13 +12,17 @@ sw_x/y/z [n, max_walls]; ss_x/y/z [n, max_cnts] (float64); ss_idx
 from __future__ import annotations
 
 import ctypes as C
+import os
 import subprocess
 from pathlib import Path
 
 import numpy as np
 
 HERE = Path(__file__).resolve().parent
-LIB = HERE / "_build" / "libdem_oracle.so"
+# ORACLE_LIB points the binding at another build of dem_oracle.cpp; the
+# mutation tests (tests/test_oracle_mutants.py) use it to run the closed-form
+# checks against deliberately broken copies of the oracle.
+LIB = Path(os.environ["ORACLE_LIB"]) if os.environ.get("ORACLE_LIB") else HERE / "_build" / "libdem_oracle.so"
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)

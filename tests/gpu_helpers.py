@@ -64,8 +64,12 @@ def granular_box(nx, ny, nz, *, spacing_r=2.05, r=1e-3, rmin=None, rmax=None, ji
 
 
 def make_system(st, w, *, dt, rc, lo, hi, cor_pp=0.9, cor_pw=0.9, fr_pp=0.3, fr_pw=0.3,
-                gravity=(0., 0., 0.), rebuild_every=1, mode=0, max_contacts=16, skin=0.0,
+                gravity=(0., 0., 0.), rebuild_every=1, mode=0, max_contacts=16, skin=None,
                 max_neighbors=0):
+    if skin is None:
+        # rows reused for several steps need a skin: the cover the cutoff
+        # leaves beyond the largest contact distance 2 r_max
+        skin = max(rc - 2. * float(st.rad.max()), 0.) if rebuild_every > 1 else 0.
     sysm = abi.System(st.n, dt=dt, cutoff=rc, lo=lo, hi=hi, n_walls=0 if w is None else w.n,
                       max_contacts=max_contacts, cor_pp=cor_pp, cor_pw=cor_pw,
                       friction_pp=fr_pp, friction_pw=fr_pw, gravity=gravity, skin=skin,

@@ -60,3 +60,17 @@ def test_bench_refuses_more_than_one_gpu():
     assert r.returncode == 2
     assert "one GPU" in r.stderr
     assert r.stdout == ""
+
+
+def test_json_line_has_no_nan():
+    out = bench.finite_or_none({"a": float("nan"), "b": [1.0, float("inf")], "c": {"d": 2.0}})
+    assert out == {"a": None, "b": [1.0, None], "c": {"d": 2.0}}
+
+
+@pytest.mark.parametrize("steps", [1, 5, 20, 37, 10000])
+def test_timed_window_always_holds_a_diagnostic_row(steps):
+    # rows are sampled at global steps that are multiples of the interval;
+    # any window of `steps` consecutive steps must contain at least one
+    every = bench.diag_interval(steps)
+    for start in (0, 1, 2000, 2005, 2019):
+        assert any((start + s) % every == 0 for s in range(steps))

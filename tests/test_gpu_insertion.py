@@ -1,0 +1,95 @@
+"""Particle insertion and removal while the solver runs on the GPU
+(tests/cpp/test_insertion_gpu.cpp, the container flow of
+examples/tst_01_particle_insertion.cpp:100-190 with SolverDEM::run between
+the resizes), against the oracle taking the same steps.
+
+The oracle side restates what resize does (src/CabanaDEM_Particles.hpp:273-297):
+the first min(n_old, n_new) entries of every field, tangential histories
+included, are kept and new entries start at zero.  When the container
+shrinks, tracked contacts with a removed particle are dropped (the
+reference would read past the end of its arrays there).  A velocity edited
+through a slice after output() must reach the device on the next run."""
+import json
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from cabanadem_amd.abi import ParticleState, Walls
+from gpu_helpers import normwise
+from oracle_driver import solver_run
+
+pytestmark = pytest.mark.gpu
+
+ROOT = Path(__file__).resolve().parents[1]
+EXE = ROOT / "examples" / "_build" / "test_insertion_gpu"
+R, RHO, E, NU = 1e-3, 2600., 1e7, 0.3
+
+
+def _init(st, pid):
+    m = 4. / 3. * np.pi * R ** 3 * RHO
+    st.x[pid] = (0.05 * R * ((pid % 3) - 1), R + 2.1 * R * pid, 0.03 * R * (pid % 2))
+    st.u[pid] = (0., -0.5, 0.)
+    st.omega[pid] = 0.
+    st.m[pid], st.I[pid], st.rad[pid] = m, 0.4 * m * R * R, R
+    st.E[pid], st.nu[pid], st.G[pid] = E, NU, E / (2. * (1. + NU))
+    st.ss_idx[pid] = -1
+    st.ss_cnt[pid] = 0
+
+
+def _resize(st, n):
+    """Particles::resize: keep the first min(n_old, n) entries, zero the rest;
+    drop tracked contacts with particles that no longer exist."""
+    out = ParticleState(n, max_cnts=st.max_cnts, max_walls=st.max_walls)
+    k = min(n, st.n)
+    for f in ParticleState.FIELDS:
+        a = getattr(out, f)
+        a[:k] = getattr(st, f)[:k]
+        if f == "ss_idx":
+            a[k:] = 0  # Kokkos zero-initialises
+    for i in range(n):
+        keep = [q for q in range(out.ss_cnt[i]) if out.ss_idx[i, q] < n]
+        for f in ("ss_idx", "ss_x", "ss_y", "ss_z"):
+            row = getattr(out, f)[i]
+            vals = row[keep].copy()
+            row[:] = -1 if f == "ss_idx" else 0.
+            row[:len(vals)] = vals
+        out.ss_cnt[i] = len(keep)
+    return out
+
+
+def test_insertion_and_removal_while_running(gpu_lib, tmp_path):
+    assert EXE.exists(), "build() compiles examples/_build/test_insertion_gpu"
+    print("driver:", EXE, flush=True)
+    p = subprocess.run([str(EXE), str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+    segs = [json.loads(l) for l in p.stdout.splitlines() if l.startswith("{")]
+    assert [s["n"] for s in segs] == [10, 20, 15]
+
+    w = Walls([(0., 0., 0.)], [(0., 1., 0.)], E, NU, E / (2. * (1. + NU)))
+    prm = dict(dt=1e-6, cor_pp=0.8, cor_pw=0.8, fr_pp=0.4, fr_pw=0.4)
+    run = dict(nsteps=2001, rc=3. * R, lo=(-10. * R, -2. * R, -10. * R), hi=(10. * R, 50. * R, 10. * R))
+    g = (0., -9.81, 0.)
+    st = ParticleState(10, max_cnts=6, max_walls=1)
+    for pid in range(10):
+        _init(st, pid)
+    contacts = []
+    for seg, n_next in enumerate((20, 15, None)):
+        solver_run(st, w, prm, run, gravity=g,
+                   record=lambda step, s: contacts.append(int(s.ss_cnt.sum())))
+        got = segs[seg]
+        for f in ("x", "u", "omega"):
+            a = np.array(got[f])
+            ref = getattr(st, f)
+            if np.abs(ref).max() > 0:
+                assert normwise(a, ref) <= 1e-9, (seg, f, normwise(a, ref))
+        if seg == 0:
+            st.u[0, 0] += 0.1  # the edit made after output()
+        if n_next is not None:
+            n_old = st.n
+            st = _resize(st, n_next)
+            for pid in range(n_old, n_next):
+                _init(st, pid)
+    assert max(contacts) >= 3  # pp and wall contacts were live across the resizes
+    assert sorted(f.name for f in tmp_path.glob("particles_1*.csv")) == ["particles_100.csv"]
