@@ -83,3 +83,64 @@ def make_system(st, w, *, dt, rc, lo, hi, cor_pp=0.9, cor_pw=0.9, fr_pp=0.3, fr_
 def oracle_params(dt, cor_pp=0.9, cor_pw=0.9, fr_pp=0.3, fr_pw=0.3, gravity=(0., 0., 0.), mode=0):
     return O.params(dt=dt, cor_pp=cor_pp, cor_pw=cor_pw, fr_pp=fr_pp, fr_pw=fr_pw,
                     gravity=gravity, mode=mode)
+
+
+def sorted_rows(counts, ids):
+    """Neighbour rows as a sortable array: each row's first counts[i] ids in
+    ascending order, the rest of the row filled with INT32_MAX (order-free
+    comparison of two lists, vectorised for millions of rows)."""
+    counts = np.asarray(counts)
+    ids = np.asarray(ids)
+    cols = np.arange(ids.shape[1])[None, :]
+    filled = np.where(cols < counts[:, None], ids, np.iinfo(np.int32).max)
+    return np.sort(filled, axis=1)
+
+
+def same_neighbor_sets(a, b):
+    """(counts, ids) pairs of two lists hold the same sets, row by row."""
+    (ca, ia), (cb, ib) = a, b
+    if not np.array_equal(np.asarray(ca), np.asarray(cb)):
+        return False
+    w = int(max(np.max(ca, initial=0), 1))
+    return np.array_equal(sorted_rows(ca, ia)[:, :w], sorted_rows(cb, ib)[:, :w])
+
+
+def history_arrays(st):
+    """Tracked pp contacts as (keys i * n + j ascending, histories [k][3]):
+    the order-free form of the (i, j) -> (hx, hy, hz) map, vectorised."""
+    n = st.n
+    cols = np.arange(st.ss_idx.shape[1])[None, :]
+    live = cols < st.ss_cnt[:, None]
+    i = np.broadcast_to(np.arange(n)[:, None], st.ss_idx.shape)[live].astype(np.int64)
+    key = i * n + st.ss_idx[live].astype(np.int64)
+    h = np.stack([st.ss_x[live], st.ss_y[live], st.ss_z[live]], axis=-1)
+    order = np.argsort(key, kind="stable")
+    return key[order], h[order]
+
+
+def compare_history_arrays(gpu_st, ref_st, tol):
+    """compare_history for large states: the same tracked (i, j) pairs and
+    their histories within tol normwise.  Returns the number of pairs."""
+    kg, hg = history_arrays(gpu_st)
+    kr, hr = history_arrays(ref_st)
+    assert np.array_equal(kg, kr), "tracked contact sets differ"
+    if len(kr):
+        assert normwise(hg, hr) <= tol, normwise(hg, hr)
+    return len(kr)
+
+
+def compare_diag(diag, diag_ref, tol):
+    """KE_trans, KE_rot and momentum histories (rows of dem_get_diag_history
+    and the oracle's run) within tol of each column's largest magnitude."""
+    assert diag.shape[0] == diag_ref.shape[0] and diag.shape[0] > 0
+    assert np.array_equal(diag[:, 0], diag_ref[:, 0])
+    worst = 0.
+    for k in range(1, 6):
+        scale = np.abs(diag_ref[:, k]).max()
+        if scale == 0:
+            assert np.abs(diag[:, k]).max() == 0, k
+            continue
+        err = np.abs(diag[:, k] - diag_ref[:, k]).max() / scale
+        assert err <= tol, (k, err)
+        worst = max(worst, err)
+    return worst

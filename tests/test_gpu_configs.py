@@ -1,0 +1,356 @@
+"""Each benchmark workload at its stated size on the GPU, against the oracle.
+
+* Plain small: 10,000 spheres (25 x 20 x 20 lattice), 5,000 steps, rebuild
+  every 20: kinetic-energy and momentum histories within 1e-8.
+* Plain large: 4,000,000 polydisperse spheres after the 2,000-step settle:
+  neighbour sets bit-exact against the oracle's cell list; forces, torques,
+  pp and wall histories within 1e-12 on that state; a 100-step KE/momentum
+  history from it within 1e-8.
+* Bonded block: 1,000,000 spheres (100^3, E = 1e9, dt = 2e-8): bond sets and
+  rest lengths bit-exact; forces, torques and bond histories within 1e-12 on
+  a live state; 200 steps of the pulled block (breakage on, contacts between
+  unbonded pairs) within 1e-10 with the same intact-bond set.
+* Ensemble: all 256 systems (about 3.9M spheres) advanced together; eight
+  systems with their own restitution and friction run alone through the
+  oracle with those materials, KE/momentum within 1e-8 at every 500 steps;
+  the same systems with other materials land far outside that bar, so the
+  per-system materials are what the GPU used.
+* Surrogate: the 64,000-sphere column's radius graph on the device equals
+  the oracle's cell-list neighbour sets (the same |x_i - x_j|^2 <= R^2
+  criterion); the 10-layer, 128-wide network's forward on that graph against
+  the oracle network with BF16 rounding at the same points.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+from oracle_bonded import GROUP_VELOCITY, bonded_run
+from cabanadem_amd import abi
+from cabanadem_amd import workloads as W
+from gpu_helpers import (compare_diag, compare_history_arrays, normwise, same_neighbor_sets,
+                         sorted_rows)
+
+pytestmark = pytest.mark.gpu
+
+TOL_STEP = 1e-12
+TOL_RUN = 1e-8
+
+
+def _say(msg):
+    print(f"[configs] {msg}", flush=True)
+
+
+def _prm(wl, mode=0, cor=None, fr=None):
+    cor = wl.cor if cor is None else cor
+    fr = wl.friction if fr is None else fr
+    return O.params(dt=wl.dt, cor_pp=cor, cor_pw=cor, fr_pp=fr, fr_pw=fr, gravity=wl.gravity,
+                    mode=mode)
+
+
+def test_plain_small_full_run(gpu_lib):
+    wl = W.plain_small()
+    nsteps, every = 5000, 250
+    t0 = time.perf_counter()
+    ref = wl.state.copy()
+    diag_ref, _ = O.run(ref, wl.walls, _prm(wl), nsteps=nsteps, rc=wl.cutoff, lo=wl.lo, hi=wl.hi,
+                        rebuild_every=wl.rebuild_every, max_n=64, diag_every=every)
+    _say(f"plain small: oracle {nsteps} steps in {time.perf_counter() - t0:.1f} s")
+    sysm = wl.system()
+    sysm.set_diag_every(every)
+    sysm.step(nsteps)
+    diag = sysm.diag_history()
+    err = compare_diag(diag[:, :6], diag_ref, TOL_RUN)
+    got = sysm.download()
+    # contacts happened: the mean tracked-contact count over the samples
+    assert diag[:, 7].max() > 0 and ref.ss_cnt.sum() > 0
+    _say(f"plain small: KE/momentum worst {err:.2e}, c-bar max {diag[:, 7].max():.3f}, "
+         f"|du|/|u| {normwise(got.u, ref.u):.2e}")
+    sysm.close()
+
+
+@pytest.fixture(scope="module")
+def settled_large():
+    """The 4M workload after its 2,000-step settle on the GPU, downloaded."""
+    wl = W.plain_large()
+    t0 = time.perf_counter()
+    sysm = wl.system()
+    sysm.step(2000)
+    st = sysm.download(abi.ParticleState(wl.state.n, wl.max_contacts, 6))
+    sysm.close()
+    _say(f"plain large: generated, settled and downloaded in {time.perf_counter() - t0:.1f} s; "
+         f"{int(st.ss_cnt.sum())} tracked pp contacts")
+    return wl, st
+
+
+def test_plain_large_neighbor_sets_and_forces(gpu_lib, settled_large):
+    wl, st = settled_large
+    # fresh context on the settled state: build the rows, one force pass
+    sysm = abi.System(st.n, dt=wl.dt, cutoff=wl.cutoff, lo=wl.lo, hi=wl.hi, n_walls=wl.walls.n,
+                      max_contacts=wl.max_contacts, cor_pp=wl.cor, cor_pw=wl.cor,
+                      friction_pp=wl.friction, friction_pw=wl.friction, gravity=wl.gravity,
+                      skin=wl.skin, rebuild_every=wl.rebuild_every)
+    sysm.upload_walls(wl.walls)
+    sysm.upload(st)
+    sysm.build_neighbors()
+    max_n = max(sysm.counters().max_neighbors_seen, 1)
+    got_rows = sysm.neighbors(max_n)
+    t0 = time.perf_counter()
+    ref_rows = O.neighbors_cell(st.x, wl.cutoff, wl.lo, wl.hi, max_n=max_n)
+    _say(f"plain large: oracle cell list in {time.perf_counter() - t0:.1f} s, max row {max_n}, "
+         f"n-bar {got_rows[0].mean():.2f}")
+    assert same_neighbor_sets(got_rows, ref_rows)
+    assert got_rows[0].sum() > 5 * st.n
+
+    ref = st.copy()
+    O.compute_forces(ref, wl.walls, ref_rows[0], ref_rows[1], _prm(wl))
+    sysm.compute_forces()
+    got = sysm.download(abi.ParticleState(st.n, wl.max_contacts, 6))
+    assert normwise(got.force, ref.force) <= TOL_STEP
+    assert normwise(got.torque, ref.torque) <= TOL_STEP
+    npairs = compare_history_arrays(got, ref, TOL_STEP)
+    for comp in ("sw_x", "sw_y", "sw_z"):
+        assert normwise(getattr(got, comp), getattr(ref, comp)) <= TOL_STEP
+    # the settled bed's contacts are pp ones (the walls sit a spacing away)
+    assert npairs > 10000
+    assert np.array_equal(got.ss_cnt, ref.ss_cnt)
+    _say(f"plain large: {npairs} tracked pp contacts, F rel {normwise(got.force, ref.force):.2e}")
+    sysm.close()
+
+
+def test_plain_large_run_from_settled_state(gpu_lib, settled_large):
+    wl, st = settled_large
+    nsteps, every = 100, 10
+    sysm = abi.System(st.n, dt=wl.dt, cutoff=wl.cutoff, lo=wl.lo, hi=wl.hi, n_walls=wl.walls.n,
+                      max_contacts=wl.max_contacts, cor_pp=wl.cor, cor_pw=wl.cor,
+                      friction_pp=wl.friction, friction_pw=wl.friction, gravity=wl.gravity,
+                      skin=wl.skin, rebuild_every=wl.rebuild_every)
+    sysm.upload_walls(wl.walls)
+    sysm.upload(st)
+    sysm.set_diag_every(every)
+    sysm.step(nsteps)
+    diag = sysm.diag_history()
+    ref = st.copy()
+    t0 = time.perf_counter()
+    diag_ref, _ = O.run(ref, wl.walls, _prm(wl), nsteps=nsteps, rc=wl.cutoff, lo=wl.lo, hi=wl.hi,
+                        rebuild_every=wl.rebuild_every, max_n=64, diag_every=every)
+    _say(f"plain large: oracle {nsteps} steps in {time.perf_counter() - t0:.1f} s")
+    err = compare_diag(diag[:, :6], diag_ref, TOL_RUN)
+    got = sysm.download(abi.ParticleState(st.n, wl.max_contacts, 6))
+    assert normwise(got.x - st.x, ref.x - st.x) <= TOL_RUN
+    assert normwise(got.u, ref.u) <= TOL_RUN
+    _say(f"plain large: KE/momentum worst {err:.2e}, |du|/|u| {normwise(got.u, ref.u):.2e}")
+    sysm.close()
+
+
+def _bond_arrays(n, max_bonds, idx, count, init_len, hist):
+    """Intact bonds as (keys i * n + j ascending, rest lengths, histories [k][9])
+    and the number of broken slots (partner -1)."""
+    cols = np.arange(max_bonds)[None, :]
+    used = cols < count[:, None]
+    intact = used & (idx >= 0)
+    i = np.broadcast_to(np.arange(n)[:, None], idx.shape)[intact].astype(np.int64)
+    key = i * n + idx[intact].astype(np.int64)
+    order = np.argsort(key, kind="stable")
+    return key[order], init_len[intact][order], hist[intact][order], int((used & (idx < 0)).sum())
+
+
+def _oracle_bonds(b):
+    return _bond_arrays(b.n, b.max_bonds, b.idx, b.count, b.init_len,
+                        np.concatenate([b.ft, b.mn, b.mt], axis=-1))
+
+
+def _gpu_bonds(b):
+    return _bond_arrays(b.n, b.max_bonds, b.idx, b.count, b.init_len, b.history())
+
+
+def _bond_view(ob):
+    v = abi.BondState(ob.n, ob.max_bonds)
+    v.idx[:], v.count[:], v.init_len[:] = ob.idx, ob.count, ob.init_len
+    for q, (arr, c) in enumerate([(ob.ft, 0), (ob.ft, 1), (ob.ft, 2), (ob.mn, 0), (ob.mn, 1),
+                                  (ob.mn, 2), (ob.mt, 0), (ob.mt, 1), (ob.mt, 2)]):
+        getattr(v, abi.BondState.HIST[q])[:] = arr[:, :, c]
+    return v
+
+
+def test_bonded_block_full_size(gpu_lib):
+    wl = W.bonded_block()
+    st0 = wl.state
+    n = st0.n
+    mode = abi.MODE_CORRECTED  # the bonded workload's mode (reference Q15 shakes bonds apart)
+    t0 = time.perf_counter()
+    ob = O.setup_bonds(st0, wl.bond_factor, wl.cutoff, wl.lo, wl.hi, wl.max_bonds)
+    _say(f"bonded: oracle bond set-up in {time.perf_counter() - t0:.1f} s")
+    sysm = wl.system(mode=mode)
+    ko, lo_, _, _ = _oracle_bonds(ob)
+    kg, lg, _, _ = _gpu_bonds(sysm.download_bonds())
+    assert np.array_equal(ko, kg) and np.array_equal(lo_, lg)
+    assert len(ko) == 6 * n - 6 * 100 * 100  # six face bonds, less one per face site
+
+    # identical live state: positions moved by up to 0.6 um (a stretch of up to
+    # 1.2 um breaks a bond: E (r - L0) / (r_i + r_j) > 1e6 Pa), velocities,
+    # spins and bond histories
+    rng = np.random.default_rng(5)
+    st = st0.copy()
+    st.x += rng.uniform(-0.6e-6, 0.6e-6, size=st.x.shape)
+    st.u[:] = rng.normal(0., 0.01, size=st.u.shape)
+    st.omega[:] = rng.normal(0., 10., size=st.omega.shape)
+    for a in (ob.ft, ob.mn, ob.mt):
+        a[:] = rng.normal(0., 1e-4, size=a.shape)
+    sysm.upload(st)
+    sysm.upload_bonds(_bond_view(ob))
+    sysm.zero_forces()
+    sysm.bond_forces()
+    g = sysm.download()
+    gbl = sysm.download_bonds()
+    bp = O.bond_params(beta=0.1, strength=wl.tensile_strength, mode=mode)
+    ref = st.copy()
+    O.zero_forces(ref, O.params(dt=wl.dt, mode=mode))
+    nbroken = O.bond_force(ref, ob, bp, wl.dt)
+    assert normwise(g.force, ref.force) <= TOL_STEP
+    assert normwise(g.torque, ref.torque) <= TOL_STEP
+    ko, _, ho, bo = _oracle_bonds(ob)
+    kg, _, hg, bg = _gpu_bonds(gbl)
+    assert np.array_equal(ko, kg) and bo == bg == nbroken
+    assert normwise(hg, ho) <= TOL_STEP
+    assert sysm.counters().bonds_broken == nbroken
+    assert 0 < nbroken < len(ko) // 10
+    _say(f"bonded: identical state, {nbroken} bonds broken, F rel {normwise(g.force, ref.force):.2e}")
+    sysm.close()
+
+    # 200 steps of the workload from its initial state
+    nsteps = 200
+    sysm = wl.system(mode=mode)
+    sysm.step(nsteps)
+    g = sysm.download()
+    gb = sysm.download_bonds()
+    ref = st0.copy()
+    obr = O.setup_bonds(ref, wl.bond_factor, wl.cutoff, wl.lo, wl.hi, wl.max_bonds)
+    t0 = time.perf_counter()
+    prm = O.params(dt=wl.dt, cor_pp=wl.cor, cor_pw=wl.cor, fr_pp=wl.friction, fr_pw=wl.friction,
+                   mode=mode)
+    nb_o = bonded_run(ref, obr, nsteps=nsteps, prm=prm, bp=bp, dt=wl.dt, group=wl.group,
+                      kinds=[GROUP_VELOCITY, GROUP_VELOCITY], vel=wl.group_vel, contacts=True,
+                      rc=wl.cutoff, lo=wl.lo, hi=wl.hi, rebuild_every=wl.rebuild_every)
+    _say(f"bonded: oracle {nsteps} steps in {time.perf_counter() - t0:.1f} s, {nb_o} broken")
+    assert sysm.counters().bonds_broken == nb_o
+    ko, _, ho, _ = _oracle_bonds(obr)
+    kg, _, hg, _ = _gpu_bonds(gb)
+    assert np.array_equal(ko, kg)
+    assert normwise(hg, ho) <= 1e-10
+    assert normwise(g.x - st0.x, ref.x - st0.x) <= 1e-10
+    assert normwise(g.u, ref.u) <= 1e-10
+    assert np.abs(ref.u).max() > 0
+    _say(f"bonded: run |du|/|u| {normwise(g.u, ref.u):.2e}")
+    sysm.close()
+
+
+def test_ensemble_256_systems_against_the_oracle(gpu_lib):
+    wl = W.ensemble()
+    assert len(wl.systems) == 256
+    counts = np.array([sd["n"] for sd in wl.systems])
+    order = np.argsort(counts, kind="stable")
+    # eight systems spread over the lower half of the sizes (the oracle's
+    # time), plus the largest
+    pick = [int(k) for k in order[::16][:8]] + [int(order[-1])]
+    chunk, nchunks = 500, 6
+    sysm = wl.system()
+    rows = []
+    for c in range(nchunks):
+        sysm.step(chunk)
+        rows.append(sysm.system_diagnostics())
+    gpu = np.stack(rows, axis=1)  # [system][sample][5]
+    got = sysm.download()
+    off = sysm.offsets
+    c_ens = sysm.counters()
+    _say(f"ensemble: {sysm.n} spheres in 256 systems, {nchunks * chunk} steps; "
+         f"c-bar at the last sample {c_ens.mean_contacts:.2e}")
+    t0 = time.perf_counter()
+    worst = 0.
+    for k in pick:
+        sd = wl.systems[k]
+        own = (sd["cor_pp"], sd["friction_pp"])
+        other = (0.99, 0.05)
+        res = {}
+        for tag, (cor, fr) in (("own", own), ("other", other)):
+            if tag == "other" and k == pick[-1]:
+                continue
+            st = wl.states[k].copy()
+            prm = O.params(dt=wl.dt, cor_pp=cor, cor_pw=cor, fr_pp=fr, fr_pw=fr,
+                           gravity=wl.gravity, mode=0)
+            # the same samples as the GPU's: the state after each chunk
+            d = []
+            for c in range(nchunks):
+                O.run(st, wl.walls[k], prm, nsteps=chunk, rc=wl.cutoff, lo=sd["lo"], hi=sd["hi"],
+                      rebuild_every=wl.rebuild_every, max_n=64)
+                d.append(O.diagnostics(st))
+            res[tag] = (np.array(d), st)
+        d, st = res["own"]
+        g = gpu[k]
+        for q in range(5):
+            scale = np.abs(d[:, q]).max()
+            if scale == 0:
+                continue
+            err = np.abs(g[:, q] - d[:, q]).max() / scale
+            worst = max(worst, err)
+            assert err <= TOL_RUN, (k, q, err)
+        gs = slice(off[k], off[k + 1])
+        assert normwise(got.x[gs] - wl.states[k].x, st.x - wl.states[k].x) <= TOL_RUN
+        if "other" in res:
+            d2 = res["other"][0]
+            sens = np.abs(d2[:, 0] - d[:, 0]).max() / np.abs(d[:, 0]).max()
+            assert sens > 1e4 * TOL_RUN, (k, sens)  # the materials act in this window
+    _say(f"ensemble: {len(pick)} systems checked against the oracle in "
+         f"{time.perf_counter() - t0:.1f} s, worst KE/momentum {worst:.2e}")
+    sysm.close()
+
+
+def test_surrogate_column_graph_and_network_at_size(gpu_lib):
+    import torch
+
+    import gnn_oracle as GO
+    from cabanadem_amd import surrogate as S
+
+    wl = W.column()
+    n = wl.state.n
+    assert n == 64000
+    radius = 2.5e-3
+    x = wl.state.x
+    gb = S.GraphBuilder(n, radius, wl.lo, wl.hi)
+    gr = gb.build(x)
+    torch.cuda.synchronize()
+    row = gr.row.cpu().numpy().astype(np.int64)
+    send = gr.send.cpu().numpy().astype(np.int64)
+    recv = gr.recv.cpu().numpy().astype(np.int64)
+    deg = np.diff(row)
+    # the oracle's cell list with the same criterion: the same sets, and the
+    # device rows ascending
+    c_o, i_o = O.neighbors_cell(x, radius, wl.lo, wl.hi, max_n=int(deg.max()) + 1)
+    assert np.array_equal(deg, c_o)
+    ids = np.full((n, int(deg.max())), np.iinfo(np.int32).max, dtype=np.int64)
+    cols = np.arange(len(send)) - row[recv]
+    ids[recv, cols] = send
+    assert np.array_equal(ids, sorted_rows(c_o, i_o)[:, :ids.shape[1]])
+    assert np.array_equal(ids, np.sort(ids, axis=1))
+    _say(f"surrogate: {n} nodes, {gr.E} edges, max degree {deg.max()}")
+
+    rng = np.random.default_rng(8)
+    xn = np.zeros((n, 32)); xn[:, :20] = rng.normal(size=(n, 20))
+    xe = np.zeros((gr.E, 32)); xe[:, :4] = rng.normal(size=(gr.E, 4))
+    L = 10
+    net = S.Network(n_layers=L, seed=4)
+    bf = lambda a: torch.from_numpy(np.asarray(a, dtype=np.float32)).to(torch.bfloat16).cuda()
+    out = net.forward(gr, bf(xn), bf(xe), train=False).cpu().numpy().astype(np.float64)
+    flat = net.flat.cpu().numpy()
+    t0 = time.perf_counter()
+    ref = GO.Net(flat, L, quant=True)
+    want = ref.forward(row, send, recv, GO.bf16(xn), GO.bf16(xe))
+    rel = np.linalg.norm(out - want) / np.linalg.norm(want)
+    _say(f"surrogate: 10-layer forward rel {rel:.2e} against the oracle "
+         f"({time.perf_counter() - t0:.1f} s of oracle)")
+    # the tolerance of the small-graph network check (BF16 rounding at the
+    # same points; FP32 summation order differs)
+    assert rel < 2e-2
