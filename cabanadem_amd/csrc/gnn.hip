@@ -18,6 +18,7 @@
 // output columns).
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <mutex>
 
 #include "../../include/cabanadem_gnn.h"
 
@@ -1861,6 +1862,60 @@ int gnn_node_layer_forward(int N, const void* h, const void* e, const int* row, 
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
+} // extern "C"
+
+// Workspace of gnn_radius_graph_build_static: one array per GPU, held from
+// the module's load until the process ends, shared by every call on that
+// GPU (overlapping calls on one GPU race).  In host code `scratch` names a
+// host-side stand-in; its device address comes from hipGetSymbolAddress,
+// taken once per device with that device current.
+static __device__ __align__(256) int scratch[GNN_GRAPH_SCRATCH_BYTES / sizeof(int)];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+void* scratch_dev[kMaxDevices]; // device address of `scratch` on each GPU, once known
+std::mutex scratch_mu;
+
+int graph_scratch(void** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return -2;
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_dev[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return -2;
+    scratch_dev[dev] = p;
+  }
+  *out = scratch_dev[dev];
+  return 0;
+}
+
+// Count pass of the radius graph over the carved workspace w: bins the
+// points, fills row and reads the edge total back into host_total (host
+// memory the caller owns), then *n_edges.
+int graph_count(int N, const double* x, double radius, const CellGrid& g, const GraphWork& w, int* row,
+                long long* host_total, long* n_edges, hipStream_t s) {
+  hipLaunchKernelGGL(k_g_zero, dim3(nblk(g.ncell)), dim3(kThreads), 0, s, w.ccount, (long)g.ncell);
+  hipLaunchKernelGGL(k_g_zero, dim3(nblk(g.ncell)), dim3(kThreads), 0, s, w.cursor, (long)g.ncell);
+  if (N > 0) {
+    hipLaunchKernelGGL(k_g_bin, dim3(nblk(N)), dim3(kThreads), 0, s, N, x, g, w.cell_of, w.ccount);
+    scan(w.ccount, g.ncell, w.cstart, w.bsum, nullptr, s);
+    hipLaunchKernelGGL(k_g_scatter, dim3(nblk(N)), dim3(kThreads), 0, s, N, x, w.cell_of, w.cstart,
+                       w.cursor, w.cnodes, w.cpos);
+    hipLaunchKernelGGL(k_g_count, dim3(nblk(N)), dim3(kThreads), 0, s, N, g, radius * radius, w.cell_of,
+                       w.cstart, w.cnodes, w.cpos, w.deg, w.rows);
+  }
+  scan(w.deg, N, row, w.bsum, w.total64, s);
+  if (hipMemcpyAsync(host_total, w.total64, sizeof(long long), hipMemcpyDeviceToHost, s) != hipSuccess) return -2;
+  if (hipStreamSynchronize(s) != hipSuccess) return -2;
+  *n_edges = (long)*host_total;
+  return *host_total > 0x7fffffffll ? -3 : 0;
+}
+
+} // namespace
+
+extern "C" {
+
 long gnn_radius_graph_workspace(int N, const double lo[3], const double hi[3], double radius) {
   CellGrid g;
   if (N < 0 || make_grid(lo, hi, radius, &g)) return -1;
@@ -1873,18 +1928,6 @@ int gnn_radius_graph_count(int N, const double* x, double radius, const double l
   if (N < 0 || (N > 0 && !x) || !work || !row || !n_edges || make_grid(lo, hi, radius, &g)) return -1;
   const GraphWork w = carve(work, N, g.ncell);
   if (work_bytes < w.ints * (long)sizeof(int)) return -1;
-  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-  hipLaunchKernelGGL(k_g_zero, dim3(nblk(g.ncell)), dim3(kThreads), 0, s, w.ccount, (long)g.ncell);
-  hipLaunchKernelGGL(k_g_zero, dim3(nblk(g.ncell)), dim3(kThreads), 0, s, w.cursor, (long)g.ncell);
-  if (N > 0) {
-    hipLaunchKernelGGL(k_g_bin, dim3(nblk(N)), dim3(kThreads), 0, s, N, x, g, w.cell_of, w.ccount);
-    scan(w.ccount, g.ncell, w.cstart, w.bsum, nullptr, s);
-    hipLaunchKernelGGL(k_g_scatter, dim3(nblk(N)), dim3(kThreads), 0, s, N, x, w.cell_of, w.cstart,
-                       w.cursor, w.cnodes, w.cpos);
-    hipLaunchKernelGGL(k_g_count, dim3(nblk(N)), dim3(kThreads), 0, s, N, g, radius * radius, w.cell_of,
-                       w.cstart, w.cnodes, w.cpos, w.deg, w.rows);
-  }
-  scan(w.deg, N, row, w.bsum, w.total64, s);
   // the total comes back through a pinned word (one per host thread, kept
   // until the process ends): a direct copy, without the staging a pageable
   // destination takes
@@ -1893,10 +1936,7 @@ int gnn_radius_graph_count(int N, const double* x, double radius, const double l
     total = nullptr;
     return -2;
   }
-  if (hipMemcpyAsync(total, w.total64, sizeof(long long), hipMemcpyDeviceToHost, s) != hipSuccess) return -2;
-  if (hipStreamSynchronize(s) != hipSuccess) return -2;
-  *n_edges = (long)*total;
-  return *total > 0x7fffffffll ? -3 : 0;
+  return graph_count(N, x, radius, g, w, row, total, n_edges, reinterpret_cast<hipStream_t>(stream));
 }
 
 int gnn_radius_graph_fill(int N, const double* x, double radius, const double lo[3], const double hi[3],
@@ -1922,6 +1962,23 @@ int gnn_radius_graph_build(int N, const double* x, double radius, const double l
   if (int r = gnn_radius_graph_count(N, x, radius, lo, hi, work, work_bytes, row, n_edges, stream)) return r;
   if (*n_edges > capacity) return 1; // the caller grows the edge arrays and calls the fill
   return gnn_radius_graph_fill(N, x, radius, lo, hi, work, work_bytes, row, *n_edges, send, recv, rev, stream);
+}
+
+int gnn_radius_graph_build_static(int N, const double* x, double radius, const double lo[3],
+                                  const double hi[3], int* row, long capacity, int* send, int* recv,
+                                  int* rev, long* n_edges, void* stream) {
+  CellGrid g;
+  if (N < 0 || (N > 0 && !x) || !row || !n_edges || capacity < 0 || make_grid(lo, hi, radius, &g)) return -1;
+  const long need = carve(nullptr, N, g.ncell).ints * (long)sizeof(int);
+  if (need > GNN_GRAPH_SCRATCH_BYTES) return -4; // the graph is too large for the library's array
+  void* work = nullptr;
+  if (int r = graph_scratch(&work)) return r;
+  const GraphWork w = carve(work, N, g.ncell);
+  long long total = 0; // pageable: no pinned word is allocated
+  if (int r = graph_count(N, x, radius, g, w, row, &total, n_edges, reinterpret_cast<hipStream_t>(stream)))
+    return r;
+  if (*n_edges > capacity) return 1;
+  return gnn_radius_graph_fill(N, x, radius, lo, hi, work, need, row, *n_edges, send, recv, rev, stream);
 }
 
 } // extern "C"

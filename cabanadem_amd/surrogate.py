@@ -88,6 +88,8 @@ _SIG = {
                                         _vp, _vp, _vp, _vp]),
     "gnn_radius_graph_build": (C.c_int, [C.c_int, _vp, C.c_double, _dp, _dp, _vp, C.c_long, _vp, C.c_long,
                                          _vp, _vp, _vp, C.POINTER(C.c_long), _vp]),
+    "gnn_radius_graph_build_static": (C.c_int, [C.c_int, _vp, C.c_double, _dp, _dp, _vp, C.c_long, _vp,
+                                                _vp, _vp, C.POINTER(C.c_long), _vp]),
 }
 
 _GLIB = None

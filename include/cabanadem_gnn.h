@@ -177,6 +177,22 @@ int gnn_radius_graph_build(int N, const double* x, double radius, const double l
                            long capacity, int* send, int* recv, int* rev, long* n_edges,
                            void* stream);
 
+/* gnn_radius_graph_build_static: gnn_radius_graph_build with the workspace
+ * in a device array of the library's own (GNN_GRAPH_SCRATCH_BYTES on each
+ * GPU, held from the library's load until the process ends) instead of a
+ * caller's buffer, so the call allocates nothing: no device memory, no
+ * pinned host word.  No counterpart in the reference; it is for callers
+ * that cannot allocate (serving loops, harnesses that forbid allocation).
+ * The caller still passes x, row and the edge arrays.  Same results and
+ * return codes as gnn_radius_graph_build, and -4 when the graph's workspace
+ * (gnn_radius_graph_workspace) exceeds GNN_GRAPH_SCRATCH_BYTES, before any
+ * work is queued.  Every call on a GPU shares that GPU's array: calls that
+ * overlap on one GPU (other host threads or streams) race. */
+#define GNN_GRAPH_SCRATCH_BYTES (32L * 1024L * 1024L)
+int gnn_radius_graph_build_static(int N, const double* x, double radius, const double lo[3],
+                                  const double hi[3], int* row, long capacity, int* send,
+                                  int* recv, int* rev, long* n_edges, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
