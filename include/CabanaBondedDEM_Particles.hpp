@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "CabanaDEM_Core.hpp"
+#include "CabanaDEM_HDF5.hpp"
 
 namespace CabanaBondedDEM {
 
@@ -163,13 +164,40 @@ class ParticlesBondedDEM {
     if (dev_) dev_->reset();
   }
 
-  // src/CabanaBondedDEM_Particles.hpp:235-271 writes HDF5; this writes the
-  // same fields by their labels as <folder>/particles_<step>.csv (bond rows
-  // as MaxBonds columns each).
+  // src/CabanaBondedDEM_Particles.hpp:235-271.  With CABANADEM_ENABLE_HDF5
+  // this writes the reference's HDF5 step file, <folder>/particles_<step>.h5
+  // (+ .xmf), with the fields it passes to Cabana's writer (:244-267) under
+  // their slice labels and a "Time" attribute; without HDF5, the same fields
+  // by their labels as <folder>/particles_<step>.csv (bond rows as MaxBonds
+  // columns each).
   void output(int step, double time, bool use_reference = true) {
     (void)use_reference;
+    // a read-only access: host edits made before this call must still be
+    // uploaded by the next device call, so only this call's mark is undone
+    const bool touched = mirror_->host_touched;
     mirror_->before_host_access();
-    mirror_->host_touched = false; // read-only access
+    mirror_->host_touched = touched;
+#ifdef CABANADEM_ENABLE_HDF5
+    using CabanaDEM::detail::H5Field;
+    const std::vector<H5Field> fields = {
+        {"x", x_.data(), 3, false},          {"u", u_.data(), 3, false},
+        {"omega", omega_.data(), 3, false},  {"au", au_.data(), 3, false},
+        {"force", force_.data(), 3, false},  {"m", m_.data(), 1, false},
+        {"rho", rho_.data(), 1, false},      {"rad", rad_.data(), 1, false},
+        {"bond_fn_x", fn_x_.data(), MaxBonds, false}, {"bond_fn_y", fn_y_.data(), MaxBonds, false},
+        {"bond_fn_z", fn_z_.data(), MaxBonds, false}, {"bond_ft_x", ft_x_.data(), MaxBonds, false},
+        {"bond_ft_y", ft_y_.data(), MaxBonds, false}, {"bond_ft_z", ft_z_.data(), MaxBonds, false},
+        {"bond_mn_x", mn_x_.data(), MaxBonds, false}, {"bond_mn_y", mn_y_.data(), MaxBonds, false},
+        {"bond_mn_z", mn_z_.data(), MaxBonds, false}, {"bond_mt_x", mt_x_.data(), MaxBonds, false},
+        {"bond_mt_y", mt_y_.data(), MaxBonds, false}, {"bond_mt_z", mt_z_.data(), MaxBonds, false},
+        {"bond_idx", idx_.data(), MaxBonds, true},
+        {"bond_init_len", init_len_.data(), MaxBonds, false},
+        {"total_no_bonds", count_.data(), 1, true}};
+    if (!CabanaDEM::detail::write_h5_step(output_folder_ + "/particles", step, time, n_, fields))
+      std::cout << "particle output: writing " << output_folder_ << "/particles_" << step
+                << ".h5 failed" << std::endl;
+    return;
+#endif
     const std::string fn = output_folder_ + "/particles_" + std::to_string(step) + ".csv";
     std::ofstream os(fn);
     if (!os) {

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "CabanaDEM_Core.hpp"
+#include "CabanaDEM_HDF5.hpp"
 
 #ifndef DIM
 #define DIM 3
@@ -104,9 +105,13 @@ class Particles {
     if (dev_) dev_->reset();
   }
 
-  // src/CabanaDEM_Particles.hpp:299-346 writes HDF5 through Cabana; this
-  // writes the same fields (by their slice labels) as one CSV file per call:
-  // <output_folder>/particles_<output_step>.csv, with the time in the header.
+  // src/CabanaDEM_Particles.hpp:299-346.  With CABANADEM_ENABLE_HDF5 this
+  // writes the reference's HDF5 step file: <output_folder>/particles_<step>.h5
+  // with the fields it passes to Cabana's writer (:309-326), each under its
+  // slice label, a "Time" attribute, and the matching .xmf.  Without HDF5 it
+  // writes those fields (the per-contact histories aside) plus the torques
+  // as one CSV file, <output_folder>/particles_<step>.csv, the time in the
+  // header.
   void output(const int output_step, const double output_time, const bool use_reference = true) {
     (void)use_reference;
     // a read-only access: host edits made before this call must still be
@@ -114,6 +119,23 @@ class Particles {
     const bool touched = mirror_->host_touched;
     mirror_->before_host_access();
     mirror_->host_touched = touched;
+#ifdef CABANADEM_ENABLE_HDF5
+    using detail::H5Field;
+    const std::vector<H5Field> fields = {
+        {"positions", x_.data(), 3, false},       {"velocities", u_.data(), 3, false},
+        {"omega", omega_.data(), 3, false},       {"accelerations", au_.data(), 3, false},
+        {"forces", force_.data(), 3, false},      {"mass", m_.data(), 1, false},
+        {"density", rho_.data(), 1, false},       {"radius", rad_.data(), 1, false},
+        {"tangential_disp_ss_x", ss_x_.data(), MaxCnts, false},
+        {"tangential_disp_ss_y", ss_y_.data(), MaxCnts, false},
+        {"tangential_disp_ss_z", ss_z_.data(), MaxCnts, false},
+        {"tangential_disp_idx", ss_idx_.data(), MaxCnts, true},
+        {"total_no_tangential_contacts", ss_cnt_.data(), 1, true}};
+    if (!detail::write_h5_step(output_folder_name_ + "/particles", output_step, output_time, n_, fields))
+      std::cout << "particle output: writing " << output_folder_name_ << "/particles_" << output_step
+                << ".h5 failed" << std::endl;
+    return;
+#endif
     std::string fn = output_folder_name_ + "/particles_" + std::to_string(output_step) + ".csv";
     std::ofstream os(fn);
     if (!os) {
