@@ -80,9 +80,10 @@ def cpu_baseline(wl, settled, settle_steps, steps_cadence, steps_faithful):
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
     prm = O.params(dt=wl.dt, cor_pp=wl.cor, cor_pw=wl.cor, fr_pp=wl.friction,
                    fr_pw=wl.friction, gravity=wl.gravity, mode=0)
+    K = wl.rebuild_every if wl.rebuild_every > 0 else 20  # the oracle has fixed cadences only
     st = settled.copy()
     _, t1 = O.run(st, wl.walls, prm, nsteps=steps_cadence, rc=wl.cutoff, lo=wl.lo, hi=wl.hi,
-                  rebuild_every=wl.rebuild_every, max_n=64, threads=threads)
+                  rebuild_every=K, max_n=64, threads=threads)
     st = settled.copy()
     _, t2 = O.run(st, wl.walls, prm, nsteps=steps_faithful, rc=wl.cutoff, lo=wl.lo, hi=wl.hi,
                   rebuild_every=1, max_n=64, threads=threads)
@@ -94,7 +95,7 @@ def cpu_baseline(wl, settled, settle_steps, steps_cadence, steps_faithful):
         "kind": "port",
         "sample": (f"{n} spheres of the same workload from the GPU's state after the "
                    f"{settle_steps}-step settle: {steps_cadence} steps rebuilding every "
-                   f"{wl.rebuild_every} ({t1:.1f} s); rebuilding every step as the reference "
+                   f"{K} ({t1:.1f} s); rebuilding every step as the reference "
                    f"does: {n * steps_faithful / t2:.4g} particle-steps/s over {steps_faithful} "
                    f"steps ({t2:.1f} s)"),
         "value_rebuild_every_step": n * steps_faithful / t2,
@@ -142,6 +143,7 @@ def run_bonded(args):
     if args.warmup > 0:
         sysm.step(args.warmup)
     nrows0 = sysm.diag_history().shape[0]
+    checks0 = sysm.counters().displacement_checks
     sysm.set_timing(True)
     sysm.sync()
     t0 = time.perf_counter()
@@ -224,7 +226,7 @@ def run_ensemble(args):
     t0 = time.perf_counter()
     tune = sysm.autotune(trial_steps=80)
     log(f"[bench] autotune: rebuild every {tune.rebuild_every}, grid {tune.grid_blocks}, graph "
-        f"{tune.use_graph}, {tune.ms_per_step:.4f} ms/step ({tune.n_trials} trials, "
+        f"{tune.use_graph}, split {tune.split}, {tune.ms_per_step:.4f} ms/step ({tune.n_trials} trials, "
         f"{time.perf_counter() - t0:.1f} s)")
     # kernel duration and contact statistics: a short timed window, eager
     use_graph = bool(tune.use_graph)
@@ -259,7 +261,7 @@ def run_ensemble(args):
                    "spheres": n, "systems": len(wl.systems), "dt": wl.dt,
                    "autotuned": {"rebuild_every": tune.rebuild_every,
                                  "grid_blocks": tune.grid_blocks, "graph": tune.use_graph,
-                                 "trials": tune.trials()},
+                                 "split": tune.split, "trials": tune.trials()},
                    "physics_mode": "reference" if args.mode == 0 else "corrected",
                    "parallelism": "single GPU, one launch per step for all systems"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -432,8 +434,15 @@ def main():
     ap.add_argument("--dims", type=str, default=None)
     ap.add_argument("--settle", type=int, default=SETTLE_STEPS,
                     help="plain_large: settle steps before the timed window (part of the workload)")
-    ap.add_argument("--workload", choices=("plain_large", "bonded_block", "ensemble", "surrogate"),
+    ap.add_argument("--workload", choices=("plain_large", "plain_dense", "bonded_block", "ensemble",
+                                           "surrogate"),
                     default="plain_large")
+    ap.add_argument("--rebuild-every", type=int, default=None,
+                    help="plain workloads: neighbour rebuild cadence in steps, or -1 for the "
+                         "automatic cadence (default: the workload's, 20)")
+    ap.add_argument("--force-pass", choices=("fused", "split"), default="fused",
+                    help="plain and ensemble workloads: one fused force kernel, or the detection "
+                         "kernel followed by the force kernel (dem_set_force_pass)")
     ap.add_argument("--surrogate-traj", type=int, default=5)
     ap.add_argument("--surrogate-frames", type=int, default=300)
     ap.add_argument("--surrogate-steps", type=int, default=3000)
@@ -473,9 +482,13 @@ def main():
 
     dims = tuple(int(v) for v in args.dims.split(","))
     t0 = time.perf_counter()
-    wl = W.plain_large(dims=dims)
+    dense = args.workload == "plain_dense"
+    wl = W.plain_dense(dims=dims) if dense else W.plain_large(dims=dims)
+    if args.rebuild_every is not None:
+        wl.rebuild_every = args.rebuild_every
     n = wl.state.n
     sysm = wl.system(mode=args.mode)
+    sysm.set_force_pass(args.force_pass == "split")
     log(f"[bench] {n} spheres generated and uploaded in {time.perf_counter() - t0:.1f} s")
 
     t0 = time.perf_counter()
@@ -492,6 +505,7 @@ def main():
     sysm.set_diag_every(every)
     nrows0 = sysm.diag_history().shape[0]
 
+    checks0 = sysm.counters().displacement_checks
     sysm.set_timing(True)
     sysm.sync()
     t0 = time.perf_counter()
@@ -516,7 +530,9 @@ def main():
     b_own = W.bytes_fused_step(nbar, cbar, wbar)          # compulsory for the fused kernel
     achieved = b_alg * n / (k_ms * 1e-3) / 1e9
     achieved_own = b_own * n / (k_ms * 1e-3) / 1e9
-    b_ref = W.bytes_reference_model(nbar, cbar, wbar, wl.rebuild_every)
+    # the rebuild term at the cadence the timed window actually ran
+    k_eff = args.steps / max(launches[1], 1)
+    b_ref = W.bytes_reference_model(nbar, cbar, wbar, k_eff)
     traffic = measured_traffic(n)
 
     out = {
@@ -533,9 +549,11 @@ def main():
         "dtype": "f64",
         "data": "synthetic (jittered-lattice packing from a seeded counter-based generator)",
         "config": {
-            "workload": "plain DEM, large: %d polydisperse spheres (%dx%dx%d lattice), "
+            "workload": "plain DEM, %s: %d polydisperse spheres (%dx%dx%d lattice), "
                         "six-wall box, gravity, settle %d + warmup %d + timed %d steps"
-                        % (n, *dims, args.settle, args.warmup, args.steps),
+                        % ("dense (spacing 2.0 mm, r U[0.97, 1.03] mm)" if dense else "large",
+                           n, *dims, args.settle, args.warmup, args.steps),
+            "force_pass": args.force_pass,
             "settle_steps": args.settle, "diag_every": every, "diag_rows_timed": int(len(diag)),
             "spheres": n, "dt": wl.dt, "cutoff": wl.cutoff, "rebuild_every": wl.rebuild_every,
             "physics_mode": "reference" if args.mode == 0 else "corrected",
@@ -548,7 +566,9 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic["bytes_per_launch"] if traffic else None,
-            "kernel": "fused step kernel (forces + stage3 + next stage1/stage2)",
+            "kernel": ("fused step kernel (forces + stage3 + next stage1/stage2)"
+                       if args.force_pass == "fused" else
+                       "detection kernel + step kernel (forces + stage3 + next stage1/stage2)"),
             "algorithmic_bytes_per_particle": b_alg,
             "kernel_ms": k_ms,
             "launches": launches[0],
@@ -564,6 +584,8 @@ def main():
             "frac_of_hbm_reference_model": b_ref * value / (HBM_PEAK_GBS * 1e9),
             "rebuild_ms": r_ms,
             "rebuilds": launches[1],
+            "steps_per_rebuild": k_eff,
+            "displacement_checks": int(cnt.displacement_checks - checks0),
         },
         "contacts": {"mean_neighbors": nbar, "mean_contacts": cbar, "mean_wall_contacts": wbar,
                      "max_neighbors": cnt.max_neighbors_seen,

@@ -15,7 +15,7 @@ import numpy as np
 LIB_PATH = Path(os.environ.get("CABANADEM_LIB") or
                 Path(__file__).resolve().parent / "lib" / "libcabanadem_hip.so")
 
-DEM_ABI_VERSION = 3  # include/cabanadem.h DEM_ABI_VERSION
+DEM_ABI_VERSION = 4  # include/cabanadem.h DEM_ABI_VERSION
 DEM_MAX_WALLS = 8
 DEM_MAX_GROUPS = 8
 DEM_MAX_BONDS = 32
@@ -34,6 +34,7 @@ ERRORS = {
 }
 MODE_REFERENCE = 0
 MODE_CORRECTED = 1
+REBUILD_AUTO = -1  # include/cabanadem.h DEM_REBUILD_AUTO
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -66,6 +67,7 @@ class DemWallsView(C.Structure):
     _fields_ = [
         ("n", C.c_int), ("reserved", C.c_int),
         ("x", _dp), ("normal", _dp), ("E", _dp), ("nu", _dp), ("G", _dp),
+        ("u", _dp), ("omega", _dp),
     ]
 
 
@@ -75,6 +77,7 @@ class DemCounters(C.Structure):
         ("max_neighbors_seen", C.c_int), ("neighbor_capacity", C.c_int),
         ("mean_neighbors", C.c_double), ("mean_contacts", C.c_double),
         ("mean_wall_contacts", C.c_double), ("bonds_broken", C.c_int64),
+        ("displacement_checks", C.c_int64),
     ]
 
 
@@ -87,22 +90,25 @@ class DemSystemDesc(C.Structure):
     ]
 
 
-DEM_TUNE_MAX_TRIALS = 64
+DEM_TUNE_MAX_TRIALS = 128
 
 
 class DemTuneResult(C.Structure):
     _fields_ = [
         ("rebuild_every", C.c_int), ("grid_blocks", C.c_int), ("use_graph", C.c_int),
-        ("n_trials", C.c_int), ("ms_per_step", C.c_double),
+        ("split", C.c_int), ("n_trials", C.c_int), ("reserved", C.c_int),
+        ("ms_per_step", C.c_double),
         ("trial_rebuild_every", C.c_int * DEM_TUNE_MAX_TRIALS),
         ("trial_grid_blocks", C.c_int * DEM_TUNE_MAX_TRIALS),
         ("trial_graph", C.c_int * DEM_TUNE_MAX_TRIALS),
+        ("trial_split", C.c_int * DEM_TUNE_MAX_TRIALS),
         ("trial_ms_per_step", C.c_double * DEM_TUNE_MAX_TRIALS),
     ]
 
     def trials(self):
         return [dict(rebuild_every=self.trial_rebuild_every[k], grid_blocks=self.trial_grid_blocks[k],
-                     graph=self.trial_graph[k], ms_per_step=self.trial_ms_per_step[k])
+                     graph=self.trial_graph[k], split=self.trial_split[k],
+                     ms_per_step=self.trial_ms_per_step[k])
                 for k in range(self.n_trials)]
 
 
@@ -162,6 +168,7 @@ SYMBOLS = {
                                         C.c_double]),
     "dem_system_diagnostics": (C.c_int, [C.c_void_p, _dp]),
     "dem_set_launch": (C.c_int, [C.c_void_p, C.c_int]),
+    "dem_set_force_pass": (C.c_int, [C.c_void_p, C.c_int]),
     "dem_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "dem_autotune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(DemTuneResult)]),
     "dem_enable_bonds": (C.c_int, [C.c_void_p, C.POINTER(DemBondConfig)]),
@@ -273,12 +280,18 @@ class ParticleState:
 
 
 class Walls:
-    def __init__(self, x, normal, E, nu, G):
+    """Infinite planes: point x, unit normal, material; optional velocity u and
+    spin omega ([n][3], used in corrected mode only)."""
+
+    def __init__(self, x, normal, E, nu, G, u=None, omega=None):
         self.x = np.ascontiguousarray(np.asarray(x, dtype=np.float64).reshape(-1, 3))
         self.normal = np.ascontiguousarray(np.asarray(normal, dtype=np.float64).reshape(-1, 3))
         n = self.x.shape[0]
         f = lambda v: np.ascontiguousarray(np.broadcast_to(np.asarray(v, dtype=np.float64), (n,)))
         self.E, self.nu, self.G = f(E), f(nu), f(G)
+        v3 = lambda v: None if v is None else np.ascontiguousarray(
+            np.broadcast_to(np.asarray(v, dtype=np.float64), (n, 3)))
+        self.u, self.omega = v3(u), v3(omega)
 
     @property
     def n(self):
@@ -288,6 +301,8 @@ class Walls:
         v = DemWallsView()
         v.n = self.n
         v.x, v.normal, v.E, v.nu, v.G = (_ptr(a) for a in (self.x, self.normal, self.E, self.nu, self.G))
+        v.u = None if self.u is None else _ptr(self.u)
+        v.omega = None if self.omega is None else _ptr(self.omega)
         return v
 
 
@@ -458,6 +473,11 @@ class System:
 
     def set_launch(self, grid_blocks: int):
         self._check(self._lib.dem_set_launch(self._h, grid_blocks))
+
+    def set_force_pass(self, split: bool):
+        """Force passes as one fused kernel (False) or detection kernel +
+        force kernel (True); identical results."""
+        self._check(self._lib.dem_set_force_pass(self._h, 1 if split else 0))
 
     def set_params(self, **changes):
         """Non-structural dem_config fields (dt, cor_pp, ..., rebuild_every, physics_mode)."""

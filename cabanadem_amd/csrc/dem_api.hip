@@ -76,6 +76,21 @@ struct dem_ctx {
   // captured step cycles (dem_set_graph): valid while the launch arguments
   // (generation gen) and the buffer parity are those of the capture
   bool use_graph = false;
+  bool split = false;        // force passes as k_detect + force kernel (no bonds)
+  // translating walls (corrected mode): positions hold at step wall_step0;
+  // the force pass of step s sits at (s + 1 - wall_step0) dt
+  int64_t wall_step0 = 0;
+  bool walls_move = false;
+
+  // automatic rebuild cadence (rebuild_every = DEM_REBUILD_AUTO)
+  unsigned long long* d_disp = nullptr; // [2] max |x - xb|^2, max |u|^2 (bits)
+  unsigned long long* h_disp = nullptr; // pinned host copy of d_disp (a pageable
+                                        // destination costs a staged copy per check)
+  int auto_left = 0;         // force passes until the next displacement check
+  int auto_last = 0;         // force passes between the last two checks
+  int64_t auto_checks = 0;   // displacement checks so far
+  bool auto_due = false;     // rebuild before the next force pass
+  double auto_v = -1.;       // largest speed at the last check (< 0: none yet)
   bool capturing = false;
   hipGraphExec_t gexec = nullptr;
   int gen = 0, g_gen = -1, g_parity = -1, g_K = -1;
@@ -155,7 +170,8 @@ struct dem_ctx {
     }
     hipFree(nn); hipFree(nbr); hipFree(xb); hipFree(key); hipFree(rank); hipFree(key_sorted);
     hipFree(cell_count); hipFree(cell_start); hipFree(perm); hipFree(inv); hipFree(scan_tmp);
-    hipFree(flags); hipFree(diag_part); hipFree(diag_rows);
+    hipFree(flags); hipFree(diag_part); hipFree(diag_rows); hipFree(d_disp);
+    if (h_disp) hipHostFree(h_disp);
     hipFree(d_sys); hipFree(d_tiles); hipFree(d_tile0); hipFree(d_sysid); hipFree(d_grids);
     hipFree(d_kbase); hipFree(d_sysdiag);
     if (gexec) hipGraphExecDestroy(gexec);
@@ -210,6 +226,8 @@ struct dem_ctx {
       }
     }
     a.fext = fext[cS];
+    a.split = (split && !bonds_on) ? 1 : 0;
+    a.tw = (double)(step + 1 - wall_step0) * cfg.dt;
     return a;
   }
 
@@ -329,19 +347,69 @@ struct dem_ctx {
 
   // The rows must hold every pair that can touch before the next rebuild:
   // cutoff >= 2 r_max, plus the skin when the rows are reused for several
-  // steps (rebuild_every > 1, which needs a skin to be checked at all).
+  // steps (rebuild_every > 1 or automatic, which need a skin to be checked
+  // at all).
   int check_list_cover() {
     if (!pp_live() || max_rad <= 0.) return DEM_OK;
-    const int K = cfg.rebuild_every;
-    if (K > 1 && !(cfg.skin > 0.))
+    const bool reuse = cfg.rebuild_every != 1;
+    if (reuse && !(cfg.skin > 0.))
       return fail(DEM_ERR_ARG, "rebuild_every > 1 needs skin > 0: without it no displacement check "
                                "guards the contacts that form between rebuilds");
-    const double need = 2. * max_rad + (K > 1 ? cfg.skin : 0.);
+    const double need = 2. * max_rad + (reuse ? cfg.skin : 0.);
     if (cfg.cutoff < need * (1. - 1e-12))
       return fail(DEM_ERR_ARG, "cutoff " + std::to_string(cfg.cutoff) + " is below 2 r_max" +
-                               (K > 1 ? " + skin" : "") + " = " + std::to_string(need) +
+                               (reuse ? " + skin" : "") + " = " + std::to_string(need) +
                                ": contacts would be missed");
     return DEM_OK;
+  }
+
+  // Automatic rebuild cadence.  Rows built with cutoff >= 2 r_max + skin
+  // hold every pair that can touch as long as no particle has moved more
+  // than skin/2 since the build.  auto_check measures the largest
+  // displacement since the build, d, and the largest speed, v (one kernel,
+  // one synchronisation), and plans the next check M force passes ahead so
+  // that d + 2 v dt (M + 1) stays within 0.9 skin/2: speeds may double in
+  // the meantime, and a call boundary adds a drift.  M is also at most
+  // twice the previous interval, starting from one pass after a build, so
+  // that a check interval never exceeds the time since the build: under a
+  // constant acceleration from rest, v >= a t then bounds the a (M dt)^2 / 2
+  // that the factor 2 leaves out.  When not one more pass fits, the rows
+  // are rebuilt before the next force pass.  The rebuild's own skin check
+  // stays on behind this as the guard (a violent impulse can still trip it).
+  static constexpr int kAutoMaxInterval = 1000;
+  int auto_interval(double budget) const {
+    const double per_step = 2. * auto_v * cfg.dt;
+    if (!(budget > 0.)) return 0;
+    const int cap = std::min(kAutoMaxInterval, std::max(1, 2 * auto_last));
+    if (!(per_step > 0.)) return cap;
+    const double m = std::floor(budget / per_step) - 1.;
+    return m < 1. ? 0 : (m > cap ? cap : (int)m);
+  }
+  int auto_check() {
+    ++auto_checks;
+    hipError_t e = hipSuccess;
+    if (!h_disp) e = hipHostMalloc(reinterpret_cast<void**>(&h_disp), 2 * sizeof(unsigned long long),
+                                   hipHostMallocDefault);
+    if (e == hipSuccess) e = launch_disp_check(n, xr[cP], xb, uw[cP], d_disp, stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(h_disp, d_disp, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return hip_fail(e, "displacement check");
+    double d2, v2;
+    std::memcpy(&d2, &h_disp[0], sizeof(double));
+    std::memcpy(&v2, &h_disp[1], sizeof(double));
+    auto_v = std::sqrt(v2);
+    const int m = auto_interval(0.45 * cfg.skin - std::sqrt(d2)); // NaN: 0, rebuild
+    auto_due = m < 1;
+    auto_left = m;
+    auto_last = m;
+    return DEM_OK;
+  }
+  // after a rebuild: the first check one pass later
+  void auto_after_rebuild() {
+    auto_due = false;
+    auto_left = 1;
+    auto_last = 1;
   }
 
   // Materials and walls of every system to the device (synchronous).
@@ -435,6 +503,10 @@ int validate(const dem_config* c, std::string* msg) {
   if (c->physics_mode != DEM_MODE_REFERENCE && c->physics_mode != DEM_MODE_CORRECTED) {
     *msg = "physics_mode must be 0 or 1"; return DEM_ERR_ARG;
   }
+  if (c->rebuild_every == DEM_REBUILD_AUTO && !(c->skin > 0.)) {
+    *msg = "rebuild_every = DEM_REBUILD_AUTO needs skin > 0 (the displacement budget)";
+    return DEM_ERR_ARG;
+  }
 #define DEM_STR2(x) #x
 #define DEM_STR(x) DEM_STR2(x)
   if (c->block_size != 0 && c->block_size != kBlock) {
@@ -526,8 +598,18 @@ int set_walls(SysParams& P, const dem_walls_view* w, std::string* msg) {
     }
     W.a1 = (1. - (w->nu[j] * w->nu[j])) / w->E[j];
     W.a2 = (2. - w->nu[j]) / w->G[j];
+    for (int d = 0; d < 3; ++d) {
+      W.u[d] = w->u ? w->u[3 * j + d] : 0.;
+      W.w[d] = w->omega ? w->omega[3 * j + d] : 0.;
+    }
   }
   return DEM_OK;
+}
+
+bool walls_translate(const SysParams& P) {
+  for (int j = 0; j < P.n_walls; ++j)
+    if (P.walls[j].u[0] != 0. || P.walls[j].u[1] != 0. || P.walls[j].u[2] != 0.) return true;
+  return false;
 }
 
 // One system's configuration: the common fields plus its own count,
@@ -563,7 +645,7 @@ int create_ctx(const dem_config* common, int nsys, const dem_system_desc* sd, de
   dem_ctx* c = new dem_ctx();
   c->cfg = system_config(*common, sd[0]);
   c->cfg.n = (int)total;
-  if (c->cfg.rebuild_every <= 0) c->cfg.rebuild_every = 1;
+  if (c->cfg.rebuild_every <= 0 && c->cfg.rebuild_every != DEM_REBUILD_AUTO) c->cfg.rebuild_every = 1;
   c->n = (int)total;
   c->ld = (c->n + 63) / 64 * 64;
   c->max_c = common->max_contacts;
@@ -625,6 +707,7 @@ int create_ctx(const dem_config* common, int nsys, const dem_system_desc* sd, de
   c->inv = dalloc<int>(n, &e);
   c->scan_tmp = dalloc<int>((size_t)(c->ncell / 4096 + 2), &e);
   c->flags = dalloc<DevFlags>(1, &e);
+  c->d_disp = dalloc<unsigned long long>(2, &e);
   c->diag_part = dalloc<double>((size_t)c->ntiles * 8, &e);
   c->d_sys = dalloc<SysParams>((size_t)nsys, &e);
   c->d_tile0 = dalloc<int>((size_t)nsys + 1, &e);
@@ -715,7 +798,11 @@ int dem_set_params(dem_ctx* c, const dem_config* cfg) {
   c->cfg = want;
   c->cfg.max_neighbors = keep.max_neighbors;
   c->cfg.n_walls = keep.n_walls;
-  if (c->cfg.rebuild_every <= 0) c->cfg.rebuild_every = 1;
+  if (c->cfg.rebuild_every <= 0 && c->cfg.rebuild_every != DEM_REBUILD_AUTO) c->cfg.rebuild_every = 1;
+  if (c->cfg.rebuild_every == DEM_REBUILD_AUTO && !(c->cfg.skin > 0.)) {
+    c->cfg = keep;
+    return c->fail(DEM_ERR_ARG, "rebuild_every = DEM_REBUILD_AUTO needs skin > 0");
+  }
   ++c->gen;
   if (c->nsys == 1) {
     set_materials(c->h_sys[0], want.cor_pp, want.cor_pw, want.friction_pp, want.friction_pw);
@@ -750,6 +837,11 @@ int dem_upload_system_walls(dem_ctx* c, int system, const dem_walls_view* w) {
   std::string msg;
   if (int r = set_walls(c->h_sys[system], w, &msg)) return c->fail(r, msg);
   if (c->nsys == 1) c->cfg.n_walls = w->n;
+  // the uploaded positions hold now: translating walls move from here on
+  c->wall_step0 = c->step;
+  c->walls_move = false;
+  for (int s = 0; s < c->nsys; ++s) c->walls_move = c->walls_move || walls_translate(c->h_sys[s]);
+  ++c->gen;
   return c->upload_sys();
 }
 
@@ -780,6 +872,13 @@ int dem_set_launch(dem_ctx* c, int grid_blocks) {
   if (int r = check_ctx(c)) return r;
   if (grid_blocks < 0) return c->fail(DEM_ERR_ARG, "grid_blocks must be >= 0");
   c->grid_blocks = grid_blocks;
+  ++c->gen;
+  return DEM_OK;
+}
+
+int dem_set_force_pass(dem_ctx* c, int split) {
+  if (int r = check_ctx(c)) return r;
+  c->split = split != 0;
   ++c->gen;
   return DEM_OK;
 }
@@ -1078,15 +1177,17 @@ int dem_set_diag_every(dem_ctx* c, int every) {
 // diagnostic rows go to diag_rows[*row...].
 static int enqueue_steps(dem_ctx* c, int nsteps, int* row) {
   const int K = c->cfg.rebuild_every;
+  const bool autoK = K == DEM_REBUILD_AUTO;
   // stage1 + stage2 of the first step (later ones are fused into the force kernel)
   StepArgs a0 = c->step_args();
   hipError_t e = launch_kick(a0, c->xr[c->cP], c->uw[c->cP], c->ft[c->cS], 1, 1, c->stream);
   if (e != hipSuccess) return c->hip_fail(e, "stage1/2 launch");
   for (int s = 0; s < nsteps; ++s) {
     const int64_t gstep = c->step;
-    if (!c->list_valid || gstep % K == 0) {
+    if (!c->list_valid || (autoK ? c->auto_due : gstep % K == 0)) {
       int r = c->rebuild(!c->built_once, c->list_valid, false);
       if (r) return r;
+      if (autoK) c->auto_after_rebuild();
     }
     const bool sample = c->diag_every > 0 && gstep % c->diag_every == 0;
     const bool last = (s == nsteps - 1);
@@ -1103,6 +1204,8 @@ static int enqueue_steps(dem_ctx* c, int nsteps, int* row) {
       ++*row;
     }
     c->step++;
+    if (autoK && --c->auto_left <= 0)
+      if (int r = c->auto_check()) return r;
   }
   return DEM_OK;
 }
@@ -1168,7 +1271,11 @@ int dem_step(dem_ctx* c, int nsteps) {
   }
   int row = 0;
   hipError_t e;
-  const bool graphs = c->use_graph && c->diag_every == 0 && !c->timing && K < (1 << 20);
+  // captured launches carry fixed arguments, so steps whose walls translate
+  // (a new wall time per step) run eagerly
+  const bool moving = c->walls_move && c->cfg.physics_mode == DEM_MODE_CORRECTED;
+  const bool graphs = c->use_graph && c->diag_every == 0 && !c->timing && K > 0 && K < (1 << 20) &&
+                      !moving;
   if (!graphs) {
     if (int r = enqueue_steps(c, nsteps, &row)) return r;
   } else {
@@ -1240,7 +1347,10 @@ struct Snapshot {
   double last_sample[3];
   dem_config cfg;
   int grid_blocks, diag_every;
-  bool use_graph, timing;
+  bool use_graph, timing, split;
+  int auto_left, auto_last;
+  bool auto_due;
+  double auto_v;
 
   void add(void* p, size_t nbytes, hipError_t* e) {
     if (!p || nbytes == 0) return;
@@ -1292,7 +1402,9 @@ int take_snapshot(dem_ctx* c, Snapshot* s) {
   s->diag_rows = c->diag_hist.size();
   std::copy(c->last_sample, c->last_sample + 3, s->last_sample);
   s->cfg = c->cfg; s->grid_blocks = c->grid_blocks; s->diag_every = c->diag_every;
-  s->use_graph = c->use_graph; s->timing = c->timing;
+  s->use_graph = c->use_graph; s->timing = c->timing; s->split = c->split;
+  s->auto_left = c->auto_left; s->auto_last = c->auto_last; s->auto_due = c->auto_due;
+  s->auto_v = c->auto_v;
   if (e == hipSuccess) snapshot_buffers(c, s, &e);
   return e == hipSuccess ? DEM_OK : c->hip_fail(e, "autotune snapshot");
 }
@@ -1316,7 +1428,9 @@ int restore_snapshot(dem_ctx* c, const Snapshot& s) {
   c->diag_hist.resize(s.diag_rows);
   std::copy(s.last_sample, s.last_sample + 3, c->last_sample);
   c->cfg = s.cfg; c->grid_blocks = s.grid_blocks; c->diag_every = s.diag_every;
-  c->use_graph = s.use_graph; c->timing = s.timing;
+  c->use_graph = s.use_graph; c->timing = s.timing; c->split = s.split;
+  c->auto_left = s.auto_left; c->auto_last = s.auto_last; c->auto_due = s.auto_due;
+  c->auto_v = s.auto_v;
   ++c->gen;
   c->err.clear();
   return DEM_OK;
@@ -1342,25 +1456,31 @@ int dem_autotune(dem_ctx* c, int trial_steps, dem_tune_result* out) {
   }
   std::vector<int> Ks;
   const int K0 = snap.cfg.rebuild_every;
-  if (snap.cfg.skin > 0.) Ks = {5, 10, 20, 40, 80};
+  if (snap.cfg.skin > 0.) Ks = {5, 10, 20, 40, 80, DEM_REBUILD_AUTO};
   if (std::find(Ks.begin(), Ks.end(), K0) == Ks.end()) Ks.push_back(K0);
   const std::vector<int> grids = {0, 2 * cus, 4 * cus, 8 * cus};
+  // the split force pass exists for systems without bonds
+  const int n_split = c->bonds_on ? 1 : 2;
   double best = -1.;
   int nt = 0;
   for (int K : Ks)
     for (int g : grids)
-      for (int graph = 0; graph < 2; ++graph) {
+      for (int graph = 0; graph < 2; ++graph)
+      for (int split = 0; split < n_split; ++split) {
         if (nt >= DEM_TUNE_MAX_TRIALS) break;
         if (int r = restore_snapshot(c, snap)) return r;
         c->cfg.rebuild_every = K;
         c->grid_blocks = g;
         c->use_graph = graph != 0;
+        c->split = split != 0;
         c->diag_every = 0;
         c->timing = false;
         ++c->gen;
-        const int steps = ((std::max(trial_steps, 2 * K) + 2 * K - 1) / (2 * K)) * (2 * K);
+        // whole captured cycles of 2K steps (automatic cadence: periods of 40)
+        const int period = K > 0 ? 2 * K : 40;
+        const int steps = ((std::max(trial_steps, period) + period - 1) / period) * period;
         double ms = -1.;
-        int r = dem_step(c, 2 * K); // warm-up: builds, captures
+        int r = dem_step(c, period); // warm-up: builds, captures
         if (r == DEM_OK) {
           auto t0 = std::chrono::steady_clock::now();
           r = dem_step(c, steps);
@@ -1371,12 +1491,14 @@ int dem_autotune(dem_ctx* c, int trial_steps, dem_tune_result* out) {
         out->trial_rebuild_every[nt] = K;
         out->trial_grid_blocks[nt] = g;
         out->trial_graph[nt] = graph;
+        out->trial_split[nt] = split;
         out->trial_ms_per_step[nt] = ms;
         if (ms > 0. && (best < 0. || ms < best)) {
           best = ms;
           out->rebuild_every = K;
           out->grid_blocks = g;
           out->use_graph = graph;
+          out->split = split;
         }
         ++nt;
       }
@@ -1387,6 +1509,7 @@ int dem_autotune(dem_ctx* c, int trial_steps, dem_tune_result* out) {
   c->cfg.rebuild_every = out->rebuild_every;
   c->grid_blocks = out->grid_blocks;
   c->use_graph = out->use_graph != 0;
+  c->split = out->split != 0;
   ++c->gen;
   return DEM_OK;
 }
@@ -1437,6 +1560,7 @@ int dem_get_counters(dem_ctx* c, dem_counters* out) {
   out->mean_contacts = c->last_sample[1];
   out->mean_wall_contacts = c->last_sample[2];
   out->bonds_broken = f.bonds_broken;
+  out->displacement_checks = c->auto_checks;
   return DEM_OK;
 }
 

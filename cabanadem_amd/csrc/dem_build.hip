@@ -419,9 +419,47 @@ __global__ __launch_bounds__(kBlock) void k_setup_bonds(BondSetupArgs a) {
   a.nbond[i] = nb < a.max_b ? nb : a.max_b;
 }
 
+// Largest squared displacement since the last build (positions x against
+// the build's xb, both in the current slot order) and largest squared speed,
+// over all particles.  One atomic per wave and quantity: non-negative
+// doubles order like their bit patterns, so atomicMax on the bits is a max
+// on the values (a NaN lands above every finite value).
+__global__ __launch_bounds__(kBlock) void k_disp_check(int n, const double4* xr, const double* xb,
+                                                       const double* uw, unsigned long long* out) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  double d2 = 0., v2 = 0.;
+  if (i < n) {
+    const double4 p = xr[i];
+    const double d0 = p.x - xb[3 * i + 0], d1 = p.y - xb[3 * i + 1], dz = p.z - xb[3 * i + 2];
+    d2 = d0 * d0 + d1 * d1 + dz * dz;
+    const double* u = uw + 6l * i;
+    v2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    const double od = __shfl_xor(d2, off), ov = __shfl_xor(v2, off);
+    d2 = (od > d2 || od != od) ? od : d2;
+    v2 = (ov > v2 || ov != ov) ? ov : v2;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(&out[0], (unsigned long long)__double_as_longlong(d2));
+    atomicMax(&out[1], (unsigned long long)__double_as_longlong(v2));
+  }
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int count) {
+  if ((int)threadIdx.x < count) p[threadIdx.x] = 0ull;
+}
+
 inline unsigned nblk(long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 } // namespace
+
+hipError_t launch_disp_check(int n, const double4* xr, const double* xb, const double* uw,
+                             unsigned long long* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, out, 2);
+  hipLaunchKernelGGL(k_disp_check, dim3(nblk(n)), dim3(kBlock), 0, s, n, xr, xb, uw, out);
+  return hipGetLastError();
+}
 
 hipError_t launch_setup_bonds(const BondSetupArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_setup_bonds, dim3(nblk(a.n)), dim3(kBlock), 0, s, a);

@@ -33,6 +33,13 @@ namespace {
 #define DEM_FORCE_MIN_WAVES 1
 #endif
 constexpr int kPhase1Batch = DEM_PHASE1_BATCH; // neighbour gathers issued together
+#ifndef DEM_DETECT_BATCH
+#define DEM_DETECT_BATCH 4
+#endif
+#ifndef DEM_PRE_MIN_WAVES
+#define DEM_PRE_MIN_WAVES 1
+#endif
+constexpr int kDetectBatch = DEM_DETECT_BATCH; // the same in k_detect
 
 __device__ inline void load6(const double* p, double* v) {
   const double2* q = reinterpret_cast<const double2*>(p);
@@ -141,9 +148,10 @@ __device__ inline void pp_contact(const double4& xi, const double* ui, const dou
 
 // Particle-wall contact (overlap > 0 already established).  sw is the
 // wall tangential history, updated in place.  src/CabanaDEM_Force.hpp.
+// wx: the wall point at this force pass (moved in corrected mode).
 template <bool REF>
 __device__ inline void pw_contact(const double4& xi, const double* ui, const double4& mi,
-                                  const WallConst& W, double overlap, double* sw,
+                                  const WallConst& W, const double* wx, double overlap, double* sw,
                                   double* F, double* T, const StepArgs& a, const SysParams& P) {
   const double nx = W.n[0], ny = W.n[1], nz = W.n[2];               // :509-511
   const double a_i = xi.w - overlap / 2.;                           // :517
@@ -152,10 +160,18 @@ __device__ inline void pw_contact(const double4& xi, const double* ui, const dou
     v0 = ui[0] + (ui[4] * nz - ui[5] * ny) * a_i;                   // :520-527
     v1 = ui[1] + (ui[5] * nx - ui[3] * nz) * a_i;
     v2 = ui[2] + (ui[3] * ny - ui[4] * nx) * a_i;
+    // :529-535: the wall's velocity is taken as 0 (Q7)
   } else {
     v0 = ui[0] - (ui[4] * nz - ui[5] * ny) * a_i;
     v1 = ui[1] - (ui[5] * nx - ui[3] * nz) * a_i;
     v2 = ui[2] - (ui[3] * ny - ui[4] * nx) * a_i;
+    // minus the wall surface's velocity at the contact point c = x_i - a_i n
+    const double c0 = xi.x - a_i * nx - wx[0];
+    const double c1 = xi.y - a_i * ny - wx[1];
+    const double c2 = xi.z - a_i * nz - wx[2];
+    v0 -= W.u[0] + (W.w[1] * c2 - W.w[2] * c1);
+    v1 -= W.u[1] + (W.w[2] * c0 - W.w[0] * c2);
+    v2 -= W.u[2] + (W.w[0] * c1 - W.w[1] * c0);
   }
   const double vdn = v0 * nx + v1 * ny + v2 * nz;                   // :538
   const double vn0 = vdn * nx, vn1 = vdn * ny, vn2 = vdn * nz;      // :539-541
@@ -377,7 +393,7 @@ __device__ inline void block_reduce8(const double* v, double* part, int blk) {
 // One tile: kBlock consecutive slots of one system.  ENS: an ensemble (tile
 // table and per-system parameters from the device); otherwise one system
 // whose parameters travel in the kernel arguments.
-template <bool REF, bool PP, bool PW, int KICK, bool ACCUM, bool DIAG, bool BOND, bool ENS>
+template <bool REF, bool PP, bool PW, int KICK, bool ACCUM, bool DIAG, bool BOND, bool ENS, bool PRE>
 __device__ __forceinline__ void force_tile(const StepArgs& a, const int tile, int* s_cj) {
   const int tid = threadIdx.x;
   int sys = 0, first = tile * kBlock, end = a.n;
@@ -436,7 +452,12 @@ __device__ __forceinline__ void force_tile(const StepArgs& a, const int tile, in
     const int nbk = a.nbond ? (BOND ? nb : a.nbond[i]) : 0;
     const bool pp_on = !a.nbond || a.unbonded_contacts;
     if (PP && !pp_on) nc = 0;
-    if (PP && pp_on) {
+    if (PP && pp_on && PRE) {
+      // Phase 1 ran in k_detect: the count is in cmeta_out, the ids (ascending)
+      // already in cj_out.
+      if (DIAG) nnb = a.nn[i];
+      nc = meta_count(a.cmeta_out[i]);
+    } else if (PP && pp_on) {
       // Phase 1: contact detection over the neighbour row.
       nnb = a.nn[i];
       nc = 0;
@@ -471,13 +492,15 @@ __device__ __forceinline__ void force_tile(const StepArgs& a, const int tile, in
         atomicMax(&a.flags->contact_overflow, nc);
         nc = a.max_c;
       }
+    }
+    if (PP && pp_on) {
       // Phase 2: contact arithmetic, merged with last step's tracked list.
       const int oc = meta_count(meta);
       int p = 0;
       int jo = oc > 0 ? a.cj_in[i] : INT_MAX;
       if (nc > 0) { mti = a.mat[i]; have_mti = true; }
       for (int m = 0; m < nc; ++m) {
-        const int j = s_cj[m * kBlock + tid];
+        const int j = PRE ? a.cj_out[m * ld + i] : s_cj[m * kBlock + tid];
         while (jo < j) {
           ++p;
           jo = (p < oc) ? a.cj_in[p * ld + i] : INT_MAX;
@@ -493,7 +516,7 @@ __device__ __forceinline__ void force_tile(const StepArgs& a, const int tile, in
         load6(a.uw_in + 6l * j, uj);
         const double4 mj = a.mat[j];
         pp_contact<REF>(xi, ui, mti, xj, uj, mj, h, F, T, a, P);
-        a.cj_out[m * ld + i] = j;
+        if (!PRE) a.cj_out[m * ld + i] = j;
         a.ch_out[(3 * m + 0) * ld + i] = h[0];
         a.ch_out[(3 * m + 1) * ld + i] = h[1];
         a.ch_out[(3 * m + 2) * ld + i] = h[2];
@@ -505,7 +528,11 @@ __device__ __forceinline__ void force_tile(const StepArgs& a, const int tile, in
       wn = 0;
       for (int w = 0; w < P.n_walls; ++w) {
         const WallConst& W = P.walls[w];
-        const double q0 = xi.x - W.x[0], q1 = xi.y - W.x[1], q2 = xi.z - W.x[2]; // :497-499
+        // the reference never moves a wall; corrected mode translates it
+        double wx[3] = {W.x[0], W.x[1], W.x[2]};
+        if (!REF)
+          for (int d = 0; d < 3; ++d) wx[d] += W.u[d] * a.tw;
+        const double q0 = xi.x - wx[0], q1 = xi.y - wx[1], q2 = xi.z - wx[2];    // :497-499
         const double tmp = q0 * W.n[0] + q1 * W.n[1] + q2 * W.n[2];              // :514
         const double overlap = xi.w - tmp;                                        // :515
         if (overlap > 0.) {                                                       // :556
@@ -516,7 +543,7 @@ __device__ __forceinline__ void force_tile(const StepArgs& a, const int tile, in
             sw[1] = a.sw[(3 * w + 1) * ld + i];
             sw[2] = a.sw[(3 * w + 2) * ld + i];
           }
-          pw_contact<REF>(xi, ui, mti, W, overlap, sw, F, T, a, P);
+          pw_contact<REF>(xi, ui, mti, W, wx, overlap, sw, F, T, a, P);
           a.sw[(3 * w + 0) * ld + i] = sw[0];
           a.sw[(3 * w + 1) * ld + i] = sw[1];
           a.sw[(3 * w + 2) * ld + i] = sw[2];
@@ -583,19 +610,80 @@ __device__ __forceinline__ void force_tile(const StepArgs& a, const int tile, in
 // The fused force / step kernel.  A grid of one workgroup per tile, or a
 // persistent grid that strides over the tiles; either way tile v is mapped
 // XCD-aware (contiguous tiles per XCD when the grid is a multiple of 8).
-template <bool REF, bool PP, bool PW, int KICK, bool ACCUM, bool DIAG, bool BOND, bool ENS>
-__global__ __launch_bounds__(kBlock, DEM_FORCE_MIN_WAVES) void k_force(StepArgs a) {
+template <bool REF, bool PP, bool PW, int KICK, bool ACCUM, bool DIAG, bool BOND, bool ENS, bool PRE>
+__global__ __launch_bounds__(kBlock, PRE ? DEM_PRE_MIN_WAVES : DEM_FORCE_MIN_WAVES) void k_force(StepArgs a) {
   extern __shared__ int s_cj[]; // [max_c][kBlock] contacts found in phase 1
   if ((int)gridDim.x >= a.ntiles) {
     // one workgroup per tile: no loop, so the compiler keeps the kernel
     // arguments where they are used instead of live across a loop
-    force_tile<REF, PP, PW, KICK, ACCUM, DIAG, BOND, ENS>(a, xcd_block(blockIdx.x, a.ntiles), s_cj);
+    force_tile<REF, PP, PW, KICK, ACCUM, DIAG, BOND, ENS, PRE>(a, xcd_block(blockIdx.x, a.ntiles), s_cj);
     return;
   }
   for (int v = blockIdx.x; v < a.ntiles; v += gridDim.x) {
-    force_tile<REF, PP, PW, KICK, ACCUM, DIAG, BOND, ENS>(a, xcd_block(v, a.ntiles), s_cj);
+    force_tile<REF, PP, PW, KICK, ACCUM, DIAG, BOND, ENS, PRE>(a, xcd_block(v, a.ntiles), s_cj);
     if (DIAG) __syncthreads(); // the reduction's LDS is reused by the next tile
   }
+}
+
+// Phase 1 of the force pass on its own (the split force pass): for each
+// particle, the neighbours j of its row that overlap it
+// (src/CabanaDEM_Force.hpp:107,162), ascending in j, into cj_out and their
+// count into cmeta_out (wall bits 0: the force kernel writes the final
+// word).  The same test and order as phase 1 of force_tile, so the lists
+// are identical; this kernel holds no FP64 contact arithmetic, so it runs at
+// a much higher occupancy than the fused kernel, and the force kernel that
+// follows (k_force with PRE) only walks the short lists.  Systems without
+// bonds only.
+template <bool ENS>
+__device__ __forceinline__ void detect_tile(const StepArgs& a, const int tile) {
+  const int tid = threadIdx.x;
+  int first = tile * kBlock, end = a.n;
+  if (ENS) {
+    const int4 t = a.tiles[tile];
+    first = t.y; end = t.z;
+  }
+  const int i = first + tid;
+  if (i >= end) return;
+  const long ld = a.ld;
+  const double4 xi = a.xr_in[i];
+  const int nnb = a.nn[i];
+  int nc = 0;
+  for (int k0 = 0; k0 < nnb; k0 += kDetectBatch) {
+    int jb[kDetectBatch];
+    double4 xb[kDetectBatch];
+#pragma unroll
+    for (int q = 0; q < kDetectBatch; ++q)
+      jb[q] = (k0 + q < nnb) ? a.nbr[(k0 + q) * ld + i] : i;
+#pragma unroll
+    for (int q = 0; q < kDetectBatch; ++q) xb[q] = a.xr_in[jb[q]];
+#pragma unroll
+    for (int q = 0; q < kDetectBatch; ++q) {
+      if (k0 + q >= nnb) break;
+      const double4 xj = xb[q];
+      const double d0 = xi.x - xj.x, d1 = xi.y - xj.y, d2 = xi.z - xj.z;
+      const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+      const double s = xi.w + xj.w;
+      // as in force_tile: the margin never filters out a contact
+      if (r2 < s * s * (1. + 1e-12) && s - sqrt(r2) > 0.) {
+        if (nc < a.max_c) a.cj_out[nc * ld + i] = jb[q];
+        ++nc;
+      }
+    }
+  }
+  if (nc > a.max_c) {
+    atomicMax(&a.flags->contact_overflow, nc);
+    nc = a.max_c;
+  }
+  a.cmeta_out[i] = make_meta(nc, 0);
+}
+
+template <bool ENS>
+__global__ __launch_bounds__(kBlock, 8) void k_detect(StepArgs a) {
+  if ((int)gridDim.x >= a.ntiles) {
+    detect_tile<ENS>(a, xcd_block(blockIdx.x, a.ntiles));
+    return;
+  }
+  for (int v = blockIdx.x; v < a.ntiles; v += gridDim.x) detect_tile<ENS>(a, xcd_block(v, a.ntiles));
 }
 
 // F = m g + f_ext, T = 0  (src/CabanaDEM_Force.hpp:39-47 plus the body and
@@ -754,36 +842,66 @@ __global__ __launch_bounds__(kBlock) void k_diag_final(const double* part, int n
 
 inline unsigned nblk(long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
+// The split force pass: k_detect, then the force kernel over its lists.
+// Whole force passes of systems without bonds only; the result is bitwise
+// that of the fused kernel.
+template <bool REF, bool ENS>
+hipError_t launch_split(ForceKind kind, const StepArgs& a, bool sample, const dim3& g, hipStream_t s) {
+  const dim3 b(kBlock);
+  hipLaunchKernelGGL(k_detect<ENS>, g, b, 0, s, a);
+  switch (kind) {
+    case FK_PP_PW_FORCES:
+      if (sample) hipLaunchKernelGGL((k_force<REF, true, true, 0, false, true, false, ENS, true>), g, b, 0, s, a);
+      else hipLaunchKernelGGL((k_force<REF, true, true, 0, false, false, false, ENS, true>), g, b, 0, s, a);
+      break;
+    case FK_STEP_FUSED:
+      if (sample) hipLaunchKernelGGL((k_force<REF, true, true, 2, false, true, false, ENS, true>), g, b, 0, s, a);
+      else hipLaunchKernelGGL((k_force<REF, true, true, 2, false, false, false, ENS, true>), g, b, 0, s, a);
+      break;
+    case FK_STEP_LAST:
+      if (sample) hipLaunchKernelGGL((k_force<REF, true, true, 1, false, true, false, ENS, true>), g, b, 0, s, a);
+      else hipLaunchKernelGGL((k_force<REF, true, true, 1, false, false, false, ENS, true>), g, b, 0, s, a);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 template <bool REF, bool BOND, bool ENS>
 hipError_t launch_force_mode(ForceKind kind, const StepArgs& a, bool sample, int grid_blocks,
                              hipStream_t s) {
   const size_t lds = (size_t)(a.max_c > 0 ? a.max_c : 1) * kBlock * sizeof(int);
   const int nb = (grid_blocks > 0 && grid_blocks < a.ntiles) ? grid_blocks : a.ntiles;
   const dim3 g(nb), b(kBlock);
+  if constexpr (!BOND) {
+    if (a.split && (kind == FK_PP_PW_FORCES || kind == FK_STEP_FUSED || kind == FK_STEP_LAST))
+      return launch_split<REF, ENS>(kind, a, sample, g, s);
+  }
   switch (kind) {
     case FK_PP_PW_FORCES:
-      if (sample) hipLaunchKernelGGL((k_force<REF, true, true, 0, false, true, BOND, ENS>), g, b, lds, s, a);
-      else hipLaunchKernelGGL((k_force<REF, true, true, 0, false, false, BOND, ENS>), g, b, lds, s, a);
+      if (sample) hipLaunchKernelGGL((k_force<REF, true, true, 0, false, true, BOND, ENS, false>), g, b, lds, s, a);
+      else hipLaunchKernelGGL((k_force<REF, true, true, 0, false, false, BOND, ENS, false>), g, b, lds, s, a);
       break;
     case FK_PP_ACCUM:
-      hipLaunchKernelGGL((k_force<REF, true, false, 0, true, false, false, ENS>), g, b, lds, s, a);
+      hipLaunchKernelGGL((k_force<REF, true, false, 0, true, false, false, ENS, false>), g, b, lds, s, a);
       break;
     case FK_PW_ACCUM:
-      hipLaunchKernelGGL((k_force<REF, false, true, 0, true, false, false, ENS>), g, b, lds, s, a);
+      hipLaunchKernelGGL((k_force<REF, false, true, 0, true, false, false, ENS, false>), g, b, lds, s, a);
       break;
     case FK_BOND_ACCUM:
       if constexpr (!ENS)
-        hipLaunchKernelGGL((k_force<REF, false, false, 0, true, false, true, false>), g, b, lds, s, a);
+        hipLaunchKernelGGL((k_force<REF, false, false, 0, true, false, true, false, false>), g, b, lds, s, a);
       else
         return hipErrorInvalidValue;
       break;
     case FK_STEP_FUSED:
-      if (sample) hipLaunchKernelGGL((k_force<REF, true, true, 2, false, true, BOND, ENS>), g, b, lds, s, a);
-      else hipLaunchKernelGGL((k_force<REF, true, true, 2, false, false, BOND, ENS>), g, b, lds, s, a);
+      if (sample) hipLaunchKernelGGL((k_force<REF, true, true, 2, false, true, BOND, ENS, false>), g, b, lds, s, a);
+      else hipLaunchKernelGGL((k_force<REF, true, true, 2, false, false, BOND, ENS, false>), g, b, lds, s, a);
       break;
     case FK_STEP_LAST:
-      if (sample) hipLaunchKernelGGL((k_force<REF, true, true, 1, false, true, BOND, ENS>), g, b, lds, s, a);
-      else hipLaunchKernelGGL((k_force<REF, true, true, 1, false, false, BOND, ENS>), g, b, lds, s, a);
+      if (sample) hipLaunchKernelGGL((k_force<REF, true, true, 1, false, true, BOND, ENS, false>), g, b, lds, s, a);
+      else hipLaunchKernelGGL((k_force<REF, true, true, 1, false, false, BOND, ENS, false>), g, b, lds, s, a);
       break;
   }
   return hipGetLastError();

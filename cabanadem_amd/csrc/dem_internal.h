@@ -59,6 +59,10 @@ struct WallConst {
   double n[3];
   double a1; // (1 - nu_w^2) / E_w
   double a2; // (2 - nu_w) / G_w
+  // corrected mode: the plane translates with u (x at wall time 0) and its
+  // surface moves with u + w x (c - x_w) at a contact point c
+  double u[3];
+  double w[3];
 };
 
 struct Grid {
@@ -160,6 +164,12 @@ struct StepArgs {
   int vgkind[DEM_MAX_GROUPS];
   double vgv[DEM_MAX_GROUPS][3];
   const double* fext;    // [n][3] constant external force, or null
+  // 1: whole force passes run split, k_detect then the force kernel over its
+  // contact lists (systems without bonds); 0: one fused kernel
+  int split;
+  // time of this force pass since the walls' positions were set (moving
+  // walls, corrected mode)
+  double tw;
 };
 
 struct BuildArgs {
@@ -228,6 +238,10 @@ hipError_t launch_setup_bonds(const BondSetupArgs& a, hipStream_t s);
 hipError_t launch_rebuild(const BuildArgs& a, hipStream_t s);
 hipError_t launch_neighbors(const BuildArgs& a, hipStream_t s);
 hipError_t launch_fill_int(int* p, long count, int value, hipStream_t s);
+// out[0] = bits of max |x - xb|^2, out[1] = bits of max |u|^2 over n slots
+// (cleared first; read back as doubles).
+hipError_t launch_disp_check(int n, const double4* xr, const double* xb, const double* uw,
+                             unsigned long long* out, hipStream_t s);
 
 enum ForceKind {
   FK_PP_PW_FORCES = 0, // zero + update + pp + pw, write F,T

@@ -1,11 +1,19 @@
-// hbm_copy.hip -- streaming copy of a known byte count.
+// hbm_copy.hip -- streaming kernels of a known byte count.
 //
-// Prints the achieved copy bandwidth and gives the HBM counters
-// (FETCH_SIZE / WRITE_SIZE) a kernel of exactly known traffic, so that the
-// counters read for the DEM kernels can be scaled to bytes.
+// Gives the HBM counters (FETCH_SIZE / WRITE_SIZE) a kernel of exactly known
+// traffic, so that the counters read for the DEM kernels can be scaled to
+// bytes, and measures the practical bandwidth ceiling of this GPU:
+//   k_stream_copy      grid-stride double4 copy (the counter calibration)
+//   k_copy_unrolled    each thread keeps U double4 loads in flight before
+//                      storing them (lane-coalesced, U blocks apart)
+//   k_copy_unrolled_nt the same with non-temporal stores
+//   k_read_sum         read-only: U loads in flight, one double per thread out
+// Each variant runs `repeats` times on buffers of `MiB` each; the JSON line
+// gives every variant's GB/s and the best copy.
 //   hbm_copy [MiB per buffer, default 1024] [repeats, default 20]
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 
@@ -16,35 +24,113 @@ __global__ __launch_bounds__(256) void k_stream_copy(const double4* __restrict__
   for (; i < n; i += stride) dst[i] = src[i];
 }
 
+template <int U, bool NT>
+__global__ __launch_bounds__(256) void k_copy_unrolled(const double4* __restrict__ src,
+                                                       double4* __restrict__ dst, long n) {
+  const long span = (long)blockDim.x * U;
+  for (long base = (long)blockIdx.x * span; base < n; base += (long)gridDim.x * span) {
+    double4 v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long i = base + (long)u * blockDim.x + threadIdx.x;
+      if (i < n) v[u] = src[i];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long i = base + (long)u * blockDim.x + threadIdx.x;
+      if (i < n) {
+        if (NT) {
+          __builtin_nontemporal_store(v[u].x, &dst[i].x);
+          __builtin_nontemporal_store(v[u].y, &dst[i].y);
+          __builtin_nontemporal_store(v[u].z, &dst[i].z);
+          __builtin_nontemporal_store(v[u].w, &dst[i].w);
+        } else {
+          dst[i] = v[u];
+        }
+      }
+    }
+  }
+}
+
+template <int U>
+__global__ __launch_bounds__(256) void k_read_sum(const double4* __restrict__ src, double* __restrict__ out,
+                                                  long n) {
+  const long span = (long)blockDim.x * U;
+  double s = 0.;
+  for (long base = (long)blockIdx.x * span; base < n; base += (long)gridDim.x * span) {
+    double4 v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long i = base + (long)u * blockDim.x + threadIdx.x;
+      v[u] = i < n ? src[i] : make_double4(0., 0., 0., 0.);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) s += (v[u].x + v[u].y) + (v[u].z + v[u].w);
+  }
+  out[(long)blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+template <class Launch>
+static double time_ms(Launch launch, int reps) {
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  launch();
+  hipDeviceSynchronize();
+  hipEventRecord(e0, 0);
+  for (int r = 0; r < reps; ++r) launch();
+  hipEventRecord(e1, 0);
+  hipEventSynchronize(e1);
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  return ms / reps;
+}
+
 int main(int argc, char** argv) {
   const long mib = argc > 1 ? std::atol(argv[1]) : 1024;
   const int reps = argc > 2 ? std::atoi(argv[2]) : 20;
   const long bytes = mib << 20;
   const long n = bytes / (long)sizeof(double4);
+  int dev = 0, cus = 256;
+  hipGetDevice(&dev);
+  hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   double4 *a = nullptr, *b = nullptr;
-  if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) {
+  double* sums = nullptr;
+  const int grid = cus * 8; // eight workgroups of 256 threads per CU
+  if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess ||
+      hipMalloc(&sums, (size_t)grid * 256 * sizeof(double)) != hipSuccess) {
     std::fprintf(stderr, "hipMalloc failed\n");
     return 1;
   }
   hipMemset(a, 0, bytes);
   hipMemset(b, 0, bytes);
-  const int blocks = 256 * 8 * 4; // 4 blocks per SIMD group on 256 CUs
-  hipEvent_t e0, e1;
-  hipEventCreate(&e0);
-  hipEventCreate(&e1);
-  hipLaunchKernelGGL(k_stream_copy, dim3(blocks), dim3(256), 0, 0, a, b, n);
-  hipDeviceSynchronize();
-  hipEventRecord(e0, 0);
-  for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_stream_copy, dim3(blocks), dim3(256), 0, 0, a, b, n);
-  hipEventRecord(e1, 0);
-  hipEventSynchronize(e1);
-  float ms = 0.f;
-  hipEventElapsedTime(&ms, e0, e1);
-  const double per = ms / reps;
+  const double gs = time_ms([&] {
+    hipLaunchKernelGGL(k_stream_copy, dim3(cus * 32), dim3(256), 0, 0, a, b, n);
+  }, reps);
+  const double u4 = time_ms([&] {
+    hipLaunchKernelGGL((k_copy_unrolled<4, false>), dim3(grid), dim3(256), 0, 0, a, b, n);
+  }, reps);
+  const double u8 = time_ms([&] {
+    hipLaunchKernelGGL((k_copy_unrolled<8, false>), dim3(grid), dim3(256), 0, 0, a, b, n);
+  }, reps);
+  const double u4nt = time_ms([&] {
+    hipLaunchKernelGGL((k_copy_unrolled<4, true>), dim3(grid), dim3(256), 0, 0, a, b, n);
+  }, reps);
+  const double rd = time_ms([&] {
+    hipLaunchKernelGGL((k_read_sum<8>), dim3(grid), dim3(256), 0, 0, a, sums, n);
+  }, reps);
+  auto copy_gbps = [&](double ms) { return 2.0 * bytes / (ms * 1e-3) / 1e9; };
+  const double best = std::max(std::max(copy_gbps(gs), copy_gbps(u4)), std::max(copy_gbps(u8), copy_gbps(u4nt)));
   std::printf("{\"bytes_read_per_launch\": %ld, \"bytes_written_per_launch\": %ld, "
-              "\"ms_per_launch\": %.6f, \"copy_GBps\": %.1f}\n",
-              bytes, bytes, per, 2.0 * bytes / (per * 1e-3) / 1e9);
+              "\"ms_per_launch\": %.6f, \"copy_GBps\": %.1f, \"copy_unrolled4_GBps\": %.1f, "
+              "\"copy_unrolled8_GBps\": %.1f, \"copy_unrolled4_nt_GBps\": %.1f, \"read_GBps\": %.1f, "
+              "\"best_copy_GBps\": %.1f, \"compute_units\": %d}\n",
+              bytes, bytes, gs, copy_gbps(gs), copy_gbps(u4), copy_gbps(u8), copy_gbps(u4nt),
+              bytes / (rd * 1e-3) / 1e9, best, cus);
   hipFree(a);
   hipFree(b);
+  hipFree(sums);
   return 0;
 }

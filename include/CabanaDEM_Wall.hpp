@@ -1,7 +1,11 @@
 // CabanaDEM_Wall.hpp -- infinite-plane walls with the reference's API
 // (src/CabanaDEM_Wall.hpp:22-209): position, velocity, omega, normal and
-// material per wall.  As in the reference, wall velocity and omega are
-// carried but not read by the contact force (src/CabanaDEM_Force.hpp:468-469).
+// material per wall.  In reference mode, as in the reference, wall velocity
+// and omega are carried but not used by the contact force
+// (src/CabanaDEM_Force.hpp:468-469,529).  In corrected mode a wall
+// translates with its velocity from the position it has when the solver
+// uploads it, and its surface moves with u + omega x (c - x_w) at a contact
+// point c; the positions in this container stay those of the upload.
 #pragma once
 
 #include <memory>
@@ -62,6 +66,8 @@ class Wall {
     w.E = E_.data();
     w.nu = nu_.data();
     w.G = G_.data();
+    w.u = u_.data();
+    w.omega = omega_.data();
     return w;
   }
 

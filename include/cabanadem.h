@@ -35,7 +35,7 @@ extern "C" {
 #define DEM_MAX_WALLS 8
 #define DEM_MAX_GROUPS 8
 #define DEM_MAX_BONDS 32
-#define DEM_ABI_VERSION 3
+#define DEM_ABI_VERSION 4
 
 enum dem_status {
   DEM_OK = 0,
@@ -50,6 +50,20 @@ enum dem_status {
 };
 
 enum dem_physics_mode { DEM_MODE_REFERENCE = 0, DEM_MODE_CORRECTED = 1 };
+
+/* dem_config.rebuild_every: automatic cadence.  Every few steps the library
+ * measures the largest displacement since the last build, d, and the
+ * largest speed, v (one small kernel and one synchronisation), and plans the
+ * next measurement M steps ahead so that d + 2 v dt (M + 1) <= 0.9 skin/2,
+ * with M at most twice the previous interval (one step after a build);
+ * when no step fits, the rows are rebuilt before the next force pass.  The
+ * rows (cutoff >= 2 r_max + skin) then hold every pair that can touch as
+ * long as speeds do not more than double between two measurements.  The
+ * rebuild's skin check (DEM_ERR_SKIN) stays on as the guard against
+ * anything faster.  Captured graphs are not used with this cadence.  No
+ * counterpart in the reference, which rebuilds every step
+ * (src/CabanaDEM_Solver.hpp:87-88). */
+#define DEM_REBUILD_AUTO (-1)
 
 /* Kinds of particle group (dem_set_groups). */
 enum dem_group_kind {
@@ -78,7 +92,9 @@ typedef struct dem_config {
   double cutoff;       /* neighbour cutoff (the reference's `delta`) */
   double skin;         /* > 0: flag DEM_ERR_SKIN if a particle moves > skin/2 between rebuilds */
   double lo[3], hi[3]; /* binning box (mesh_lo / mesh_hi); outside particles are clamped in */
-  int rebuild_every;   /* neighbour rebuild cadence K in steps (1 = every step, as the reference) */
+  int rebuild_every;   /* neighbour rebuild cadence K in steps (1 = every step, as the reference),
+                        * or DEM_REBUILD_AUTO: rebuild when the displacement since the last
+                        * build may reach skin/2 (needs skin > 0) */
   int physics_mode;    /* DEM_MODE_* */
   int block_size;      /* threads per block for the particle kernels (0 or 128) */
   int reserved;
@@ -113,13 +129,19 @@ typedef struct dem_particles_view {
   int *ss_cnt;                            /* [n]            tracked contacts */
 } dem_particles_view;
 
-/* Host view of the walls (fields of src/CabanaDEM_Wall.hpp:192-201; wall u
- * and omega are not read by the reference force kernel, :468-469). */
+/* Host view of the walls (fields of src/CabanaDEM_Wall.hpp:192-201).  The
+ * reference reads the wall velocity and spin but leaves them unused
+ * (src/CabanaDEM_Force.hpp:468-469,529): in reference mode the walls stay
+ * put.  In corrected mode a wall translates with u from the position x it
+ * has at upload (one dt per dem_step step), and its surface moves with
+ * u + omega x (c - x_w) at a contact point c (a belt, a turntable); the
+ * normal does not turn.  u / omega NULL: static. */
 typedef struct dem_walls_view {
   int n;
   int reserved;
   const double *x, *normal; /* [n][3] */
   const double *E, *nu, *G; /* [n] */
+  const double *u, *omega;  /* [n][3] or NULL */
 } dem_walls_view;
 
 /* Bond parameters of a bonded system.  The reference hard-codes them in
@@ -165,6 +187,7 @@ typedef struct dem_counters {
   double mean_contacts;   /* c-bar (tracked pp contacts) at the last sampled step */
   double mean_wall_contacts; /* w-bar at the last sampled step */
   int64_t bonds_broken;   /* bonds broken so far (tensile_strength > 0) */
+  int64_t displacement_checks; /* measurements of the automatic rebuild cadence */
 } dem_counters;
 
 /* ---- lifecycle ---------------------------------------------------------- */
@@ -306,6 +329,14 @@ int dem_system_diagnostics(dem_ctx* ctx, double* out);
  * tile of 128 particles; > 0 launches a persistent grid of that many
  * workgroups that stride over the tiles.  Results do not depend on it. */
 int dem_set_launch(dem_ctx* ctx, int grid_blocks);
+/* Force-pass shape (systems without bonds): split = 0 runs each force pass
+ * as one fused kernel (contact detection over the neighbour row, the
+ * contact arithmetic, the kicks); split = 1 runs the detection as a kernel
+ * of its own that writes each particle's overlapping neighbours, then the
+ * force kernel over those lists.  Results are identical either way; which
+ * is faster depends on the contact density.  Bonded systems always run
+ * fused. */
+int dem_set_force_pass(dem_ctx* ctx, int split);
 
 /* Captured step sequences: with enable set (and no diagnostics sampling or
  * kernel timing on), dem_step runs whole cycles of 2 rebuild_every steps
@@ -315,23 +346,27 @@ int dem_set_launch(dem_ctx* ctx, int grid_blocks);
 int dem_set_graph(dem_ctx* ctx, int enable);
 
 /* Autotuner: times trial_steps steps of every candidate configuration from
- * the current state -- neighbour-rebuild cadence (5, 10, 20, 40, 80 and the
- * configured one, when a skin is set; a candidate that breaks the skin
- * check is rejected), force-kernel launch shape (one workgroup per tile, or
- * persistent grids of 2, 4 and 8 workgroups per compute unit) and captured
- * graphs on or off -- restores the state exactly, and applies the fastest.
+ * the current state -- neighbour-rebuild cadence (5, 10, 20, 40, 80,
+ * DEM_REBUILD_AUTO and the configured one, when a skin is set; a candidate
+ * that breaks the skin check is rejected), force-kernel launch shape (one workgroup per tile, or
+ * persistent grids of 2, 4 and 8 workgroups per compute unit), captured
+ * graphs on or off and the force-pass shape (fused or split, systems
+ * without bonds) -- restores the state exactly, and applies the fastest.
  * Rebuild cadence changes the summation order of the contact sums after a
  * re-sort, so runs with different cadences agree to rounding, not bitwise. */
-#define DEM_TUNE_MAX_TRIALS 64
+#define DEM_TUNE_MAX_TRIALS 128
 typedef struct dem_tune_result {
   int rebuild_every;   /* chosen configuration */
   int grid_blocks;
   int use_graph;
+  int split;
   int n_trials;
+  int reserved;
   double ms_per_step;  /* of the chosen configuration */
   int trial_rebuild_every[DEM_TUNE_MAX_TRIALS];
   int trial_grid_blocks[DEM_TUNE_MAX_TRIALS];
   int trial_graph[DEM_TUNE_MAX_TRIALS];
+  int trial_split[DEM_TUNE_MAX_TRIALS];
   double trial_ms_per_step[DEM_TUNE_MAX_TRIALS]; /* < 0: rejected */
 } dem_tune_result;
 int dem_autotune(dem_ctx* ctx, int trial_steps, dem_tune_result* out);

@@ -407,9 +407,17 @@ void oracle_force_pw(OracleParticles* p, const OracleWalls* wl, const OraclePara
       double* swx = p->sw_x + (long)i * MW + j;
       double* swy = p->sw_y + (long)i * MW + j;
       double* swz = p->sw_z + (long)i * MW + j;
-      double pij0 = x[3 * i + 0] - wl->x[3 * j + 0];           // :497-499
-      double pij1 = x[3 * i + 1] - wl->x[3 * j + 1];
-      double pij2 = x[3 * i + 2] - wl->x[3 * j + 2];
+      // the wall point at the force pass: static in reference mode (the
+      // reference never moves a wall); corrected mode translates it with u
+      double xw0 = wl->x[3 * j + 0], xw1 = wl->x[3 * j + 1], xw2 = wl->x[3 * j + 2];
+      if (!ref && wl->u) {
+        xw0 += wl->u[3 * j + 0] * prm->wall_time;
+        xw1 += wl->u[3 * j + 1] * prm->wall_time;
+        xw2 += wl->u[3 * j + 2] * prm->wall_time;
+      }
+      double pij0 = x[3 * i + 0] - xw0;                        // :497-499
+      double pij1 = x[3 * i + 1] - xw1;
+      double pij2 = x[3 * i + 2] - xw2;
       double nx = wl->normal[3 * j + 0], ny = wl->normal[3 * j + 1], nz = wl->normal[3 * j + 2];
       double tmp = pij0 * nx + pij1 * ny + pij2 * nz;          // :514
       double overlap = p->rad[i] - tmp;                        // :515
@@ -424,7 +432,23 @@ void oracle_force_pw(OracleParticles* p, const OracleWalls* wl, const OraclePara
         vi1 = u[3 * i + 1] - (w[3 * i + 2] * nx - w[3 * i + 0] * nz) * a_i;
         vi2 = u[3 * i + 2] - (w[3 * i + 0] * ny - w[3 * i + 1] * nx) * a_i;
       }
-      double v0 = vi0 - 0., v1 = vi1 - 0., v2 = vi2 - 0.;     // :529-535 (wall velocity unused, Q7)
+      // :529-535: the reference sets the wall's velocity to 0 (Q7);
+      // corrected mode takes the surface velocity at the contact point
+      // c = x_i - a_i n: u_w + omega_w x (c - x_w)
+      double vw0 = 0., vw1 = 0., vw2 = 0.;
+      if (!ref && wl->u) {
+        vw0 = wl->u[3 * j + 0]; vw1 = wl->u[3 * j + 1]; vw2 = wl->u[3 * j + 2];
+      }
+      if (!ref && wl->omega) {
+        const double* ow = wl->omega + 3 * j;
+        const double c0 = x[3 * i + 0] - a_i * nx - xw0;
+        const double c1 = x[3 * i + 1] - a_i * ny - xw1;
+        const double c2 = x[3 * i + 2] - a_i * nz - xw2;
+        vw0 += ow[1] * c2 - ow[2] * c1;
+        vw1 += ow[2] * c0 - ow[0] * c2;
+        vw2 += ow[0] * c1 - ow[1] * c0;
+      }
+      double v0 = vi0 - vw0, v1 = vi1 - vw1, v2 = vi2 - vw2;
       double vdn = v0 * nx + v1 * ny + v2 * nz;                // :538
       double vn0 = vdn * nx, vn1 = vdn * ny, vn2 = vdn * nz;
       double vt0 = v0 - vn0, vt1 = v1 - vn1, vt2 = v2 - vn2;
@@ -567,7 +591,13 @@ int oracle_run(OracleParticles* p, const OracleWalls* w, const OracleParams* prm
     oracle_zero_forces(p, prm);
     oracle_update_tangential(p);
     if (oracle_force_pp(p, counts.data(), ids.data(), max_n, prm)) { err = -4; break; }
-    if (w && w->n > 0) oracle_force_pw(p, w, prm);
+    if (w && w->n > 0) {
+      // moving walls: the force pass of this run's step `step` sits at
+      // wall_time + (step + 1) dt (after the drift)
+      OracleParams ps = *prm;
+      ps.wall_time = prm->wall_time + (step + 1) * prm->dt;
+      oracle_force_pw(p, w, &ps);
+    }
     oracle_stage3(p, prm->dt);
     if (run->diag_every > 0 && step % run->diag_every == 0 && diag && nd < diag_cap) {
       double d5[5];

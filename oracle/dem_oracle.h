@@ -49,6 +49,11 @@ typedef struct OracleWalls {
   int n;
   double *x, *normal; /* [n][3] */
   double *E, *nu, *G; /* [n] */
+  /* extension, corrected mode only (the reference reads them at
+   * src/CabanaDEM_Force.hpp:468-469 and leaves them unused, :529): the
+   * plane translates with u ([n][3]) from x at wall_time 0, and its surface
+   * moves with u + omega x (c - x_w(t)) at a contact point c.  NULL: static. */
+  double *u, *omega;
 } OracleWalls;
 
 typedef struct OracleParams {
@@ -56,6 +61,8 @@ typedef struct OracleParams {
   double dt;
   double gravity[3]; /* extension: F_i starts at m_i g in the force pass */
   int mode;          /* ORACLE_MODE_* */
+  double wall_time;  /* extension: time since the walls' x, for moving walls
+                      * (oracle_run sets it per step from its own start) */
 } OracleParams;
 
 /* Driver options for oracle_run (SolverDEM::run, src/CabanaDEM_Solver.hpp:66-98). */

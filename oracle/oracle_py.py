@@ -40,13 +40,14 @@ class OracleParticles(C.Structure):
 
 
 class OracleWalls(C.Structure):
-    _fields_ = [("n", C.c_int), ("x", _dp), ("normal", _dp), ("E", _dp), ("nu", _dp), ("G", _dp)]
+    _fields_ = [("n", C.c_int), ("x", _dp), ("normal", _dp), ("E", _dp), ("nu", _dp), ("G", _dp),
+                ("u", _dp), ("omega", _dp)]
 
 
 class OracleParams(C.Structure):
     _fields_ = [("cor_pp", C.c_double), ("cor_pw", C.c_double), ("fr_pp", C.c_double),
                 ("fr_pw", C.c_double), ("dt", C.c_double), ("gravity", C.c_double * 3),
-                ("mode", C.c_int)]
+                ("mode", C.c_int), ("wall_time", C.c_double)]
 
 
 class OracleRun(C.Structure):
@@ -145,6 +146,9 @@ def walls(w) -> OracleWalls:
     o.n = w.x.shape[0]
     o.x, o.normal, o.E, o.nu, o.G = (_p(np.ascontiguousarray(a, dtype=np.float64))
                                      for a in (w.x, w.normal, w.E, w.nu, w.G))
+    for f in ("u", "omega"):
+        a = getattr(w, f, None)
+        setattr(o, f, None if a is None else _p(np.ascontiguousarray(a, dtype=np.float64)))
     o._keep = w
     return o
 

@@ -176,3 +176,80 @@ def test_run_ke_momentum_history(gpu_lib, mode, K):
     self_x = normwise(pert.x - st.x, ref.x - st.x)
     assert normwise(got.u, ref.u) <= max(TOL_RUN, 10. * self_u)
     assert normwise(got.x - st.x, ref.x - st.x) <= max(TOL_RUN, 10. * self_x)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_split_force_pass_is_bitwise_equal_to_fused(gpu_lib, mode):
+    """dem_set_force_pass(1) (a detection kernel, then the force kernel over
+    its contact lists) against the fused kernel: a dense run with rebuilds
+    every 5 steps and sampled diagnostics, then a stage-wise force pass, every
+    field bitwise."""
+    st, w, lo, hi, dt, rc = _dense_state(seed=8, mode=mode)
+    g = (0., 0., -9.81)
+    kw = dict(dt=dt, rc=rc, lo=lo, hi=hi, gravity=g, mode=mode, rebuild_every=5)
+    a = make_system(st, w, **kw)
+    b = make_system(st, w, **kw)
+    b.set_force_pass(True)
+    for s_ in (a, b):
+        s_.set_diag_every(7)
+    for nsteps in (1, 23, 36):
+        a.step(nsteps)
+        b.step(nsteps)
+    assert np.array_equal(a.diag_history(), b.diag_history())
+    fields = ("x", "u", "omega", "force", "torque", "ss_idx", "ss_cnt", "ss_x", "ss_y", "ss_z",
+              "sw_x", "sw_y", "sw_z")
+    ga, gb = a.download(), b.download()
+    assert ga.ss_cnt.sum() > 100
+    for f in fields:
+        assert np.array_equal(getattr(ga, f), getattr(gb, f)), f
+    for s_ in (a, b):
+        s_.build_neighbors()
+        s_.compute_forces()
+    ga, gb = a.download(), b.download()
+    for f in fields:
+        assert np.array_equal(getattr(ga, f), getattr(gb, f)), f
+    a.close()
+    b.close()
+
+
+def test_automatic_rebuild_cadence_matches_every_step_rebuilds(gpu_lib):
+    """rebuild_every = DEM_REBUILD_AUTO: the rows are rebuilt only when the
+    displacement budget (skin/2) may run out, and the run equals the oracle
+    rebuilding every step (the reference's cadence) within the run bar.
+    Spheres start at 0.5 m/s so the budget runs out several times."""
+    st, w, lo, hi = granular_box(12, 12, 12, spacing_r=2.05, jitter_r=0.02, vsigma=0.5, seed=21)
+    dt, rc = 1e-6, 2.4e-3  # skin = rc - 2 r = 0.4 mm, budget 0.2 mm
+    nsteps, every = 1500, 100
+    g = (0., 0., -9.81)
+    ref = st.copy()
+    diag_ref, _ = O.run(ref, w, oracle_params(dt, gravity=g), nsteps=nsteps, rc=rc, lo=lo, hi=hi,
+                        rebuild_every=1, max_n=64, diag_every=every)
+    sysm = make_system(st, w, dt=dt, rc=rc, lo=lo, hi=hi, gravity=g, rebuild_every=abi.REBUILD_AUTO,
+                       skin=rc - 2e-3)
+    sysm.set_diag_every(every)
+    for chunk in (1, 499, 1000):  # call boundaries inside the budget too
+        sysm.step(chunk)
+    c = sysm.counters()
+    # 0.2 mm at up to ~1.5 m/s: a rebuild every ~100 steps or more, not every step
+    assert 3 <= c.rebuilds <= nsteps // 20, c.rebuilds
+    diag = sysm.diag_history()
+    assert np.array_equal(diag[:, 0], diag_ref[:, 0])
+    for k in range(1, 6):
+        scale = np.abs(diag_ref[:, k]).max()
+        assert np.abs(diag[:, k] - diag_ref[:, k]).max() <= TOL_RUN * scale, k
+    got = sysm.download()
+    assert ref.ss_cnt.sum() > 0
+    # per-particle: within ten times the oracle's own sensitivity (a restart
+    # one ulp away in u), or the run bar
+    pert = st.copy()
+    pert.u[:] = np.nextafter(pert.u, np.inf)
+    O.run(pert, w, oracle_params(dt, gravity=g), nsteps=nsteps, rc=rc, lo=lo, hi=hi,
+          rebuild_every=1, max_n=64)
+    assert normwise(got.u, ref.u) <= max(TOL_RUN, 10. * normwise(pert.u, ref.u))
+    sysm.close()
+
+
+def test_automatic_rebuild_cadence_needs_a_skin(gpu_lib):
+    st, w, lo, hi = granular_box(4, 4, 4, seed=2)
+    with pytest.raises(abi.DemError):
+        make_system(st, w, dt=1e-6, rc=2.2e-3, lo=lo, hi=hi, rebuild_every=abi.REBUILD_AUTO, skin=0.)
