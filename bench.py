@@ -226,7 +226,7 @@ def run_ensemble(args):
     t0 = time.perf_counter()
     tune = sysm.autotune(trial_steps=80)
     log(f"[bench] autotune: rebuild every {tune.rebuild_every}, grid {tune.grid_blocks}, graph "
-        f"{tune.use_graph}, split {tune.split}, {tune.ms_per_step:.4f} ms/step ({tune.n_trials} trials, "
+        f"{tune.use_graph}, force pass {tune.force_pass}, {tune.ms_per_step:.4f} ms/step ({tune.n_trials} trials, "
         f"{time.perf_counter() - t0:.1f} s)")
     # kernel duration and contact statistics: a short timed window, eager
     use_graph = bool(tune.use_graph)
@@ -261,7 +261,7 @@ def run_ensemble(args):
                    "spheres": n, "systems": len(wl.systems), "dt": wl.dt,
                    "autotuned": {"rebuild_every": tune.rebuild_every,
                                  "grid_blocks": tune.grid_blocks, "graph": tune.use_graph,
-                                 "split": tune.split, "trials": tune.trials()},
+                                 "force_pass": tune.force_pass, "trials": tune.trials()},
                    "physics_mode": "reference" if args.mode == 0 else "corrected",
                    "parallelism": "single GPU, one launch per step for all systems"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -440,9 +440,11 @@ def main():
     ap.add_argument("--rebuild-every", type=int, default=None,
                     help="plain workloads: neighbour rebuild cadence in steps, or -1 for the "
                          "automatic cadence (default: the workload's, 20)")
-    ap.add_argument("--force-pass", choices=("fused", "split"), default="fused",
-                    help="plain and ensemble workloads: one fused force kernel, or the detection "
-                         "kernel followed by the force kernel (dem_set_force_pass)")
+    ap.add_argument("--force-pass", choices=("fused", "split", "lane"), default="fused",
+                    help="plain workloads: one force kernel with the contact arithmetic spread "
+                         "over each wave (fused), a detection kernel then the force kernel "
+                         "(split), or one kernel whose lanes walk their own contacts (lane); "
+                         "dem_set_force_pass")
     ap.add_argument("--surrogate-traj", type=int, default=5)
     ap.add_argument("--surrogate-frames", type=int, default=300)
     ap.add_argument("--surrogate-steps", type=int, default=3000)
@@ -488,7 +490,8 @@ def main():
         wl.rebuild_every = args.rebuild_every
     n = wl.state.n
     sysm = wl.system(mode=args.mode)
-    sysm.set_force_pass(args.force_pass == "split")
+    sysm.set_force_pass({"fused": abi.FORCE_PASS_FUSED, "split": abi.FORCE_PASS_SPLIT,
+                         "lane": abi.FORCE_PASS_LANE}[args.force_pass])
     log(f"[bench] {n} spheres generated and uploaded in {time.perf_counter() - t0:.1f} s")
 
     t0 = time.perf_counter()
@@ -566,9 +569,9 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic["bytes_per_launch"] if traffic else None,
-            "kernel": ("fused step kernel (forces + stage3 + next stage1/stage2)"
-                       if args.force_pass == "fused" else
-                       "detection kernel + step kernel (forces + stage3 + next stage1/stage2)"),
+            "kernel": ("detection kernel + step kernel (forces + stage3 + next stage1/stage2)"
+                       if args.force_pass == "split" else
+                       "fused step kernel (forces + stage3 + next stage1/stage2)"),
             "algorithmic_bytes_per_particle": b_alg,
             "kernel_ms": k_ms,
             "launches": launches[0],

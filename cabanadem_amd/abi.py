@@ -35,6 +35,7 @@ ERRORS = {
 MODE_REFERENCE = 0
 MODE_CORRECTED = 1
 REBUILD_AUTO = -1  # include/cabanadem.h DEM_REBUILD_AUTO
+FORCE_PASS_FUSED, FORCE_PASS_SPLIT, FORCE_PASS_LANE = 0, 1, 2  # DEM_FORCE_PASS_*
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -96,18 +97,18 @@ DEM_TUNE_MAX_TRIALS = 128
 class DemTuneResult(C.Structure):
     _fields_ = [
         ("rebuild_every", C.c_int), ("grid_blocks", C.c_int), ("use_graph", C.c_int),
-        ("split", C.c_int), ("n_trials", C.c_int), ("reserved", C.c_int),
+        ("force_pass", C.c_int), ("n_trials", C.c_int), ("reserved", C.c_int),
         ("ms_per_step", C.c_double),
         ("trial_rebuild_every", C.c_int * DEM_TUNE_MAX_TRIALS),
         ("trial_grid_blocks", C.c_int * DEM_TUNE_MAX_TRIALS),
         ("trial_graph", C.c_int * DEM_TUNE_MAX_TRIALS),
-        ("trial_split", C.c_int * DEM_TUNE_MAX_TRIALS),
+        ("trial_force_pass", C.c_int * DEM_TUNE_MAX_TRIALS),
         ("trial_ms_per_step", C.c_double * DEM_TUNE_MAX_TRIALS),
     ]
 
     def trials(self):
         return [dict(rebuild_every=self.trial_rebuild_every[k], grid_blocks=self.trial_grid_blocks[k],
-                     graph=self.trial_graph[k], split=self.trial_split[k],
+                     graph=self.trial_graph[k], force_pass=self.trial_force_pass[k],
                      ms_per_step=self.trial_ms_per_step[k])
                 for k in range(self.n_trials)]
 
@@ -474,10 +475,16 @@ class System:
     def set_launch(self, grid_blocks: int):
         self._check(self._lib.dem_set_launch(self._h, grid_blocks))
 
-    def set_force_pass(self, split: bool):
-        """Force passes as one fused kernel (False) or detection kernel +
-        force kernel (True); identical results."""
-        self._check(self._lib.dem_set_force_pass(self._h, 1 if split else 0))
+    def set_force_pass(self, mode):
+        """FORCE_PASS_FUSED (wave pass over the contacts), FORCE_PASS_SPLIT
+        (detection kernel + force kernel) or FORCE_PASS_LANE (each lane walks
+        its own contacts); identical results.  True / False stand for split /
+        fused."""
+        if mode is True:
+            mode = FORCE_PASS_SPLIT
+        elif mode is False:
+            mode = FORCE_PASS_FUSED
+        self._check(self._lib.dem_set_force_pass(self._h, int(mode)))
 
     def set_params(self, **changes):
         """Non-structural dem_config fields (dt, cor_pp, ..., rebuild_every, physics_mode)."""

@@ -76,7 +76,7 @@ struct dem_ctx {
   // captured step cycles (dem_set_graph): valid while the launch arguments
   // (generation gen) and the buffer parity are those of the capture
   bool use_graph = false;
-  bool split = false;        // force passes as k_detect + force kernel (no bonds)
+  int force_pass = 0;        // DEM_FORCE_PASS_* (systems without bonds)
   // translating walls (corrected mode): positions hold at step wall_step0;
   // the force pass of step s sits at (s + 1 - wall_step0) dt
   int64_t wall_step0 = 0;
@@ -226,7 +226,8 @@ struct dem_ctx {
       }
     }
     a.fext = fext[cS];
-    a.split = (split && !bonds_on) ? 1 : 0;
+    a.split = (force_pass == DEM_FORCE_PASS_SPLIT && !bonds_on) ? 1 : 0;
+    a.lane_walk = force_pass == DEM_FORCE_PASS_LANE ? 1 : 0;
     a.tw = (double)(step + 1 - wall_step0) * cfg.dt;
     return a;
   }
@@ -876,9 +877,11 @@ int dem_set_launch(dem_ctx* c, int grid_blocks) {
   return DEM_OK;
 }
 
-int dem_set_force_pass(dem_ctx* c, int split) {
+int dem_set_force_pass(dem_ctx* c, int mode) {
   if (int r = check_ctx(c)) return r;
-  c->split = split != 0;
+  if (mode != DEM_FORCE_PASS_FUSED && mode != DEM_FORCE_PASS_SPLIT && mode != DEM_FORCE_PASS_LANE)
+    return c->fail(DEM_ERR_ARG, "force pass must be DEM_FORCE_PASS_FUSED, _SPLIT or _LANE");
+  c->force_pass = mode;
   ++c->gen;
   return DEM_OK;
 }
@@ -1347,7 +1350,8 @@ struct Snapshot {
   double last_sample[3];
   dem_config cfg;
   int grid_blocks, diag_every;
-  bool use_graph, timing, split;
+  bool use_graph, timing;
+  int force_pass;
   int auto_left, auto_last;
   bool auto_due;
   double auto_v;
@@ -1402,7 +1406,7 @@ int take_snapshot(dem_ctx* c, Snapshot* s) {
   s->diag_rows = c->diag_hist.size();
   std::copy(c->last_sample, c->last_sample + 3, s->last_sample);
   s->cfg = c->cfg; s->grid_blocks = c->grid_blocks; s->diag_every = c->diag_every;
-  s->use_graph = c->use_graph; s->timing = c->timing; s->split = c->split;
+  s->use_graph = c->use_graph; s->timing = c->timing; s->force_pass = c->force_pass;
   s->auto_left = c->auto_left; s->auto_last = c->auto_last; s->auto_due = c->auto_due;
   s->auto_v = c->auto_v;
   if (e == hipSuccess) snapshot_buffers(c, s, &e);
@@ -1428,7 +1432,7 @@ int restore_snapshot(dem_ctx* c, const Snapshot& s) {
   c->diag_hist.resize(s.diag_rows);
   std::copy(s.last_sample, s.last_sample + 3, c->last_sample);
   c->cfg = s.cfg; c->grid_blocks = s.grid_blocks; c->diag_every = s.diag_every;
-  c->use_graph = s.use_graph; c->timing = s.timing; c->split = s.split;
+  c->use_graph = s.use_graph; c->timing = s.timing; c->force_pass = s.force_pass;
   c->auto_left = s.auto_left; c->auto_last = s.auto_last; c->auto_due = s.auto_due;
   c->auto_v = s.auto_v;
   ++c->gen;
@@ -1458,21 +1462,21 @@ int dem_autotune(dem_ctx* c, int trial_steps, dem_tune_result* out) {
   const int K0 = snap.cfg.rebuild_every;
   if (snap.cfg.skin > 0.) Ks = {5, 10, 20, 40, 80, DEM_REBUILD_AUTO};
   if (std::find(Ks.begin(), Ks.end(), K0) == Ks.end()) Ks.push_back(K0);
-  const std::vector<int> grids = {0, 2 * cus, 4 * cus, 8 * cus};
-  // the split force pass exists for systems without bonds
-  const int n_split = c->bonds_on ? 1 : 2;
+  const std::vector<int> grids = {0, 2 * cus, 8 * cus};
+  // the force-pass shapes exist for systems without bonds
+  const int n_pass = c->bonds_on ? 1 : 3;
   double best = -1.;
   int nt = 0;
   for (int K : Ks)
     for (int g : grids)
       for (int graph = 0; graph < 2; ++graph)
-      for (int split = 0; split < n_split; ++split) {
+      for (int pass = 0; pass < n_pass; ++pass) {
         if (nt >= DEM_TUNE_MAX_TRIALS) break;
         if (int r = restore_snapshot(c, snap)) return r;
         c->cfg.rebuild_every = K;
         c->grid_blocks = g;
         c->use_graph = graph != 0;
-        c->split = split != 0;
+        c->force_pass = pass;
         c->diag_every = 0;
         c->timing = false;
         ++c->gen;
@@ -1491,14 +1495,14 @@ int dem_autotune(dem_ctx* c, int trial_steps, dem_tune_result* out) {
         out->trial_rebuild_every[nt] = K;
         out->trial_grid_blocks[nt] = g;
         out->trial_graph[nt] = graph;
-        out->trial_split[nt] = split;
+        out->trial_force_pass[nt] = pass;
         out->trial_ms_per_step[nt] = ms;
         if (ms > 0. && (best < 0. || ms < best)) {
           best = ms;
           out->rebuild_every = K;
           out->grid_blocks = g;
           out->use_graph = graph;
-          out->split = split;
+          out->force_pass = pass;
         }
         ++nt;
       }
@@ -1509,7 +1513,7 @@ int dem_autotune(dem_ctx* c, int trial_steps, dem_tune_result* out) {
   c->cfg.rebuild_every = out->rebuild_every;
   c->grid_blocks = out->grid_blocks;
   c->use_graph = out->use_graph != 0;
-  c->split = out->split != 0;
+  c->force_pass = out->force_pass;
   ++c->gen;
   return DEM_OK;
 }

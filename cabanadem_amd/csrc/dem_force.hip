@@ -388,13 +388,111 @@ __device__ inline void block_reduce8(const double* v, double* part, int blk) {
   }
 }
 
+// A wave's own LDS columns: its writes before its later reads, and back.
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Bytes of dynamic LDS per workgroup: the contact ids found in phase 1
+// (int [max_c][kBlock]), for the wave pass also the old slot of each (byte
+// [max_c][kBlock], padded to 8) and the contact shares (double [kBlock][6]).
+__host__ __device__ inline size_t force_lds_bytes(int max_c, bool wave) {
+  const size_t mc = (size_t)(max_c > 0 ? max_c : 1);
+  size_t b = mc * kBlock * sizeof(int);
+  if (wave) b += (mc * kBlock + 7) / 8 * 8 + (size_t)kBlock * 6 * sizeof(double);
+  return b;
+}
+
+// Particle-particle contact arithmetic spread over the lanes of a wave.
+// Lane l owns nc_l contacts: ids ascending in its s_cj column (with PRE in
+// its cj_out row), old history slot + 1 in its s_op column (0: a new
+// contact).  The wave numbers its contacts owner-major (an exclusive scan of
+// nc) and, in rounds of 64, every lane takes one: it gathers both particles,
+// runs the contact arithmetic into zero accumulators of negative sign (so
+// each share is exactly the term the owner would add), writes the contact's
+// id and history into the owner's rows, and leaves the owner's share of F
+// and T in LDS.  Owners add their shares in contact order: the same sums in
+// the same order as one lane walking its own list, so the result is bitwise
+// the same, while the FP64 block no longer waits on the lane with the most
+// contacts.  Every lane of the wave must call this (shuffles).
+template <bool REF, bool PRE>
+__device__ __forceinline__ void wave_contacts(const StepArgs& a, const SysParams& P, const int first,
+                                              const int nc, const int* s_cj, const unsigned char* s_op,
+                                              double* s_con, double* F, double* T) {
+  const int tid = threadIdx.x, lane = tid & 63, wbase = tid - lane;
+  const long ld = a.ld;
+  int incl = nc;
+  for (int off = 1; off < 64; off <<= 1) {
+    const int t = __shfl_up(incl, off);
+    if (lane >= off) incl += t;
+  }
+  const int excl = incl - nc;
+  const int total = __shfl(incl, 63);
+  double* con = s_con + (long)wbase * 6; // [64][6] shares of this wave's round
+  for (int base = 0; base < total; base += 64) {
+    const int k = base + lane;
+    // owner: the last lane whose first contact number is <= k
+    int L = 0;
+#pragma unroll
+    for (int st = 32; st > 0; st >>= 1) {
+      const int e = __shfl(excl, L + st);
+      if (e <= k) L += st;
+    }
+    const int eL = __shfl(excl, L);
+    if (k < total) {
+      const int m = k - eL;
+      const int o = wbase + L; // the owner's thread in the workgroup
+      const int io = first + o;
+      const int j = PRE ? a.cj_out[m * ld + io] : s_cj[m * kBlock + o];
+      const int p1 = s_op[m * kBlock + o];
+      double h[3] = {0., 0., 0.};
+      if (p1 > 0) { // continued contact: src/CabanaDEM_Force.hpp:166-178
+        h[0] = a.ch_in[(3 * (p1 - 1) + 0) * ld + io];
+        h[1] = a.ch_in[(3 * (p1 - 1) + 1) * ld + io];
+        h[2] = a.ch_in[(3 * (p1 - 1) + 2) * ld + io];
+      }
+      const double4 xo = a.xr_in[io];
+      double uo[6];
+      load6(a.uw_in + 6l * io, uo);
+      const double4 mo = a.mat[io];
+      const double4 xj = a.xr_in[j];
+      double uj[6];
+      load6(a.uw_in + 6l * j, uj);
+      const double4 mj = a.mat[j];
+      double Fc[3] = {-0., -0., -0.}, Tc[3] = {-0., -0., -0.};
+      pp_contact<REF>(xo, uo, mo, xj, uj, mj, h, Fc, Tc, a, P);
+      if (!PRE) a.cj_out[m * ld + io] = j;
+      a.ch_out[(3 * m + 0) * ld + io] = h[0];
+      a.ch_out[(3 * m + 1) * ld + io] = h[1];
+      a.ch_out[(3 * m + 2) * ld + io] = h[2];
+      double* c = con + lane * 6;
+      c[0] = Fc[0]; c[1] = Fc[1]; c[2] = Fc[2];
+      c[3] = Tc[0]; c[4] = Tc[1]; c[5] = Tc[2];
+    }
+    wave_lds_sync();
+    const int q0 = excl > base ? excl : base;
+    const int q1 = incl < base + 64 ? incl : base + 64;
+    for (int q = q0; q < q1; ++q) {
+      const double* c = con + (q - base) * 6;
+      F[0] += c[0]; F[1] += c[1]; F[2] += c[2];
+      if (!REF) { T[0] += c[3]; T[1] += c[4]; T[2] += c[5]; }
+    }
+    wave_lds_sync(); // the next round rewrites the shares
+  }
+}
+
 // KICK: 0 = forces only (write F,T); 1 = + stage3 (write state, F,T);
 //       2 = + stage3 + stage1 + stage2 of the next step (write state).
 // One tile: kBlock consecutive slots of one system.  ENS: an ensemble (tile
 // table and per-system parameters from the device); otherwise one system
-// whose parameters travel in the kernel arguments.
+// whose parameters travel in the kernel arguments.  Systems without bonds
+// run the contact arithmetic as a wave pass (wave_contacts); bonded ones
+// walk each lane's own list.
 template <bool REF, bool PP, bool PW, int KICK, bool ACCUM, bool DIAG, bool BOND, bool ENS, bool PRE>
 __device__ __forceinline__ void force_tile(const StepArgs& a, const int tile, int* s_cj) {
+  constexpr bool WAVE = PP && !BOND;
   const int tid = threadIdx.x;
   int sys = 0, first = tile * kBlock, end = a.n;
   if (ENS) {
@@ -403,17 +501,27 @@ __device__ __forceinline__ void force_tile(const StepArgs& a, const int tile, in
   }
   const SysParams& P = ENS ? a.sys[sys] : a.sys0;
   const int i = first + tid;
+  const bool act = i < end;
+  const long ld = a.ld;
+  unsigned char* s_op = reinterpret_cast<unsigned char*>(s_cj + (long)(a.max_c > 0 ? a.max_c : 1) * kBlock);
+  double* s_con = reinterpret_cast<double*>(
+      reinterpret_cast<char*>(s_cj) + force_lds_bytes(a.max_c, false) +
+      ((size_t)(a.max_c > 0 ? a.max_c : 1) * kBlock + 7) / 8 * 8);
   double dg[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
-  if (i < end) {
-    const long ld = a.ld;
-    const double4 xi = a.xr_in[i];
-    double ui[6];
+  double4 xi = make_double4(0., 0., 0., 0.);
+  double ui[6] = {0., 0., 0., 0., 0., 0.};
+  double2 mI = make_double2(1., 1.); // m, I
+  // (1-nu^2)/E, (2-nu)/G, 1/r, 1/m of i: read once, on the first contact
+  double4 mti = make_double4(0., 0., 0., 0.);
+  bool have_mti = false;
+  double F[3] = {0., 0., 0.}, T[3] = {0., 0., 0.};
+  int meta = 0, nnb = 0, nc = 0, nwc = 0;
+  // in a bonded run, contacts only between unbonded pairs (or none at all)
+  const bool pp_on = !a.nbond || a.unbonded_contacts;
+  if (act) {
+    xi = a.xr_in[i];
     load6(a.uw_in + 6l * i, ui);
-    const double2 mI = a.mI[i]; // m, I
-    // (1-nu^2)/E, (2-nu)/G, 1/r, 1/m of i: read once, on the first contact
-    double4 mti = make_double4(0., 0., 0., 0.);
-    bool have_mti = false;
-    double F[3], T[3];
+    mI = a.mI[i];
     if (ACCUM) {
       double ft[6];
       load6(a.ft + 6l * i, ft);
@@ -421,7 +529,6 @@ __device__ __forceinline__ void force_tile(const StepArgs& a, const int tile, in
       T[0] = ft[3]; T[1] = ft[4]; T[2] = ft[5];
     } else {
       initial_force(a, i, mI.x, F);
-      T[0] = 0.; T[1] = 0.; T[2] = 0.;
     }
     int nb = 0;
     if (BOND) {
@@ -446,11 +553,9 @@ __device__ __forceinline__ void force_tile(const StepArgs& a, const int tile, in
         for (int c = 0; c < 9; ++c) a.bh[(9 * b + c) * ld + i] = h[c];
       }
     }
-    const int meta = a.cmeta_in[i];
-    int nnb = 0, nc = meta_count(meta), nwc = 0;
-    // in a bonded run, contacts only between unbonded pairs (or none at all)
+    meta = a.cmeta_in[i];
+    nc = meta_count(meta);
     const int nbk = a.nbond ? (BOND ? nb : a.nbond[i]) : 0;
-    const bool pp_on = !a.nbond || a.unbonded_contacts;
     if (PP && !pp_on) nc = 0;
     if (PP && pp_on && PRE) {
       // Phase 1 ran in k_detect: the count is in cmeta_out, the ids (ascending)
@@ -493,35 +598,59 @@ __device__ __forceinline__ void force_tile(const StepArgs& a, const int tile, in
         nc = a.max_c;
       }
     }
-    if (PP && pp_on) {
-      // Phase 2: contact arithmetic, merged with last step's tracked list.
+    if (WAVE && pp_on && !a.lane_walk) {
+      // merge with last step's tracked list (both ascending in j): the old
+      // slot of each contact, or 0 for a new one
+      // (src/CabanaDEM_Force.hpp:166-192, 399-433)
       const int oc = meta_count(meta);
       int p = 0;
       int jo = oc > 0 ? a.cj_in[i] : INT_MAX;
-      if (nc > 0) { mti = a.mat[i]; have_mti = true; }
       for (int m = 0; m < nc; ++m) {
         const int j = PRE ? a.cj_out[m * ld + i] : s_cj[m * kBlock + tid];
         while (jo < j) {
           ++p;
           jo = (p < oc) ? a.cj_in[p * ld + i] : INT_MAX;
         }
-        double h[3] = {0., 0., 0.};
-        if (jo == j) {
-          h[0] = a.ch_in[(3 * p + 0) * ld + i];
-          h[1] = a.ch_in[(3 * p + 1) * ld + i];
-          h[2] = a.ch_in[(3 * p + 2) * ld + i];
-        }
-        const double4 xj = a.xr_in[j];
-        double uj[6];
-        load6(a.uw_in + 6l * j, uj);
-        const double4 mj = a.mat[j];
-        pp_contact<REF>(xi, ui, mti, xj, uj, mj, h, F, T, a, P);
-        if (!PRE) a.cj_out[m * ld + i] = j;
-        a.ch_out[(3 * m + 0) * ld + i] = h[0];
-        a.ch_out[(3 * m + 1) * ld + i] = h[1];
-        a.ch_out[(3 * m + 2) * ld + i] = h[2];
+        s_op[m * kBlock + tid] = (unsigned char)(jo == j ? p + 1 : 0);
       }
     }
+  }
+  const bool wave = WAVE && pp_on && !a.lane_walk; // uniform over the launch
+  if (wave) {
+    // every lane of the wave, active or not: the contacts of the wave
+    wave_lds_sync();
+    wave_contacts<REF, PRE>(a, P, first, act ? nc : 0, s_cj, s_op, s_con, F, T);
+  } else if (PP && pp_on && act) {
+    // Phase 2 walking each lane's own list (bonded systems, or on request):
+    // contact arithmetic, merged with last step's tracked list.
+    const int oc = meta_count(meta);
+    int p = 0;
+    int jo = oc > 0 ? a.cj_in[i] : INT_MAX;
+    if (nc > 0) { mti = a.mat[i]; have_mti = true; }
+    for (int m = 0; m < nc; ++m) {
+      const int j = PRE ? a.cj_out[m * ld + i] : s_cj[m * kBlock + tid];
+      while (jo < j) {
+        ++p;
+        jo = (p < oc) ? a.cj_in[p * ld + i] : INT_MAX;
+      }
+      double h[3] = {0., 0., 0.};
+      if (jo == j) {
+        h[0] = a.ch_in[(3 * p + 0) * ld + i];
+        h[1] = a.ch_in[(3 * p + 1) * ld + i];
+        h[2] = a.ch_in[(3 * p + 2) * ld + i];
+      }
+      const double4 xj = a.xr_in[j];
+      double uj[6];
+      load6(a.uw_in + 6l * j, uj);
+      const double4 mj = a.mat[j];
+      pp_contact<REF>(xi, ui, mti, xj, uj, mj, h, F, T, a, P);
+      if (!PRE) a.cj_out[m * ld + i] = j;
+      a.ch_out[(3 * m + 0) * ld + i] = h[0];
+      a.ch_out[(3 * m + 1) * ld + i] = h[1];
+      a.ch_out[(3 * m + 2) * ld + i] = h[2];
+    }
+  }
+  if (act) {
     int wn = meta_walls(meta);
     if (PW) {
       const int wm = wn;
@@ -848,19 +977,20 @@ inline unsigned nblk(long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 template <bool REF, bool ENS>
 hipError_t launch_split(ForceKind kind, const StepArgs& a, bool sample, const dim3& g, hipStream_t s) {
   const dim3 b(kBlock);
+  const size_t lds = force_lds_bytes(a.max_c, true);
   hipLaunchKernelGGL(k_detect<ENS>, g, b, 0, s, a);
   switch (kind) {
     case FK_PP_PW_FORCES:
-      if (sample) hipLaunchKernelGGL((k_force<REF, true, true, 0, false, true, false, ENS, true>), g, b, 0, s, a);
-      else hipLaunchKernelGGL((k_force<REF, true, true, 0, false, false, false, ENS, true>), g, b, 0, s, a);
+      if (sample) hipLaunchKernelGGL((k_force<REF, true, true, 0, false, true, false, ENS, true>), g, b, lds, s, a);
+      else hipLaunchKernelGGL((k_force<REF, true, true, 0, false, false, false, ENS, true>), g, b, lds, s, a);
       break;
     case FK_STEP_FUSED:
-      if (sample) hipLaunchKernelGGL((k_force<REF, true, true, 2, false, true, false, ENS, true>), g, b, 0, s, a);
-      else hipLaunchKernelGGL((k_force<REF, true, true, 2, false, false, false, ENS, true>), g, b, 0, s, a);
+      if (sample) hipLaunchKernelGGL((k_force<REF, true, true, 2, false, true, false, ENS, true>), g, b, lds, s, a);
+      else hipLaunchKernelGGL((k_force<REF, true, true, 2, false, false, false, ENS, true>), g, b, lds, s, a);
       break;
     case FK_STEP_LAST:
-      if (sample) hipLaunchKernelGGL((k_force<REF, true, true, 1, false, true, false, ENS, true>), g, b, 0, s, a);
-      else hipLaunchKernelGGL((k_force<REF, true, true, 1, false, false, false, ENS, true>), g, b, 0, s, a);
+      if (sample) hipLaunchKernelGGL((k_force<REF, true, true, 1, false, true, false, ENS, true>), g, b, lds, s, a);
+      else hipLaunchKernelGGL((k_force<REF, true, true, 1, false, false, false, ENS, true>), g, b, lds, s, a);
       break;
     default:
       return hipErrorInvalidValue;
@@ -871,7 +1001,8 @@ hipError_t launch_split(ForceKind kind, const StepArgs& a, bool sample, const di
 template <bool REF, bool BOND, bool ENS>
 hipError_t launch_force_mode(ForceKind kind, const StepArgs& a, bool sample, int grid_blocks,
                              hipStream_t s) {
-  const size_t lds = (size_t)(a.max_c > 0 ? a.max_c : 1) * kBlock * sizeof(int);
+  // systems without bonds run the wave pass, which needs its LDS columns
+  const size_t lds = force_lds_bytes(a.max_c, !BOND);
   const int nb = (grid_blocks > 0 && grid_blocks < a.ntiles) ? grid_blocks : a.ntiles;
   const dim3 g(nb), b(kBlock);
   if constexpr (!BOND) {

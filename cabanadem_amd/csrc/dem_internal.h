@@ -167,6 +167,9 @@ struct StepArgs {
   // 1: whole force passes run split, k_detect then the force kernel over its
   // contact lists (systems without bonds); 0: one fused kernel
   int split;
+  // 1: the contact arithmetic walks each lane's own list (the wave pass
+  // otherwise; systems without bonds)
+  int lane_walk;
   // time of this force pass since the walls' positions were set (moving
   // walls, corrected mode)
   double tw;

@@ -329,14 +329,17 @@ int dem_system_diagnostics(dem_ctx* ctx, double* out);
  * tile of 128 particles; > 0 launches a persistent grid of that many
  * workgroups that stride over the tiles.  Results do not depend on it. */
 int dem_set_launch(dem_ctx* ctx, int grid_blocks);
-/* Force-pass shape (systems without bonds): split = 0 runs each force pass
- * as one fused kernel (contact detection over the neighbour row, the
- * contact arithmetic, the kicks); split = 1 runs the detection as a kernel
- * of its own that writes each particle's overlapping neighbours, then the
- * force kernel over those lists.  Results are identical either way; which
- * is faster depends on the contact density.  Bonded systems always run
- * fused. */
-int dem_set_force_pass(dem_ctx* ctx, int split);
+/* Force-pass shape (systems without bonds).  Results are bitwise the same
+ * in every shape; which is fastest depends on the contact density.
+ *   DEM_FORCE_PASS_FUSED (default): one kernel per force pass; each lane
+ *     finds its particle's contacts over the neighbour row, then the wave
+ *     runs the contact arithmetic of all its contacts spread over its lanes;
+ *   DEM_FORCE_PASS_SPLIT: the detection as a kernel of its own that writes
+ *     each particle's overlapping neighbours, then the force kernel;
+ *   DEM_FORCE_PASS_LANE: one kernel, each lane walking its own contacts.
+ * Bonded systems always run fused with each lane walking its own contacts. */
+enum dem_force_pass { DEM_FORCE_PASS_FUSED = 0, DEM_FORCE_PASS_SPLIT = 1, DEM_FORCE_PASS_LANE = 2 };
+int dem_set_force_pass(dem_ctx* ctx, int mode);
 
 /* Captured step sequences: with enable set (and no diagnostics sampling or
  * kernel timing on), dem_step runs whole cycles of 2 rebuild_every steps
@@ -349,8 +352,8 @@ int dem_set_graph(dem_ctx* ctx, int enable);
  * the current state -- neighbour-rebuild cadence (5, 10, 20, 40, 80,
  * DEM_REBUILD_AUTO and the configured one, when a skin is set; a candidate
  * that breaks the skin check is rejected), force-kernel launch shape (one workgroup per tile, or
- * persistent grids of 2, 4 and 8 workgroups per compute unit), captured
- * graphs on or off and the force-pass shape (fused or split, systems
+ * persistent grids of 2 and 8 workgroups per compute unit), captured
+ * graphs on or off and the force-pass shape (DEM_FORCE_PASS_*, systems
  * without bonds) -- restores the state exactly, and applies the fastest.
  * Rebuild cadence changes the summation order of the contact sums after a
  * re-sort, so runs with different cadences agree to rounding, not bitwise. */
@@ -359,14 +362,14 @@ typedef struct dem_tune_result {
   int rebuild_every;   /* chosen configuration */
   int grid_blocks;
   int use_graph;
-  int split;
+  int force_pass;      /* DEM_FORCE_PASS_* */
   int n_trials;
   int reserved;
   double ms_per_step;  /* of the chosen configuration */
   int trial_rebuild_every[DEM_TUNE_MAX_TRIALS];
   int trial_grid_blocks[DEM_TUNE_MAX_TRIALS];
   int trial_graph[DEM_TUNE_MAX_TRIALS];
-  int trial_split[DEM_TUNE_MAX_TRIALS];
+  int trial_force_pass[DEM_TUNE_MAX_TRIALS];
   double trial_ms_per_step[DEM_TUNE_MAX_TRIALS]; /* < 0: rejected */
 } dem_tune_result;
 int dem_autotune(dem_ctx* ctx, int trial_steps, dem_tune_result* out);

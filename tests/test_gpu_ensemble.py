@@ -123,26 +123,29 @@ def test_ensemble_argument_errors(gpu_lib):
     twin.close()
 
 
-def test_split_force_pass_ensemble_is_bitwise_equal(gpu_lib):
-    """The split force pass (detection kernel + force kernel) over the tile
-    table of an ensemble equals the fused kernel, graphs and persistent grid
-    included."""
+def test_force_pass_shapes_ensemble_are_bitwise_equal(gpu_lib):
+    """The split force pass (detection kernel + force kernel) and the
+    lane-walk pass over the tile table of an ensemble equal the default fused
+    pass, graphs and persistent grid included."""
     wl = small_ensemble(nsys=5, seed=9)
     a = wl.system()
     b = wl.system()
-    b.set_force_pass(True)
+    b.set_force_pass(abi.FORCE_PASS_SPLIT)
     b.set_launch(24)
     b.set_graph(True)
+    c = wl.system()
+    c.set_force_pass(abi.FORCE_PASS_LANE)
     for nsteps in (17, 63):
-        a.step(nsteps)
-        b.step(nsteps)
-        sa, sb = a.download(), b.download()
+        for s_ in (a, b, c):
+            s_.step(nsteps)
+        sa, sb, sc = a.download(), b.download(), c.download()
         for f in FIELDS:
             assert np.array_equal(getattr(sa, f), getattr(sb, f)), (nsteps, f)
+            assert np.array_equal(getattr(sa, f), getattr(sc, f)), (nsteps, f)
     assert sa.ss_cnt.sum() > 0
     assert np.array_equal(a.system_diagnostics(), b.system_diagnostics())
-    a.close()
-    b.close()
+    for s_ in (a, b, c):
+        s_.close()
 
 
 def test_graph_replay_is_bitwise_equal(gpu_lib):
@@ -184,8 +187,8 @@ def test_autotune_restores_state_and_applies_choice(gpu_lib):
     b.set_params(rebuild_every=r.rebuild_every)
     b.set_launch(r.grid_blocks)
     b.set_graph(bool(r.use_graph))
-    b.set_force_pass(bool(r.split))
-    assert {t["split"] for t in trials} == {0, 1}
+    b.set_force_pass(r.force_pass)
+    assert {t["force_pass"] for t in trials} == {0, 1, 2}
     a.step(100)
     b.step(100)
     sa, sb = a.download(), b.download()

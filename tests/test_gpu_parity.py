@@ -179,37 +179,44 @@ def test_run_ke_momentum_history(gpu_lib, mode, K):
 
 
 @pytest.mark.parametrize("mode", [0, 1])
-def test_split_force_pass_is_bitwise_equal_to_fused(gpu_lib, mode):
-    """dem_set_force_pass(1) (a detection kernel, then the force kernel over
-    its contact lists) against the fused kernel: a dense run with rebuilds
-    every 5 steps and sampled diagnostics, then a stage-wise force pass, every
-    field bitwise."""
+def test_force_pass_shapes_are_bitwise_equal(gpu_lib, mode):
+    """The three force-pass shapes: fused with the contact arithmetic spread
+    over each wave (the default), split (a detection kernel, then the force
+    kernel over its contact lists) and fused with each lane walking its own
+    contacts.  A dense run with rebuilds every 5 steps and sampled
+    diagnostics, then a stage-wise force pass: every field bitwise equal."""
     st, w, lo, hi, dt, rc = _dense_state(seed=8, mode=mode)
     g = (0., 0., -9.81)
     kw = dict(dt=dt, rc=rc, lo=lo, hi=hi, gravity=g, mode=mode, rebuild_every=5)
-    a = make_system(st, w, **kw)
-    b = make_system(st, w, **kw)
-    b.set_force_pass(True)
-    for s_ in (a, b):
+    shapes = (abi.FORCE_PASS_FUSED, abi.FORCE_PASS_SPLIT, abi.FORCE_PASS_LANE)
+    systems = []
+    for shape in shapes:
+        s_ = make_system(st, w, **kw)
+        s_.set_force_pass(shape)
         s_.set_diag_every(7)
+        systems.append(s_)
     for nsteps in (1, 23, 36):
-        a.step(nsteps)
-        b.step(nsteps)
-    assert np.array_equal(a.diag_history(), b.diag_history())
+        for s_ in systems:
+            s_.step(nsteps)
     fields = ("x", "u", "omega", "force", "torque", "ss_idx", "ss_cnt", "ss_x", "ss_y", "ss_z",
               "sw_x", "sw_y", "sw_z")
-    ga, gb = a.download(), b.download()
-    assert ga.ss_cnt.sum() > 100
-    for f in fields:
-        assert np.array_equal(getattr(ga, f), getattr(gb, f)), f
-    for s_ in (a, b):
+    ref = systems[0].download()
+    assert ref.ss_cnt.sum() > 100 and ref.ss_cnt.max() >= 4
+    for s_ in systems[1:]:
+        assert np.array_equal(systems[0].diag_history(), s_.diag_history())
+        got = s_.download()
+        for f in fields:
+            assert np.array_equal(getattr(ref, f), getattr(got, f)), f
+    for s_ in systems:
         s_.build_neighbors()
         s_.compute_forces()
-    ga, gb = a.download(), b.download()
-    for f in fields:
-        assert np.array_equal(getattr(ga, f), getattr(gb, f)), f
-    a.close()
-    b.close()
+    ref = systems[0].download()
+    for s_ in systems[1:]:
+        got = s_.download()
+        for f in fields:
+            assert np.array_equal(getattr(ref, f), getattr(got, f)), f
+    for s_ in systems:
+        s_.close()
 
 
 def test_automatic_rebuild_cadence_matches_every_step_rebuilds(gpu_lib):
