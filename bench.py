@@ -1,7 +1,10 @@
 """Benchmark: FP64 plain-DEM particle-steps/s at 4M spheres on one MI355X.
 
 Workload (plain_large): 4,000,000 polydisperse spheres on a jittered lattice in
-a six-wall box, gravity, dt = 1e-6 s, cutoff 2.6 mm, rebuild every 20 steps.
+a six-wall box, gravity, dt = 1e-6 s, skin 0.2 mm (cutoff 2.6 mm); the
+neighbour rows are rebuilt on the library's automatic cadence (whenever the
+measured displacement since the last build may exceed the skin budget; the
+workload leaves the cadence open), or every --rebuild-every steps.
 The packing first settles for 2,000 steps (part of the workload, untimed,
 whatever --warmup says), then --warmup further untimed steps, then --steps
 full time steps are timed (10,000 by default).  Inputs are resident in HBM
@@ -226,7 +229,8 @@ def run_ensemble(args):
     t0 = time.perf_counter()
     tune = sysm.autotune(trial_steps=80)
     log(f"[bench] autotune: rebuild every {tune.rebuild_every}, grid {tune.grid_blocks}, graph "
-        f"{tune.use_graph}, force pass {tune.force_pass}, {tune.ms_per_step:.4f} ms/step ({tune.n_trials} trials, "
+        f"{tune.use_graph}, force pass {tune.force_pass}, tiles {tune.block} slots / "
+        f"{tune.tile_rows} cell rows, {tune.ms_per_step:.4f} ms/step ({tune.n_trials} trials, "
         f"{time.perf_counter() - t0:.1f} s)")
     # kernel duration and contact statistics: a short timed window, eager
     use_graph = bool(tune.use_graph)
@@ -261,7 +265,8 @@ def run_ensemble(args):
                    "spheres": n, "systems": len(wl.systems), "dt": wl.dt,
                    "autotuned": {"rebuild_every": tune.rebuild_every,
                                  "grid_blocks": tune.grid_blocks, "graph": tune.use_graph,
-                                 "force_pass": tune.force_pass, "trials": tune.trials()},
+                                 "force_pass": tune.force_pass, "block": tune.block,
+                                 "tile_rows": tune.tile_rows, "trials": tune.trials()},
                    "physics_mode": "reference" if args.mode == 0 else "corrected",
                    "parallelism": "single GPU, one launch per step for all systems"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -379,7 +384,8 @@ def run_surrogate(args):
     try:
         dem_ms_macro, x_ref, tune = dem_reference(True)
         ref_cfg = {"rebuild_every": tune.rebuild_every, "grid_blocks": tune.grid_blocks,
-                   "graph": bool(tune.use_graph)}
+                   "graph": bool(tune.use_graph), "force_pass": tune.force_pass,
+                   "block": tune.block, "tile_rows": tune.tile_rows}
     except Exception as exc:  # the tuned cadence broke the skin check later on
         log(f"[surrogate] tuned DEM reference failed ({exc}); using the workload's cadence")
         dem_ms_macro, x_ref, _ = dem_reference(False)
@@ -434,16 +440,20 @@ def main():
     ap.add_argument("--dims", type=str, default=None)
     ap.add_argument("--settle", type=int, default=SETTLE_STEPS,
                     help="plain_large: settle steps before the timed window (part of the workload)")
-    ap.add_argument("--workload", choices=("plain_large", "plain_dense", "bonded_block", "ensemble",
-                                           "surrogate"),
+    ap.add_argument("--workload", choices=("plain_large", "plain_dense", "plain_packed",
+                                           "bonded_block", "ensemble", "surrogate"),
                     default="plain_large")
-    ap.add_argument("--rebuild-every", type=int, default=None,
+    ap.add_argument("--rebuild-every", type=int, default=-1,
                     help="plain workloads: neighbour rebuild cadence in steps, or -1 for the "
-                         "automatic cadence (default: the workload's, 20)")
-    ap.add_argument("--force-pass", choices=("fused", "split", "lane"), default="fused",
-                    help="plain workloads: one force kernel with the contact arithmetic spread "
-                         "over each wave (fused), a detection kernel then the force kernel "
-                         "(split), or one kernel whose lanes walk their own contacts (lane); "
+                         "automatic cadence (default)")
+    ap.add_argument("--block", type=int, default=0, choices=(0, 64, 128, 256),
+                    help="plain workloads: force-kernel tile size (dem_set_shape; 0: 128)")
+    ap.add_argument("--tile-rows", type=int, default=0,
+                    help="plain workloads: y-rows per cell tile (dem_set_shape; 0: 8)")
+    ap.add_argument("--force-pass", choices=("fused", "split", "wave"), default="fused",
+                    help="plain workloads: one force kernel whose lanes walk their own contacts "
+                         "(fused), a detection kernel then the force kernel (split), or one "
+                         "kernel with the contact arithmetic spread over each wave (wave); "
                          "dem_set_force_pass")
     ap.add_argument("--surrogate-traj", type=int, default=5)
     ap.add_argument("--surrogate-frames", type=int, default=300)
@@ -484,14 +494,14 @@ def main():
 
     dims = tuple(int(v) for v in args.dims.split(","))
     t0 = time.perf_counter()
-    dense = args.workload == "plain_dense"
-    wl = W.plain_dense(dims=dims) if dense else W.plain_large(dims=dims)
-    if args.rebuild_every is not None:
-        wl.rebuild_every = args.rebuild_every
+    wl = {"plain_large": W.plain_large, "plain_dense": W.plain_dense,
+          "plain_packed": W.plain_packed}[args.workload](dims=dims)
+    wl.rebuild_every = args.rebuild_every
     n = wl.state.n
     sysm = wl.system(mode=args.mode)
+    sysm.set_shape(args.block, args.tile_rows)
     sysm.set_force_pass({"fused": abi.FORCE_PASS_FUSED, "split": abi.FORCE_PASS_SPLIT,
-                         "lane": abi.FORCE_PASS_LANE}[args.force_pass])
+                         "wave": abi.FORCE_PASS_WAVE}[args.force_pass])
     log(f"[bench] {n} spheres generated and uploaded in {time.perf_counter() - t0:.1f} s")
 
     t0 = time.perf_counter()
@@ -554,9 +564,13 @@ def main():
         "config": {
             "workload": "plain DEM, %s: %d polydisperse spheres (%dx%dx%d lattice), "
                         "six-wall box, gravity, settle %d + warmup %d + timed %d steps"
-                        % ("dense (spacing 2.0 mm, r U[0.97, 1.03] mm)" if dense else "large",
+                        % ({"plain_large": "large",
+                            "plain_dense": "dense (spacing 2.0 mm, r U[0.97, 1.03] mm)",
+                            "plain_packed": "packed bed held by its walls (spacing 1.98 mm, "
+                                            "r U[0.995, 1.005] mm)"}[args.workload],
                            n, *dims, args.settle, args.warmup, args.steps),
             "force_pass": args.force_pass,
+            "tile_shape": {"block": sysm.shape()[0], "tile_rows": sysm.shape()[1]},
             "settle_steps": args.settle, "diag_every": every, "diag_rows_timed": int(len(diag)),
             "spheres": n, "dt": wl.dt, "cutoff": wl.cutoff, "rebuild_every": wl.rebuild_every,
             "physics_mode": "reference" if args.mode == 0 else "corrected",

@@ -15,7 +15,7 @@ import numpy as np
 LIB_PATH = Path(os.environ.get("CABANADEM_LIB") or
                 Path(__file__).resolve().parent / "lib" / "libcabanadem_hip.so")
 
-DEM_ABI_VERSION = 4  # include/cabanadem.h DEM_ABI_VERSION
+DEM_ABI_VERSION = 5  # include/cabanadem.h DEM_ABI_VERSION
 DEM_MAX_WALLS = 8
 DEM_MAX_GROUPS = 8
 DEM_MAX_BONDS = 32
@@ -35,7 +35,7 @@ ERRORS = {
 MODE_REFERENCE = 0
 MODE_CORRECTED = 1
 REBUILD_AUTO = -1  # include/cabanadem.h DEM_REBUILD_AUTO
-FORCE_PASS_FUSED, FORCE_PASS_SPLIT, FORCE_PASS_LANE = 0, 1, 2  # DEM_FORCE_PASS_*
+FORCE_PASS_FUSED, FORCE_PASS_SPLIT, FORCE_PASS_WAVE = 0, 1, 2  # DEM_FORCE_PASS_*
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -97,18 +97,22 @@ DEM_TUNE_MAX_TRIALS = 128
 class DemTuneResult(C.Structure):
     _fields_ = [
         ("rebuild_every", C.c_int), ("grid_blocks", C.c_int), ("use_graph", C.c_int),
-        ("force_pass", C.c_int), ("n_trials", C.c_int), ("reserved", C.c_int),
+        ("force_pass", C.c_int), ("block", C.c_int), ("tile_rows", C.c_int),
+        ("n_trials", C.c_int), ("reserved", C.c_int),
         ("ms_per_step", C.c_double),
         ("trial_rebuild_every", C.c_int * DEM_TUNE_MAX_TRIALS),
         ("trial_grid_blocks", C.c_int * DEM_TUNE_MAX_TRIALS),
         ("trial_graph", C.c_int * DEM_TUNE_MAX_TRIALS),
         ("trial_force_pass", C.c_int * DEM_TUNE_MAX_TRIALS),
+        ("trial_block", C.c_int * DEM_TUNE_MAX_TRIALS),
+        ("trial_tile_rows", C.c_int * DEM_TUNE_MAX_TRIALS),
         ("trial_ms_per_step", C.c_double * DEM_TUNE_MAX_TRIALS),
     ]
 
     def trials(self):
         return [dict(rebuild_every=self.trial_rebuild_every[k], grid_blocks=self.trial_grid_blocks[k],
                      graph=self.trial_graph[k], force_pass=self.trial_force_pass[k],
+                     block=self.trial_block[k], tile_rows=self.trial_tile_rows[k],
                      ms_per_step=self.trial_ms_per_step[k])
                 for k in range(self.n_trials)]
 
@@ -170,6 +174,8 @@ SYMBOLS = {
     "dem_system_diagnostics": (C.c_int, [C.c_void_p, _dp]),
     "dem_set_launch": (C.c_int, [C.c_void_p, C.c_int]),
     "dem_set_force_pass": (C.c_int, [C.c_void_p, C.c_int]),
+    "dem_set_shape": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "dem_get_shape": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "dem_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "dem_autotune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(DemTuneResult)]),
     "dem_enable_bonds": (C.c_int, [C.c_void_p, C.POINTER(DemBondConfig)]),
@@ -476,15 +482,25 @@ class System:
         self._check(self._lib.dem_set_launch(self._h, grid_blocks))
 
     def set_force_pass(self, mode):
-        """FORCE_PASS_FUSED (wave pass over the contacts), FORCE_PASS_SPLIT
-        (detection kernel + force kernel) or FORCE_PASS_LANE (each lane walks
-        its own contacts); identical results.  True / False stand for split /
-        fused."""
+        """FORCE_PASS_FUSED (one kernel, each lane walks its own contacts),
+        FORCE_PASS_SPLIT (detection kernel + force kernel) or FORCE_PASS_WAVE
+        (one kernel, contact arithmetic spread over each wave); identical
+        results.  True / False stand for split / fused."""
         if mode is True:
             mode = FORCE_PASS_SPLIT
         elif mode is False:
             mode = FORCE_PASS_FUSED
         self._check(self._lib.dem_set_force_pass(self._h, int(mode)))
+
+    def set_shape(self, block: int = 0, tile_rows: int = 0):
+        """Force-kernel tile size (64, 128, 256; 0: 128) and cell-tile rows
+        (1-32; 0: 8): dem_set_shape."""
+        self._check(self._lib.dem_set_shape(self._h, int(block), int(tile_rows)))
+
+    def shape(self):
+        b, r = C.c_int(), C.c_int()
+        self._check(self._lib.dem_get_shape(self._h, C.byref(b), C.byref(r)))
+        return b.value, r.value
 
     def set_params(self, **changes):
         """Non-structural dem_config fields (dt, cor_pp, ..., rebuild_every, physics_mode)."""

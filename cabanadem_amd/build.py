@@ -17,7 +17,10 @@ CSRC = PKG / "csrc"
 LIB = PKG / "lib" / "libcabanadem_hip.so"
 ORACLE_DIR = ROOT / "oracle"
 
-HIP_SOURCES = ["dem_api.hip", "dem_build.hip", "dem_force.hip", "dem_gen.cpp"]
+# the force kernels are instantiated per tile size in three translation units
+# (dem_force*.hip), compiled in parallel with the others
+HIP_SOURCES = ["dem_api.hip", "dem_build.hip", "dem_force.hip", "dem_force_b64.hip",
+               "dem_force_b256.hip", "dem_gen.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -39,11 +42,23 @@ def _run(cmd, cwd=None):
 
 
 def build_library(extra_flags=None) -> Path:
+    """Each source to an object in parallel (hipcc -c), then one link."""
+    from concurrent.futures import ThreadPoolExecutor
+
     LIB.parent.mkdir(parents=True, exist_ok=True)
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    obj_dir = LIB.parent / "obj"
+    obj_dir.mkdir(exist_ok=True)
+    flags = [f for f in HIPCC_FLAGS if f != "-shared"] + list(extra_flags or [])
+    objs = [obj_dir / (src + ".o") for src in HIP_SOURCES]
+    workers = max(1, min(len(HIP_SOURCES), os.cpu_count() or 1, 16))
+    with ThreadPoolExecutor(workers) as pool:
+        jobs = [pool.submit(_run, [hipcc] + flags + ["-c", CSRC / src, "-o", o], CSRC)
+                for src, o in zip(HIP_SOURCES, objs)]
+        for j in jobs:
+            j.result()
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc] + HIPCC_FLAGS + list(extra_flags or []) + [CSRC / s for s in HIP_SOURCES] + ["-o", tmp]
-    _run(cmd, cwd=CSRC)
+    _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + ["-o", tmp], cwd=CSRC)
     os.replace(tmp, LIB)
     return LIB
 

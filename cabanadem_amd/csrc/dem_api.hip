@@ -46,6 +46,9 @@ double eta_prefactor(double cor) {
   return -2. * std::sqrt(5. / 6.) * beta;
 }
 
+// defined below
+Grid make_grid(const dem_config& c, long* ncell, int rows);
+
 } // namespace
 
 struct dem_ctx {
@@ -57,16 +60,26 @@ struct dem_ctx {
   double rc2 = 0.;
 
   // systems: one, or an ensemble of independent systems in one context.
-  // System s owns slots [sys_off[s], sys_off[s+1]) and tiles [tile0[s], tile0[s+1]).
+  // System s owns slots [sys_off[s], sys_off[s+1]) and, for the force
+  // kernels' tile size kForceBlocks[b], tiles [tile0_b[b][s], tile0_b[b][s+1]).
   int nsys = 1;
-  std::vector<int> sys_off, tile0;
+  std::vector<int> sys_off;
+  std::vector<dem_config> sys_cfg; // each system's configuration (binning box)
   std::vector<Grid> grids;
   std::vector<long> kbase;
   std::vector<SysParams> h_sys;
+  long ncell_cap = 0;        // cells the cell arrays hold
+  int tile_rows = kTileRows; // y-rows per cell tile of the binning grids
+  int bidx = 1;              // force-kernel tile size kForceBlocks[bidx] (no bonds)
+  std::vector<int> tile0_b[3];
+  int ntiles_b[3] = {0, 0, 0};
+  int4* d_tiles_b[3] = {nullptr, nullptr, nullptr}; // ensembles only
+  int* d_tile0_b[3] = {nullptr, nullptr, nullptr};
+  // the current tile set: kForceBlocks[bonds_on ? 1 : bidx] (select_block)
   int ntiles = 0;
-  SysParams* d_sys = nullptr;
-  int4* d_tiles = nullptr;   // ensembles only
+  int4* d_tiles = nullptr;
   int* d_tile0 = nullptr;
+  SysParams* d_sys = nullptr;
   int* d_sysid = nullptr;    // ensembles only
   Grid* d_grids = nullptr;   // ensembles only
   long* d_kbase = nullptr;   // ensembles only
@@ -87,7 +100,8 @@ struct dem_ctx {
   unsigned long long* h_disp = nullptr; // pinned host copy of d_disp (a pageable
                                         // destination costs a staged copy per check)
   int auto_left = 0;         // force passes until the next displacement check
-  int auto_last = 0;         // force passes between the last two checks
+  int auto_first = 0;        // interval planned right after the last build
+  int64_t auto_since = 0;    // step at which the positions were last set
   int64_t auto_checks = 0;   // displacement checks so far
   bool auto_due = false;     // rebuild before the next force pass
   double auto_v = -1.;       // largest speed at the last check (< 0: none yet)
@@ -172,7 +186,8 @@ struct dem_ctx {
     hipFree(cell_count); hipFree(cell_start); hipFree(perm); hipFree(inv); hipFree(scan_tmp);
     hipFree(flags); hipFree(diag_part); hipFree(diag_rows); hipFree(d_disp);
     if (h_disp) hipHostFree(h_disp);
-    hipFree(d_sys); hipFree(d_tiles); hipFree(d_tile0); hipFree(d_sysid); hipFree(d_grids);
+    for (int b = 0; b < 3; ++b) { hipFree(d_tiles_b[b]); hipFree(d_tile0_b[b]); }
+    hipFree(d_sys); hipFree(d_sysid); hipFree(d_grids);
     hipFree(d_kbase); hipFree(d_sysdiag);
     if (gexec) hipGraphExecDestroy(gexec);
     for (auto e : ev_pool) hipEventDestroy(e);
@@ -200,6 +215,7 @@ struct dem_ctx {
     a.sys0 = h_sys[0];
     a.tiles = nsys > 1 ? d_tiles : nullptr;
     a.ntiles = ntiles;
+    a.block = kForceBlocks[bonds_on ? 1 : bidx];
     a.xr_in = xr[cP]; a.xr_out = xr[cP ^ 1];
     a.uw_in = uw[cP]; a.uw_out = uw[cP ^ 1];
     a.mat = mat[cS]; a.mI = mI[cS]; a.ft = ft[cS];
@@ -227,7 +243,7 @@ struct dem_ctx {
     }
     a.fext = fext[cS];
     a.split = (force_pass == DEM_FORCE_PASS_SPLIT && !bonds_on) ? 1 : 0;
-    a.lane_walk = force_pass == DEM_FORCE_PASS_LANE ? 1 : 0;
+    a.wave = (force_pass != DEM_FORCE_PASS_FUSED && !bonds_on) ? 1 : 0;
     a.tw = (double)(step + 1 - wall_step0) * cfg.dt;
     return a;
   }
@@ -364,24 +380,90 @@ struct dem_ctx {
     return DEM_OK;
   }
 
+  // The tile set of the force kernels: kForceBlocks[k] for systems without
+  // bonds, kBlock (k = 1) with bonds.
+  void select_block(int k) {
+    bidx = k;
+    const int e = bonds_on ? 1 : k;
+    ntiles = ntiles_b[e];
+    d_tiles = d_tiles_b[e];
+    d_tile0 = d_tile0_b[e];
+  }
+  // Binning grids with `rows` y-rows per cell tile (the cell order of the
+  // re-sort; any value gives exact rows).  Grows the cell arrays when the
+  // padded cell count does; the rows are rebuilt at the next step.
+  int set_tile_rows(int rows) {
+    std::vector<Grid> g(nsys);
+    std::vector<long> kb(nsys + 1, 0);
+    for (int s = 0; s < nsys; ++s) {
+      long nc = 0;
+      g[s] = make_grid(sys_cfg[s], &nc, rows);
+      kb[s + 1] = kb[s] + nc;
+    }
+    if (kb[nsys] > (1l << 30)) return fail(DEM_ERR_ARG, "the binning grids need more than 2^30 cells in total");
+    hipError_t e = hipStreamSynchronize(stream);
+    if (e == hipSuccess && kb[nsys] > ncell_cap) {
+      hipFree(cell_count); hipFree(cell_start); hipFree(scan_tmp);
+      cell_count = cell_start = scan_tmp = nullptr;
+      cell_count = dalloc<int>((size_t)kb[nsys], &e);
+      cell_start = dalloc<int>((size_t)kb[nsys] + 1, &e);
+      scan_tmp = dalloc<int>((size_t)(kb[nsys] / 4096 + 2), &e);
+      if (e == hipSuccess) ncell_cap = kb[nsys];
+    }
+    if (e == hipSuccess && nsys > 1) {
+      e = hipMemcpy(d_grids, g.data(), nsys * sizeof(Grid), hipMemcpyHostToDevice);
+      if (e == hipSuccess) e = hipMemcpy(d_kbase, kb.data(), nsys * sizeof(long), hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) return hip_fail(e, "binning grids");
+    grids = g;
+    kbase = kb;
+    ncell = kb[nsys];
+    tile_rows = rows;
+    return DEM_OK;
+  }
+
   // Automatic rebuild cadence.  Rows built with cutoff >= 2 r_max + skin
   // hold every pair that can touch as long as no particle has moved more
   // than skin/2 since the build.  auto_check measures the largest
   // displacement since the build, d, and the largest speed, v (one kernel,
   // one synchronisation), and plans the next check M force passes ahead so
   // that d + 2 v dt (M + 1) stays within 0.9 skin/2: speeds may double in
-  // the meantime, and a call boundary adds a drift.  M is also at most
-  // twice the previous interval, starting from one pass after a build, so
-  // that a check interval never exceeds the time since the build: under a
-  // constant acceleration from rest, v >= a t then bounds the a (M dt)^2 / 2
-  // that the factor 2 leaves out.  When not one more pass fits, the rows
-  // are rebuilt before the next force pass.  The rebuild's own skin check
-  // stays on behind this as the guard (a violent impulse can still trip it).
+  // the meantime, and a call boundary adds a drift.  M is at most the
+  // number of force passes since the positions were set, so that under a
+  // constant acceleration from rest v >= a t bounds the a (M dt)^2 / 2 that
+  // the factor 2 leaves out; walls that move (corrected mode) count with
+  // twice their surface speed, the most a bounce off them can give.  Right
+  // after a build the first check is planned from the speed measured just
+  // before it; the rows are rebuilt once not one more pass fits, or once
+  // the plan has shrunk below a quarter of that first interval (each check
+  // roughly halves what is left, so this caps the checks per build at about
+  // three).  The rebuild's own skin check stays on behind this as the guard
+  // (a violent impulse can still trip it).
   static constexpr int kAutoMaxInterval = 1000;
+  double wall_speed() const {
+    if (cfg.physics_mode != DEM_MODE_CORRECTED) return 0.; // walls never move in reference mode
+    double vmax = 0.;
+    for (int s = 0; s < nsys; ++s) {
+      const Grid& g = grids[s];
+      for (int j = 0; j < h_sys[s].n_walls; ++j) {
+        const WallConst& W = h_sys[s].walls[j];
+        double reach2 = 0.; // farthest corner of the binning box from x_w
+        for (int d = 0; d < 3; ++d) {
+          const double a = std::fabs(g.lo[d] - W.x[d]), b = std::fabs(g.lo[d] + g.nc[d] * g.h[d] - W.x[d]);
+          reach2 += std::max(a, b) * std::max(a, b);
+        }
+        const double u = std::sqrt(W.u[0] * W.u[0] + W.u[1] * W.u[1] + W.u[2] * W.u[2]);
+        const double w = std::sqrt(W.w[0] * W.w[0] + W.w[1] * W.w[1] + W.w[2] * W.w[2]);
+        vmax = std::max(vmax, u + w * std::sqrt(reach2));
+      }
+    }
+    return vmax;
+  }
   int auto_interval(double budget) const {
     const double per_step = 2. * auto_v * cfg.dt;
     if (!(budget > 0.)) return 0;
-    const int cap = std::min(kAutoMaxInterval, std::max(1, 2 * auto_last));
+    const int64_t age = step - auto_since;
+    const int cap = (int)std::min<int64_t>(kAutoMaxInterval, std::max<int64_t>(1, age));
     if (!(per_step > 0.)) return cap;
     const double m = std::floor(budget / per_step) - 1.;
     return m < 1. ? 0 : (m > cap ? cap : (int)m);
@@ -399,18 +481,27 @@ struct dem_ctx {
     double d2, v2;
     std::memcpy(&d2, &h_disp[0], sizeof(double));
     std::memcpy(&v2, &h_disp[1], sizeof(double));
-    auto_v = std::sqrt(v2);
+    auto_v = std::max(std::sqrt(v2), 2. * wall_speed()); // NaN stays NaN
+    if (v2 != v2) auto_v = v2;
     const int m = auto_interval(0.45 * cfg.skin - std::sqrt(d2)); // NaN: 0, rebuild
-    auto_due = m < 1;
+    auto_due = m < 1 || 4 * m < auto_first;
     auto_left = m;
-    auto_last = m;
     return DEM_OK;
   }
-  // after a rebuild: the first check one pass later
+  // after a build: the first check planned from the last measured speed
+  // (one pass ahead without one)
   void auto_after_rebuild() {
     auto_due = false;
-    auto_left = 1;
-    auto_last = 1;
+    auto_first = auto_v >= 0. ? std::max(1, auto_interval(0.45 * cfg.skin)) : 1;
+    auto_left = auto_first;
+  }
+  // positions or velocities set from outside: measure afresh
+  void auto_reset() {
+    auto_since = step;
+    auto_v = -1.;
+    auto_first = 0;
+    auto_left = 0;
+    auto_due = false;
   }
 
   // Materials and walls of every system to the device (synchronous).
@@ -510,16 +601,17 @@ int validate(const dem_config* c, std::string* msg) {
   }
 #define DEM_STR2(x) #x
 #define DEM_STR(x) DEM_STR2(x)
-  if (c->block_size != 0 && c->block_size != kBlock) {
-    *msg = "block_size must be 0 or " DEM_STR(DEM_BLOCK);
+  if (c->block_size != 0 && c->block_size != 64 && c->block_size != 128 && c->block_size != 256) {
+    *msg = "block_size must be 0 (" DEM_STR(DEM_BLOCK) "), 64, 128 or 256";
     return DEM_ERR_ARG;
   }
   return DEM_OK;
 }
 
 // Cells of edge >= cutoff (1e-9 margin against rounding in (x-lo)/h),
-// capped at 64M cells by coarsening (coarser cells stay exact).
-Grid make_grid(const dem_config& c, long* ncell) {
+// capped at 64M cells by coarsening (coarser cells stay exact); cells
+// numbered in y-tiles of `rows` rows.
+Grid make_grid(const dem_config& c, long* ncell, int rows) {
   Grid g{};
   long cap = 64l << 20;
   int nc[3];
@@ -531,7 +623,7 @@ Grid make_grid(const dem_config& c, long* ncell) {
     if (f > 1e6) f = 1e6;
     nc[d] = (int)f;
   }
-  auto padded = [&]() { return (long)nc[0] * ((nc[1] + kTileRows - 1) / kTileRows) * kTileRows * nc[2]; };
+  auto padded = [&]() { return (long)nc[0] * ((nc[1] + rows - 1) / rows) * rows * nc[2]; };
   while (padded() > cap) {
     int d = (nc[0] >= nc[1] && nc[0] >= nc[2]) ? 0 : (nc[1] >= nc[2] ? 1 : 2);
     nc[d] = std::max(1, nc[d] / 2);
@@ -544,7 +636,8 @@ Grid make_grid(const dem_config& c, long* ncell) {
     double cs = c.cutoff * (1.0 + 1e-9);
     if (g.h[d] < cs) g.h[d] = cs;
   }
-  g.nty = (nc[1] + kTileRows - 1) / kTileRows;
+  g.tr = rows;
+  g.nty = (nc[1] + rows - 1) / rows;
   *ncell = cell_key_count(g);
   return g;
 }
@@ -654,28 +747,34 @@ int create_ctx(const dem_config* common, int nsys, const dem_system_desc* sd, de
   // systems: slot ranges, tiles, grids and key ranges, materials
   c->nsys = nsys;
   c->sys_off.assign(nsys + 1, 0);
-  c->tile0.assign(nsys + 1, 0);
+  c->sys_cfg.resize(nsys);
   c->grids.resize(nsys);
   c->kbase.assign(nsys + 1, 0);
   c->h_sys.assign(nsys, SysParams{});
-  std::vector<int4> tiles;
+  std::vector<int4> tiles[3];
   std::vector<int> sysid;
+  for (int b = 0; b < 3; ++b) c->tile0_b[b].assign(nsys + 1, 0);
   for (int s = 0; s < nsys; ++s) {
     dem_config cs = system_config(*common, sd[s]);
+    c->sys_cfg[s] = cs;
     long nc = 0;
-    c->grids[s] = make_grid(cs, &nc);
+    c->grids[s] = make_grid(cs, &nc, c->tile_rows);
     c->kbase[s + 1] = c->kbase[s] + nc;
     c->sys_off[s + 1] = c->sys_off[s] + sd[s].n;
-    const int nt = (sd[s].n + kBlock - 1) / kBlock;
-    c->tile0[s + 1] = c->tile0[s] + nt;
-    for (int t = 0; t < nt; ++t)
-      tiles.push_back(make_int4(s, c->sys_off[s] + t * kBlock, c->sys_off[s + 1], 0));
+    for (int b = 0; b < 3; ++b) {
+      const int B = kForceBlocks[b];
+      const int nt = (sd[s].n + B - 1) / B;
+      c->tile0_b[b][s + 1] = c->tile0_b[b][s] + nt;
+      for (int t = 0; t < nt; ++t)
+        tiles[b].push_back(make_int4(s, c->sys_off[s] + t * B, c->sys_off[s + 1], 0));
+    }
     if (nsys > 1) sysid.insert(sysid.end(), (size_t)sd[s].n, s);
     set_materials(c->h_sys[s], sd[s].cor_pp, sd[s].cor_pw, sd[s].friction_pp, sd[s].friction_pw);
     c->h_sys[s].n_walls = 0;
   }
   c->ncell = c->kbase[nsys];
-  c->ntiles = c->tile0[nsys];
+  c->ncell_cap = c->ncell;
+  for (int b = 0; b < 3; ++b) c->ntiles_b[b] = c->tile0_b[b][nsys];
   if (c->ncell > (1l << 30)) {
     delete c;
     set_global_error("the binning grids need more than 2^30 cells in total");
@@ -709,12 +808,12 @@ int create_ctx(const dem_config* common, int nsys, const dem_system_desc* sd, de
   c->scan_tmp = dalloc<int>((size_t)(c->ncell / 4096 + 2), &e);
   c->flags = dalloc<DevFlags>(1, &e);
   c->d_disp = dalloc<unsigned long long>(2, &e);
-  c->diag_part = dalloc<double>((size_t)c->ntiles * 8, &e);
+  c->diag_part = dalloc<double>((size_t)c->ntiles_b[0] * 8, &e); // the most tiles (64 slots)
   c->d_sys = dalloc<SysParams>((size_t)nsys, &e);
-  c->d_tile0 = dalloc<int>((size_t)nsys + 1, &e);
+  for (int b = 0; b < 3; ++b) c->d_tile0_b[b] = dalloc<int>((size_t)nsys + 1, &e);
   c->d_sysdiag = dalloc<double>((size_t)nsys * 8, &e);
   if (nsys > 1) {
-    c->d_tiles = dalloc<int4>(tiles.size(), &e);
+    for (int b = 0; b < 3; ++b) c->d_tiles_b[b] = dalloc<int4>(tiles[b].size(), &e);
     c->d_sysid = dalloc<int>(n, &e);
     c->d_grids = dalloc<Grid>((size_t)nsys, &e);
     c->d_kbase = dalloc<long>((size_t)nsys, &e);
@@ -733,14 +832,19 @@ int create_ctx(const dem_config* common, int nsys, const dem_system_desc* sd, de
   if (e == hipSuccess) e = hipMemset(c->xr[0], 0, n * sizeof(double4));
   if (e == hipSuccess) e = hipMemset(c->nn, 0, n * sizeof(int));
   if (e == hipSuccess) e = hipMemset(c->flags, 0, sizeof(DevFlags));
-  if (e == hipSuccess) e = hipMemcpy(c->d_tile0, c->tile0.data(), (nsys + 1) * sizeof(int), hipMemcpyHostToDevice);
+  for (int b = 0; b < 3; ++b) {
+    if (e == hipSuccess)
+      e = hipMemcpy(c->d_tile0_b[b], c->tile0_b[b].data(), (nsys + 1) * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess && nsys > 1)
+      e = hipMemcpy(c->d_tiles_b[b], tiles[b].data(), tiles[b].size() * sizeof(int4), hipMemcpyHostToDevice);
+  }
   if (e == hipSuccess && nsys > 1) {
-    e = hipMemcpy(c->d_tiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(c->d_sysid, sysid.data(), n * sizeof(int), hipMemcpyHostToDevice);
+    e = hipMemcpy(c->d_sysid, sysid.data(), n * sizeof(int), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(c->d_grids, c->grids.data(), nsys * sizeof(Grid), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(c->d_kbase, c->kbase.data(), nsys * sizeof(long), hipMemcpyHostToDevice);
   }
   if (e != hipSuccess) { int r = c->hip_fail(e, "initial fill"); delete c; return r; }
+  c->select_block(common->block_size == 64 ? 0 : (common->block_size == 256 ? 2 : 1));
   if ((rc = c->upload_sys())) { delete c; return rc; }
   c->h_m.assign(n, 1.); c->h_rad.assign(n, 1.); c->h_E.assign(n, 1.);
   c->h_nu.assign(n, 0.); c->h_G.assign(n, 1.); c->h_I.assign(n, 1.);
@@ -879,10 +983,40 @@ int dem_set_launch(dem_ctx* c, int grid_blocks) {
 
 int dem_set_force_pass(dem_ctx* c, int mode) {
   if (int r = check_ctx(c)) return r;
-  if (mode != DEM_FORCE_PASS_FUSED && mode != DEM_FORCE_PASS_SPLIT && mode != DEM_FORCE_PASS_LANE)
-    return c->fail(DEM_ERR_ARG, "force pass must be DEM_FORCE_PASS_FUSED, _SPLIT or _LANE");
+  if (mode != DEM_FORCE_PASS_FUSED && mode != DEM_FORCE_PASS_SPLIT && mode != DEM_FORCE_PASS_WAVE)
+    return c->fail(DEM_ERR_ARG, "force pass must be DEM_FORCE_PASS_FUSED, _SPLIT or _WAVE");
   c->force_pass = mode;
   ++c->gen;
+  return DEM_OK;
+}
+
+int dem_set_shape(dem_ctx* c, int block, int tile_rows) {
+  if (int r = check_ctx(c)) return r;
+  if (block != 0 && block != 64 && block != 128 && block != 256)
+    return c->fail(DEM_ERR_ARG, "block must be 0 (128), 64, 128 or 256");
+  if (tile_rows < 0 || tile_rows > kMaxTileRows)
+    return c->fail(DEM_ERR_ARG, "tile_rows must be 0 (8) or 1 to 32");
+  if (block == 0) block = kBlock;
+  if (tile_rows == 0) tile_rows = kTileRows;
+  if (c->bonds_on && block != kBlock)
+    return c->fail(DEM_ERR_ARG, "bonded systems run force tiles of 128");
+  const int k = block == 64 ? 0 : (block == 256 ? 2 : 1);
+  if (k != c->bidx) {
+    c->select_block(k);
+    ++c->gen;
+  }
+  if (tile_rows != c->tile_rows) {
+    if (int r = c->set_tile_rows(tile_rows)) return r;
+    c->list_valid = false; // the next step re-sorts into the new cell order
+    ++c->gen;
+  }
+  return DEM_OK;
+}
+
+int dem_get_shape(dem_ctx* c, int* block, int* tile_rows) {
+  if (int r = check_ctx(c)) return r;
+  if (block) *block = kForceBlocks[c->bonds_on ? 1 : c->bidx];
+  if (tile_rows) *tile_rows = c->tile_rows;
   return DEM_OK;
 }
 
@@ -997,6 +1131,7 @@ int dem_upload_particles(dem_ctx* c, const dem_particles_view* v) {
   if (c->bonds_on && (v->E || v->G))
     if (int r = c->upload_bEG(id)) return r;
   if (v->x) c->list_valid = false;
+  if (v->x || v->u) c->auto_reset();
   return DEM_OK;
 }
 
@@ -1120,6 +1255,7 @@ int dem_set_kinematics_device(dem_ctx* c, const double* x, const double* u, int 
                                        c->ft[c->cS], c->cmeta[c->cC], c->stream);
   if (e != hipSuccess) return c->hip_fail(e, "set kinematics");
   if (x) c->list_valid = false;
+  if (x || u) c->auto_reset();
   return DEM_OK;
 }
 
@@ -1351,8 +1487,9 @@ struct Snapshot {
   dem_config cfg;
   int grid_blocks, diag_every;
   bool use_graph, timing;
-  int force_pass;
-  int auto_left, auto_last;
+  int force_pass, bidx, tile_rows;
+  int auto_left, auto_first;
+  int64_t auto_since;
   bool auto_due;
   double auto_v;
 
@@ -1407,7 +1544,9 @@ int take_snapshot(dem_ctx* c, Snapshot* s) {
   std::copy(c->last_sample, c->last_sample + 3, s->last_sample);
   s->cfg = c->cfg; s->grid_blocks = c->grid_blocks; s->diag_every = c->diag_every;
   s->use_graph = c->use_graph; s->timing = c->timing; s->force_pass = c->force_pass;
-  s->auto_left = c->auto_left; s->auto_last = c->auto_last; s->auto_due = c->auto_due;
+  s->bidx = c->bidx; s->tile_rows = c->tile_rows;
+  s->auto_left = c->auto_left; s->auto_first = c->auto_first; s->auto_due = c->auto_due;
+  s->auto_since = c->auto_since;
   s->auto_v = c->auto_v;
   if (e == hipSuccess) snapshot_buffers(c, s, &e);
   return e == hipSuccess ? DEM_OK : c->hip_fail(e, "autotune snapshot");
@@ -1433,7 +1572,11 @@ int restore_snapshot(dem_ctx* c, const Snapshot& s) {
   std::copy(s.last_sample, s.last_sample + 3, c->last_sample);
   c->cfg = s.cfg; c->grid_blocks = s.grid_blocks; c->diag_every = s.diag_every;
   c->use_graph = s.use_graph; c->timing = s.timing; c->force_pass = s.force_pass;
-  c->auto_left = s.auto_left; c->auto_last = s.auto_last; c->auto_due = s.auto_due;
+  c->select_block(s.bidx);
+  if (s.tile_rows != c->tile_rows)
+    if (int r = c->set_tile_rows(s.tile_rows)) return r;
+  c->auto_left = s.auto_left; c->auto_first = s.auto_first; c->auto_due = s.auto_due;
+  c->auto_since = s.auto_since;
   c->auto_v = s.auto_v;
   ++c->gen;
   c->err.clear();
@@ -1463,49 +1606,69 @@ int dem_autotune(dem_ctx* c, int trial_steps, dem_tune_result* out) {
   if (snap.cfg.skin > 0.) Ks = {5, 10, 20, 40, 80, DEM_REBUILD_AUTO};
   if (std::find(Ks.begin(), Ks.end(), K0) == Ks.end()) Ks.push_back(K0);
   const std::vector<int> grids = {0, 2 * cus, 8 * cus};
-  // the force-pass shapes exist for systems without bonds
+  // the force-pass shapes and tile sizes exist for systems without bonds
   const int n_pass = c->bonds_on ? 1 : 3;
   double best = -1.;
   int nt = 0;
+  auto trial = [&](int K, int g, int graph, int pass, int bk, int rows) -> int {
+    if (nt >= DEM_TUNE_MAX_TRIALS) return DEM_OK;
+    if (int r = restore_snapshot(c, snap)) return r;
+    c->cfg.rebuild_every = K;
+    c->grid_blocks = g;
+    c->use_graph = graph != 0;
+    c->force_pass = pass;
+    c->diag_every = 0;
+    c->timing = false;
+    ++c->gen;
+    int r = dem_set_shape(c, kForceBlocks[bk], rows);
+    // whole captured cycles of 2K steps (automatic cadence: periods of 40)
+    const int period = K > 0 ? 2 * K : 40;
+    const int steps = ((std::max(trial_steps, period) + period - 1) / period) * period;
+    double ms = -1.;
+    if (r == DEM_OK) r = dem_step(c, period); // warm-up: builds, captures
+    if (r == DEM_OK) {
+      auto t0 = std::chrono::steady_clock::now();
+      r = dem_step(c, steps);
+      auto t1 = std::chrono::steady_clock::now();
+      if (r == DEM_OK) ms = std::chrono::duration<double, std::milli>(t1 - t0).count() / steps;
+    }
+    if (r == DEM_ERR_HIP) return r;
+    out->trial_rebuild_every[nt] = K;
+    out->trial_grid_blocks[nt] = g;
+    out->trial_graph[nt] = graph;
+    out->trial_force_pass[nt] = pass;
+    out->trial_block[nt] = kForceBlocks[bk];
+    out->trial_tile_rows[nt] = rows;
+    out->trial_ms_per_step[nt] = ms;
+    if (ms > 0. && (best < 0. || ms < best)) {
+      best = ms;
+      out->rebuild_every = K;
+      out->grid_blocks = g;
+      out->use_graph = graph;
+      out->force_pass = pass;
+      out->block = kForceBlocks[bk];
+      out->tile_rows = rows;
+    }
+    ++nt;
+    return DEM_OK;
+  };
+  // stage 1: cadence x launch grid x graphs x force-pass shape, at the
+  // context's tile size and cell tiles
   for (int K : Ks)
     for (int g : grids)
       for (int graph = 0; graph < 2; ++graph)
-      for (int pass = 0; pass < n_pass; ++pass) {
-        if (nt >= DEM_TUNE_MAX_TRIALS) break;
-        if (int r = restore_snapshot(c, snap)) return r;
-        c->cfg.rebuild_every = K;
-        c->grid_blocks = g;
-        c->use_graph = graph != 0;
-        c->force_pass = pass;
-        c->diag_every = 0;
-        c->timing = false;
-        ++c->gen;
-        // whole captured cycles of 2K steps (automatic cadence: periods of 40)
-        const int period = K > 0 ? 2 * K : 40;
-        const int steps = ((std::max(trial_steps, period) + period - 1) / period) * period;
-        double ms = -1.;
-        int r = dem_step(c, period); // warm-up: builds, captures
-        if (r == DEM_OK) {
-          auto t0 = std::chrono::steady_clock::now();
-          r = dem_step(c, steps);
-          auto t1 = std::chrono::steady_clock::now();
-          if (r == DEM_OK) ms = std::chrono::duration<double, std::milli>(t1 - t0).count() / steps;
-        }
-        if (r == DEM_ERR_HIP) return r;
-        out->trial_rebuild_every[nt] = K;
-        out->trial_grid_blocks[nt] = g;
-        out->trial_graph[nt] = graph;
-        out->trial_force_pass[nt] = pass;
-        out->trial_ms_per_step[nt] = ms;
-        if (ms > 0. && (best < 0. || ms < best)) {
-          best = ms;
-          out->rebuild_every = K;
-          out->grid_blocks = g;
-          out->use_graph = graph;
-          out->force_pass = pass;
-        }
-        ++nt;
-      }
+        for (int pass = 0; pass < n_pass; ++pass)
+          if (int r = trial(K, g, graph, pass, snap.bidx, snap.tile_rows)) return r;
+  // stage 2: tile sizes (systems without bonds) x cell-tile rows at the best
+  // of stage 1
+  if (best > 0.) {
+    const int K = out->rebuild_every, g = out->grid_blocks, graph = out->use_graph, pass = out->force_pass;
+    const std::vector<int> bks = c->bonds_on ? std::vector<int>{1} : std::vector<int>{0, 1, 2};
+    for (int bk : bks)
+      for (int rows : {4, 8, 16})
+        if (bk != snap.bidx || rows != snap.tile_rows)
+          if (int r = trial(K, g, graph, pass, bk, rows)) return r;
+  }
   out->n_trials = nt;
   out->ms_per_step = best;
   if (int r = restore_snapshot(c, snap)) return r;
@@ -1515,7 +1678,7 @@ int dem_autotune(dem_ctx* c, int trial_steps, dem_tune_result* out) {
   c->use_graph = out->use_graph != 0;
   c->force_pass = out->force_pass;
   ++c->gen;
-  return DEM_OK;
+  return dem_set_shape(c, out->block, out->tile_rows);
 }
 
 int dem_get_diag_history(dem_ctx* c, double* rows, int cap, int* nrows) {
@@ -1634,6 +1797,7 @@ int dem_enable_bonds(dem_ctx* c, const dem_bond_config* b) {
   c->max_b = b->max_bonds;
   c->bcfg = *b;
   c->bonds_on = true;
+  c->select_block(c->bidx); // bonded systems run tiles of kBlock
   ++c->gen;
   return c->upload_bEG(id);
 }

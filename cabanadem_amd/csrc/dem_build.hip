@@ -25,6 +25,7 @@
 //                    ascending j (the order the force kernel merges in).
 #include "dem_internal.h"
 
+#include <algorithm>
 #include <climits>
 
 namespace demhip {
@@ -323,10 +324,10 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(BuildArgs a) {
     int key = (int)(a.key_sorted[p] - base);
     const int cx = key % nx;
     key /= nx;
-    const int ly = key % kTileRows;
-    key /= kTileRows;
+    const int ly = key % g.tr;
+    key /= g.tr;
     const int cz = key % nz;
-    const int cy = (key / nz) * kTileRows + ly;
+    const int cy = (key / nz) * g.tr + ly;
     xi = a.xr_out[p];
     const int xlo = cx > 0 ? cx - 1 : 0;
     const int xhi = cx < nx - 1 ? cx + 1 : nx - 1;
@@ -424,23 +425,40 @@ __global__ __launch_bounds__(kBlock) void k_setup_bonds(BondSetupArgs a) {
 // over all particles.  One atomic per wave and quantity: non-negative
 // doubles order like their bit patterns, so atomicMax on the bits is a max
 // on the values (a NaN lands above every finite value).
-__global__ __launch_bounds__(kBlock) void k_disp_check(int n, const double4* xr, const double* xb,
-                                                       const double* uw, unsigned long long* out) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
+// Largest |x - xb|^2 and |u|^2 over all particles (NaN wins): a grid-stride
+// loop, a wave reduction, a block reduction through LDS, then one atomicMax
+// per block and value on the bit patterns (order-preserving for doubles
+// >= 0 and sending NaN to the top).
+constexpr int kDispBlock = 256;
+constexpr int kDispBlocksPerCU = 4;
+__device__ __forceinline__ double max_nan(double a, double b) { return (b > a || b != b) ? b : a; }
+
+__global__ __launch_bounds__(kDispBlock) void k_disp_check(int n, const double4* xr, const double* xb,
+                                                          const double* uw, unsigned long long* out) {
   double d2 = 0., v2 = 0.;
-  if (i < n) {
+  for (int i = blockIdx.x * kDispBlock + threadIdx.x; i < n; i += gridDim.x * kDispBlock) {
     const double4 p = xr[i];
     const double d0 = p.x - xb[3 * i + 0], d1 = p.y - xb[3 * i + 1], dz = p.z - xb[3 * i + 2];
-    d2 = d0 * d0 + d1 * d1 + dz * dz;
+    d2 = max_nan(d2, d0 * d0 + d1 * d1 + dz * dz);
     const double* u = uw + 6l * i;
-    v2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    v2 = max_nan(v2, u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
   }
   for (int off = 32; off > 0; off >>= 1) {
-    const double od = __shfl_xor(d2, off), ov = __shfl_xor(v2, off);
-    d2 = (od > d2 || od != od) ? od : d2;
-    v2 = (ov > v2 || ov != ov) ? ov : v2;
+    d2 = max_nan(d2, __shfl_xor(d2, off));
+    v2 = max_nan(v2, __shfl_xor(v2, off));
   }
+  __shared__ double s_d[kDispBlock / 64], s_v[kDispBlock / 64];
+  const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
+    s_d[w] = d2;
+    s_v[w] = v2;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < kDispBlock / 64; ++k) {
+      d2 = max_nan(d2, s_d[k]);
+      v2 = max_nan(v2, s_v[k]);
+    }
     atomicMax(&out[0], (unsigned long long)__double_as_longlong(d2));
     atomicMax(&out[1], (unsigned long long)__double_as_longlong(v2));
   }
@@ -457,7 +475,15 @@ inline unsigned nblk(long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 hipError_t launch_disp_check(int n, const double4* xr, const double* xb, const double* uw,
                              unsigned long long* out, hipStream_t s) {
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, out, 2);
-  hipLaunchKernelGGL(k_disp_check, dim3(nblk(n)), dim3(kBlock), 0, s, n, xr, xb, uw, out);
+  static const int cus = [] {
+    int dev = 0, c = 0;
+    return (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && c > 0)
+               ? c : 256;
+  }();
+  const long need = ((long)n + kDispBlock - 1) / kDispBlock;
+  const unsigned grid = (unsigned)std::max(1L, std::min(need, (long)cus * kDispBlocksPerCU));
+  hipLaunchKernelGGL(k_disp_check, dim3(grid), dim3(kDispBlock), 0, s, n, xr, xb, uw, out);
   return hipGetLastError();
 }
 

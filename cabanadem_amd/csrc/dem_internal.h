@@ -17,7 +17,7 @@
 //                           tracked pp contacts: compact, ascending j, with the
 //                           tangential history of each
 //   sw    double[DEM_MAX_WALLS*3][ld]  wall tangential history
-// Cell order: cells are numbered in y-tiles of kTileRows full x-rows,
+// Cell order: cells are numbered in y-tiles of Grid::tr full x-rows,
 // z-major inside a tile, so the z-neighbours of a particle sit a tile's
 // area of cells away (tens of kB) instead of a whole x-y layer; the
 // neighbour gathers then hit L2.
@@ -36,13 +36,17 @@ namespace demhip {
 #define DEM_BLOCK 128
 #endif
 constexpr int kBlock = DEM_BLOCK;  // threads per block of the particle kernels
+// Tile (workgroup) sizes of the force kernels of systems without bonds,
+// chosen at run time (dem_set_shape); bonded systems run kBlock.
+constexpr int kForceBlocks[3] = {64, 128, 256};
 constexpr int kMaxContacts = 32;  // upper bound of max_contacts
 constexpr int kMaxNeighbors = 255;
 constexpr int kXcds = 8;          // MI355X: 8 XCDs, workgroups dealt round-robin
 #ifndef DEM_TILE_ROWS
 #define DEM_TILE_ROWS 8
 #endif
-constexpr int kTileRows = DEM_TILE_ROWS;      // y-rows per cell tile
+constexpr int kTileRows = DEM_TILE_ROWS;      // default y-rows per cell tile
+constexpr int kMaxTileRows = 32;              // dem_set_shape accepts 1..32
 
 // Logical block index that gives each XCD one contiguous run of blocks, so
 // that the neighbour gathers of one XCD stay inside its own L2.  Workgroup b
@@ -70,6 +74,8 @@ struct Grid {
   double h[3];
   int nc[3];  // cells per dimension
   int nty;    // y-tiles
+  int tr;     // y-rows per tile
+  int pad;
 };
 
 // Per-system constants of the contact model.  A context holds one system
@@ -86,11 +92,11 @@ struct SysParams {
 // Cell key of cell (cx, cy, cz): tile-major, then z, then y inside the
 // tile, then x.  Keys of x-adjacent cells of one row are consecutive.
 __host__ __device__ inline long cell_key(const Grid& g, int cx, int cy, int cz) {
-  const int ty = cy / kTileRows, ly = cy % kTileRows;
-  return (((long)ty * g.nc[2] + cz) * kTileRows + ly) * g.nc[0] + cx;
+  const int ty = cy / g.tr, ly = cy % g.tr;
+  return (((long)ty * g.nc[2] + cz) * g.tr + ly) * g.nc[0] + cx;
 }
 __host__ __device__ inline long cell_key_count(const Grid& g) {
-  return (long)g.nty * g.nc[2] * kTileRows * g.nc[0];
+  return (long)g.nty * g.nc[2] * g.tr * g.nc[0];
 }
 
 struct DevFlags {
@@ -123,10 +129,12 @@ struct StepArgs {
   const SysParams* sys;  // [n_systems] (ensembles)
   SysParams sys0;        // the single system's parameters, by value: kernel-argument
                          // loads are scalar and can be repeated at will
-  // Tiles of kBlock slots, each inside one system: (system, first, end, -).
-  // Null for a single system (tile t = slots [t kBlock, min((t+1) kBlock, n))).
+  // Tiles of `block` slots (the force kernels' workgroup size, one of
+  // kForceBlocks), each inside one system: (system, first, end, -).  Null
+  // for a single system (tile t = slots [t block, min((t+1) block, n))).
   const int4* tiles;
   int ntiles;
+  int block;
 
   const double4* xr_in;
   double4* xr_out;
@@ -167,9 +175,9 @@ struct StepArgs {
   // 1: whole force passes run split, k_detect then the force kernel over its
   // contact lists (systems without bonds); 0: one fused kernel
   int split;
-  // 1: the contact arithmetic walks each lane's own list (the wave pass
-  // otherwise; systems without bonds)
-  int lane_walk;
+  // 1: the contact arithmetic runs as a wave pass over the contacts of all
+  // lanes of a wave (systems without bonds); 0: each lane walks its own list
+  int wave;
   // time of this force pass since the walls' positions were set (moving
   // walls, corrected mode)
   double tw;
@@ -268,6 +276,15 @@ hipError_t launch_kick(const StepArgs& a, double4* xr, double* uw, const double*
                        int do_kick, int do_drift, hipStream_t s);
 // Per-tile partial sums of the current state (one partial row per tile).
 hipError_t launch_diag_current(const StepArgs& a, const double* uw, hipStream_t s);
+// The force kernels for one tile size each (a.block), in translation units
+// of their own: systems without bonds at 64 and 256 slots, and everything
+// (bonded systems included) at 128.
+hipError_t launch_force_b64(bool ref, bool ens, ForceKind kind, const StepArgs& a, bool sample,
+                            int grid_blocks, hipStream_t s);
+hipError_t launch_force_b128(bool ref, bool bond, bool ens, ForceKind kind, const StepArgs& a,
+                             bool sample, int grid_blocks, hipStream_t s);
+hipError_t launch_force_b256(bool ref, bool ens, ForceKind kind, const StepArgs& a, bool sample,
+                             int grid_blocks, hipStream_t s);
 hipError_t launch_diag_final(const double* part, int nblocks, double* out_row,
                              double step, hipStream_t s);
 // Per-system sums of the tile partials: out[s][8] over tiles [tile0[s], tile0[s+1]).

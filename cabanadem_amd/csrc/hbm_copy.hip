@@ -8,8 +8,9 @@
 //                      storing them (lane-coalesced, U blocks apart)
 //   k_copy_unrolled_nt the same with non-temporal stores
 //   k_read_sum         read-only: U loads in flight, one double per thread out
-// Each variant runs `repeats` times on buffers of `MiB` each; the JSON line
-// gives every variant's GB/s and the best copy.
+// Each variant runs `repeats` times on buffers of `MiB` each, at 4 to 64
+// workgroups of 256 threads per CU; the JSON line gives every variant's
+// GB/s (copy: bytes read + written) and the best copy.
 //   hbm_copy [MiB per buffer, default 1024] [repeats, default 20]
 #include <hip/hip_runtime.h>
 
@@ -98,7 +99,7 @@ int main(int argc, char** argv) {
   hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   double4 *a = nullptr, *b = nullptr;
   double* sums = nullptr;
-  const int grid = cus * 8; // eight workgroups of 256 threads per CU
+  const int grid = cus * 64; // the largest grid of the sweep (partial sums of the read kernel)
   if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess ||
       hipMalloc(&sums, (size_t)grid * 256 * sizeof(double)) != hipSuccess) {
     std::fprintf(stderr, "hipMalloc failed\n");
@@ -109,26 +110,40 @@ int main(int argc, char** argv) {
   const double gs = time_ms([&] {
     hipLaunchKernelGGL(k_stream_copy, dim3(cus * 32), dim3(256), 0, 0, a, b, n);
   }, reps);
-  const double u4 = time_ms([&] {
-    hipLaunchKernelGGL((k_copy_unrolled<4, false>), dim3(grid), dim3(256), 0, 0, a, b, n);
-  }, reps);
-  const double u8 = time_ms([&] {
-    hipLaunchKernelGGL((k_copy_unrolled<8, false>), dim3(grid), dim3(256), 0, 0, a, b, n);
-  }, reps);
-  const double u4nt = time_ms([&] {
-    hipLaunchKernelGGL((k_copy_unrolled<4, true>), dim3(grid), dim3(256), 0, 0, a, b, n);
-  }, reps);
-  const double rd = time_ms([&] {
-    hipLaunchKernelGGL((k_read_sum<8>), dim3(grid), dim3(256), 0, 0, a, sums, n);
-  }, reps);
   auto copy_gbps = [&](double ms) { return 2.0 * bytes / (ms * 1e-3) / 1e9; };
-  const double best = std::max(std::max(copy_gbps(gs), copy_gbps(u4)), std::max(copy_gbps(u8), copy_gbps(u4nt)));
+  // sweep: grid-stride copy and unrolled copies at 4 to 64 workgroups per CU
+  double best = copy_gbps(gs);
   std::printf("{\"bytes_read_per_launch\": %ld, \"bytes_written_per_launch\": %ld, "
-              "\"ms_per_launch\": %.6f, \"copy_GBps\": %.1f, \"copy_unrolled4_GBps\": %.1f, "
-              "\"copy_unrolled8_GBps\": %.1f, \"copy_unrolled4_nt_GBps\": %.1f, \"read_GBps\": %.1f, "
-              "\"best_copy_GBps\": %.1f, \"compute_units\": %d}\n",
-              bytes, bytes, gs, copy_gbps(gs), copy_gbps(u4), copy_gbps(u8), copy_gbps(u4nt),
-              bytes / (rd * 1e-3) / 1e9, best, cus);
+              "\"ms_per_launch\": %.6f, \"copy_GBps\": %.1f, \"compute_units\": %d, \"sweep\": [",
+              bytes, bytes, gs, copy_gbps(gs), cus);
+  const int per_cu[] = {4, 8, 16, 32, 64};
+  bool first = true;
+  for (int m : per_cu) {
+    const int g = cus * m;
+    const double t_s = time_ms([&] {
+      hipLaunchKernelGGL(k_stream_copy, dim3(g), dim3(256), 0, 0, a, b, n);
+    }, reps);
+    const double t_2 = time_ms([&] {
+      hipLaunchKernelGGL((k_copy_unrolled<2, false>), dim3(g), dim3(256), 0, 0, a, b, n);
+    }, reps);
+    const double t_4 = time_ms([&] {
+      hipLaunchKernelGGL((k_copy_unrolled<4, false>), dim3(g), dim3(256), 0, 0, a, b, n);
+    }, reps);
+    const double t_4nt = time_ms([&] {
+      hipLaunchKernelGGL((k_copy_unrolled<4, true>), dim3(g), dim3(256), 0, 0, a, b, n);
+    }, reps);
+    const double t_r = time_ms([&] {
+      hipLaunchKernelGGL((k_read_sum<8>), dim3(std::min(g, grid)), dim3(256), 0, 0, a, sums, n);
+    }, reps);
+    best = std::max(best, std::max(std::max(copy_gbps(t_s), copy_gbps(t_2)),
+                                   std::max(copy_gbps(t_4), copy_gbps(t_4nt))));
+    std::printf("%s{\"wg_per_cu\": %d, \"stream\": %.1f, \"unrolled2\": %.1f, \"unrolled4\": %.1f, "
+                "\"unrolled4_nt\": %.1f, \"read\": %.1f}",
+                first ? "" : ", ", m, copy_gbps(t_s), copy_gbps(t_2), copy_gbps(t_4), copy_gbps(t_4nt),
+                bytes / (t_r * 1e-3) / 1e9);
+    first = false;
+  }
+  std::printf("], \"best_copy_GBps\": %.1f}\n", best);
   hipFree(a);
   hipFree(b);
   hipFree(sums);

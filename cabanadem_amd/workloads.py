@@ -9,7 +9,13 @@ plain_large : 4,000,000 polydisperse spheres (200 x 200 x 100 lattice, spacing
               mu 0.4, jitter U(+-0.1 mm), u = 0, gravity -9.81 z, dt 1e-6,
               skin 0.2 mm -> cutoff 2 r_max + skin = 2.6 mm, rebuild every 20
               steps, six walls one spacing outside the lattice.
-Both are built with the library's counter-based generator (dem_generate_lattice).
+plain_dense : plain_large's lattice at spacing 2.0 mm, r ~ U[0.97, 1.03] mm, no
+              jitter (about 1.3 tracked contacts per sphere once settled).
+plain_packed: a pre-compressed bed held by its walls: spacing 1.98 mm, r ~
+              U[0.995, 1.005] mm, so every axial lattice pair overlaps by 10-30
+              um and the walls press the outer layers by about 16 um (about six
+              tracked contacts per sphere, one per neighbour in the rows).
+All are built with the library's counter-based generator (dem_generate_lattice).
 """
 from __future__ import annotations
 
@@ -107,6 +113,29 @@ def plain_dense(seed: int = 1, dims=(200, 200, 100)) -> Workload:
     lo, hi, w = _box(*dims, sp, E, nu)
     skin = 0.2e-3
     return Workload("plain_dense", st, w, 1e-6, 2 * rmax + skin, skin, lo, hi, 0.8, 0.4,
+                    (0., 0., -9.81), 20, extra={"dims": dims})
+
+
+def plain_packed(seed: int = 1, dims=(200, 200, 100)) -> Workload:
+    """The large workload as a pre-compressed bed, held together by its walls:
+    spacing 1.98 mm with radii U[0.995, 1.005] mm, no jitter, so each of the
+    six axial lattice neighbours overlaps by 10-30 um (the diagonal ones,
+    2.8 mm apart, lie outside the 2.21 mm cutoff).  The walls stand 0.984 mm
+    outside the outer lattice planes, pressing the outer spheres by about
+    16 um: the Hertz force of a sphere on a wall of the same material at
+    2^(-1/3) of a pair's overlap equals the pair's, so the bed starts close
+    to balance and stays packed under gravity (a bottom sphere carries the
+    weight of 100 layers, about 0.01 N, against about 0.015 N per contact).
+    Otherwise plain_large (E 1e7, nu 0.3, e 0.8, mu 0.4, gravity, dt 1e-6,
+    skin 0.2 mm, cutoff 2 r_max + skin, rebuild every 20 steps)."""
+    E, nu = 1e7, 0.3
+    sp = 1.98e-3
+    rmax = 1.005e-3
+    st = lattice(*dims, spacing=sp, jitter=0.0, rmin=0.995e-3, rmax=rmax, vsigma=0.0,
+                 rho=2600., E=E, nu=nu, seed=seed)
+    lo, hi, w = _box(*dims, sp, E, nu, off=0.984e-3)
+    skin = 0.2e-3
+    return Workload("plain_packed", st, w, 1e-6, 2 * rmax + skin, skin, lo, hi, 0.8, 0.4,
                     (0., 0., -9.81), 20, extra={"dims": dims})
 
 

@@ -35,7 +35,7 @@ extern "C" {
 #define DEM_MAX_WALLS 8
 #define DEM_MAX_GROUPS 8
 #define DEM_MAX_BONDS 32
-#define DEM_ABI_VERSION 4
+#define DEM_ABI_VERSION 5
 
 enum dem_status {
   DEM_OK = 0,
@@ -96,7 +96,7 @@ typedef struct dem_config {
                         * or DEM_REBUILD_AUTO: rebuild when the displacement since the last
                         * build may reach skin/2 (needs skin > 0) */
   int physics_mode;    /* DEM_MODE_* */
-  int block_size;      /* threads per block for the particle kernels (0 or 128) */
+  int block_size;      /* force-kernel tile size: 0 (128), 64, 128 or 256 (dem_set_shape) */
   int reserved;
 } dem_config;
 
@@ -331,15 +331,30 @@ int dem_system_diagnostics(dem_ctx* ctx, double* out);
 int dem_set_launch(dem_ctx* ctx, int grid_blocks);
 /* Force-pass shape (systems without bonds).  Results are bitwise the same
  * in every shape; which is fastest depends on the contact density.
- *   DEM_FORCE_PASS_FUSED (default): one kernel per force pass; each lane
- *     finds its particle's contacts over the neighbour row, then the wave
- *     runs the contact arithmetic of all its contacts spread over its lanes;
+ *   DEM_FORCE_PASS_FUSED (default): one kernel per force pass, each lane
+ *     finding its particle's contacts over the neighbour row and walking
+ *     them (the contact arithmetic, the kicks);
  *   DEM_FORCE_PASS_SPLIT: the detection as a kernel of its own that writes
- *     each particle's overlapping neighbours, then the force kernel;
- *   DEM_FORCE_PASS_LANE: one kernel, each lane walking its own contacts.
- * Bonded systems always run fused with each lane walking its own contacts. */
-enum dem_force_pass { DEM_FORCE_PASS_FUSED = 0, DEM_FORCE_PASS_SPLIT = 1, DEM_FORCE_PASS_LANE = 2 };
+ *     each particle's overlapping neighbours, then the force kernel running
+ *     the contact arithmetic as a wave pass (below);
+ *   DEM_FORCE_PASS_WAVE: one kernel; each lane finds its contacts, then the
+ *     wave runs the contact arithmetic of all its lanes' contacts spread
+ *     over its 64 lanes, in the same summation order.
+ * Bonded systems always run DEM_FORCE_PASS_FUSED. */
+enum dem_force_pass { DEM_FORCE_PASS_FUSED = 0, DEM_FORCE_PASS_SPLIT = 1, DEM_FORCE_PASS_WAVE = 2 };
 int dem_set_force_pass(dem_ctx* ctx, int mode);
+
+/* Tile shapes.  block: the force kernels' workgroup size and tile of
+ * consecutive slots, 64, 128 or 256 (0: 128; systems with bonds run 128).
+ * tile_rows: the binning grid numbers its cells in y-tiles of tile_rows
+ * full x-rows, z-major inside a tile (1 to 32; 0: 8), which sets how far
+ * apart in memory the z-neighbours of a particle sit after the re-sort.
+ * Particle states do not depend on block; sampled diagnostics sum per tile,
+ * so they agree to rounding across block sizes.  A new tile_rows re-sorts
+ * the particles at the next step, which changes the order of the contact
+ * sums (results agree to rounding, as across rebuild cadences). */
+int dem_set_shape(dem_ctx* ctx, int block, int tile_rows);
+int dem_get_shape(dem_ctx* ctx, int* block, int* tile_rows);
 
 /* Captured step sequences: with enable set (and no diagnostics sampling or
  * kernel timing on), dem_step runs whole cycles of 2 rebuild_every steps
@@ -348,21 +363,27 @@ int dem_set_force_pass(dem_ctx* ctx, int mode);
  * usual.  Results are identical either way. */
 int dem_set_graph(dem_ctx* ctx, int enable);
 
-/* Autotuner: times trial_steps steps of every candidate configuration from
- * the current state -- neighbour-rebuild cadence (5, 10, 20, 40, 80,
- * DEM_REBUILD_AUTO and the configured one, when a skin is set; a candidate
- * that breaks the skin check is rejected), force-kernel launch shape (one workgroup per tile, or
- * persistent grids of 2 and 8 workgroups per compute unit), captured
- * graphs on or off and the force-pass shape (DEM_FORCE_PASS_*, systems
- * without bonds) -- restores the state exactly, and applies the fastest.
- * Rebuild cadence changes the summation order of the contact sums after a
- * re-sort, so runs with different cadences agree to rounding, not bitwise. */
+/* Autotuner: times trial_steps steps of candidate configurations from the
+ * current state and applies the fastest, restoring the state exactly before
+ * each trial and after the last.  Stage 1 tries every combination of
+ * neighbour-rebuild cadence (5, 10, 20, 40, 80, DEM_REBUILD_AUTO and the
+ * configured one, when a skin is set; a candidate that breaks the skin
+ * check is rejected), force-kernel launch grid (one workgroup per tile, or
+ * persistent grids of 2 and 8 workgroups per compute unit), captured graphs
+ * on or off and force-pass shape (DEM_FORCE_PASS_*, systems without bonds)
+ * at the context's tile shapes; stage 2 tries, at the best of stage 1, the
+ * tile shapes of dem_set_shape: blocks of 64, 128 and 256 (systems without
+ * bonds) times cell tiles of 4, 8 and 16 rows.  Rebuild cadence and cell
+ * tiles change the summation order of the contact sums after a re-sort, so
+ * runs with different ones agree to rounding, not bitwise. */
 #define DEM_TUNE_MAX_TRIALS 128
 typedef struct dem_tune_result {
   int rebuild_every;   /* chosen configuration */
   int grid_blocks;
   int use_graph;
   int force_pass;      /* DEM_FORCE_PASS_* */
+  int block;           /* dem_set_shape */
+  int tile_rows;
   int n_trials;
   int reserved;
   double ms_per_step;  /* of the chosen configuration */
@@ -370,6 +391,8 @@ typedef struct dem_tune_result {
   int trial_grid_blocks[DEM_TUNE_MAX_TRIALS];
   int trial_graph[DEM_TUNE_MAX_TRIALS];
   int trial_force_pass[DEM_TUNE_MAX_TRIALS];
+  int trial_block[DEM_TUNE_MAX_TRIALS];
+  int trial_tile_rows[DEM_TUNE_MAX_TRIALS];
   double trial_ms_per_step[DEM_TUNE_MAX_TRIALS]; /* < 0: rejected */
 } dem_tune_result;
 int dem_autotune(dem_ctx* ctx, int trial_steps, dem_tune_result* out);

@@ -50,7 +50,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_no_device_error_path():
     lib = abi.lib()
-    assert lib.dem_abi_version() == abi.DEM_ABI_VERSION == 4
+    assert lib.dem_abi_version() == abi.DEM_ABI_VERSION == 5
     assert lib.dem_last_error(None) is not None
 
 

@@ -15,6 +15,12 @@
   oracle with those materials, KE/momentum within 1e-8 at every 500 steps;
   the same systems with other materials land far outside that bar, so the
   per-system materials are what the GPU used.
+* Packed bed (the contact-dense variant of the large workload, about six
+  contacts per sphere) at 13,824 spheres in both arithmetic modes: after
+  200 GPU steps neighbour sets bit-exact, forces, torques and histories
+  within 1e-12; then 500 steps with KE/momentum within ten times the
+  oracle's own sensitivity to a one-ulp restart (the bed is
+  ill-conditioned: 2e-3 in KE).
 * Surrogate: the 64,000-sphere column's radius graph on the device equals
   the oracle's cell-list neighbour sets (the same |x_i - x_j|^2 <= R^2
   criterion); the 10-layer, 128-wide network's forward on that graph against
@@ -143,6 +149,68 @@ def test_plain_large_run_from_settled_state(gpu_lib, settled_large):
     assert normwise(got.x - st.x, ref.x - st.x) <= TOL_RUN
     assert normwise(got.u, ref.u) <= TOL_RUN
     _say(f"plain large: KE/momentum worst {err:.2e}, |du|/|u| {normwise(got.u, ref.u):.2e}")
+    sysm.close()
+
+
+@pytest.mark.parametrize("mode", [abi.MODE_REFERENCE, abi.MODE_CORRECTED])
+def test_packed_bed_contact_dense(gpu_lib, mode):
+    wl = W.plain_packed(dims=(24, 24, 24))
+    sysm = wl.system(mode=mode)
+    sysm.step(200)
+    st = sysm.download(abi.ParticleState(wl.state.n, wl.max_contacts, 6))
+    sysm.close()
+    # a fresh context on that state: rows, one force pass
+    sysm = wl.system(mode=mode)
+    sysm.upload(st)
+    sysm.build_neighbors()
+    max_n = max(sysm.counters().max_neighbors_seen, 1)
+    got_rows = sysm.neighbors(max_n)
+    ref_rows = O.neighbors_cell(st.x, wl.cutoff, wl.lo, wl.hi, max_n=max_n)
+    assert same_neighbor_sets(got_rows, ref_rows)
+    ref = st.copy()
+    O.compute_forces(ref, wl.walls, ref_rows[0], ref_rows[1], _prm(wl, mode))
+    sysm.compute_forces()
+    got = sysm.download(abi.ParticleState(st.n, wl.max_contacts, 6))
+    assert normwise(got.force, ref.force) <= TOL_STEP
+    assert normwise(got.torque, ref.torque) <= TOL_STEP
+    npairs = compare_history_arrays(got, ref, TOL_STEP)
+    for comp in ("sw_x", "sw_y", "sw_z"):
+        assert normwise(getattr(got, comp), getattr(ref, comp)) <= TOL_STEP
+    cbar = npairs / st.n
+    assert cbar > 5.0, cbar  # six axial neighbours, fewer on the faces
+    sysm.close()
+    # a run from the same state.  A compressed bed is ill-conditioned: each
+    # net force is the small difference of six large contact forces, so the
+    # oracle itself, restarted one ulp away in u, moves KE by about 2e-3 and
+    # the momenta by up to 3e-2 within 500 steps; the bar per column is ten
+    # times that sensitivity (or the run bar, whichever is larger).
+    nsteps, every = 500, 50
+    sysm = wl.system(mode=mode)
+    sysm.upload(st)
+    sysm.set_diag_every(every)
+    sysm.step(nsteps)
+    diag = sysm.diag_history()
+    kw = dict(nsteps=nsteps, rc=wl.cutoff, lo=wl.lo, hi=wl.hi, rebuild_every=wl.rebuild_every,
+              max_n=64, diag_every=every)
+    ref = st.copy()
+    t0 = time.perf_counter()
+    diag_ref, _ = O.run(ref, wl.walls, _prm(wl, mode), **kw)
+    pert = st.copy()
+    pert.u[:] = np.nextafter(pert.u, np.inf)
+    diag_pert, _ = O.run(pert, wl.walls, _prm(wl, mode), **kw)
+    assert np.array_equal(diag[:, 0], diag_ref[:, 0])
+    worst = []
+    for k in range(1, 6):
+        scale = np.abs(diag_ref[:, k]).max()
+        sens = np.abs(diag_pert[:, k] - diag_ref[:, k]).max() / scale
+        err = np.abs(diag[:, k] - diag_ref[:, k]).max() / scale
+        assert err <= max(TOL_RUN, 10. * sens), (k, err, sens)
+        worst.append((err, sens))
+    got = sysm.download(abi.ParticleState(st.n, wl.max_contacts, 6))
+    assert normwise(got.u, ref.u) <= max(TOL_RUN, 10. * normwise(pert.u, ref.u))
+    _say(f"packed bed, mode {mode}: c-bar {cbar:.2f}, oracle 2 x {nsteps} steps in "
+         f"{time.perf_counter() - t0:.1f} s; KE/momentum (gap, oracle 1-ulp sensitivity) "
+         + ", ".join(f"({e:.1e}, {q:.1e})" for e, q in worst))
     sysm.close()
 
 

@@ -47,10 +47,13 @@ def assert_same(a, b, where):
         assert np.array_equal(a[f], b[f]), f"{where}: field {f} differs"
 
 
-@pytest.mark.parametrize("mode", [0, 1])
-def test_ensemble_equals_independent_runs_bitwise(gpu_lib, mode):
+@pytest.mark.parametrize("mode,block,rows", [(0, 0, 0), (1, 0, 0), (0, 64, 16), (1, 256, 4)])
+def test_ensemble_equals_independent_runs_bitwise(gpu_lib, mode, block, rows):
+    """Also at other force tiles (each system's tile table for that size)
+    and other cell tiles (each system's grid and key range re-laid)."""
     wl = small_ensemble()
     ens = wl.system(mode=mode)
+    ens.set_shape(block, rows)
     ens.set_diag_every(25)
     ens.step(300)
     c, w = contact_means(ens)
@@ -58,6 +61,7 @@ def test_ensemble_equals_independent_runs_bitwise(gpu_lib, mode):
     got = split(ens.download(), ens.offsets)
     for k in range(len(wl.systems)):
         one = wl.single(k, mode=mode)
+        one.set_shape(block, rows)
         one.step(300)
         want = one.download()
         assert_same(got[k], {f: getattr(want, f) for f in FIELDS}, f"system {k}")
@@ -124,9 +128,10 @@ def test_ensemble_argument_errors(gpu_lib):
 
 
 def test_force_pass_shapes_ensemble_are_bitwise_equal(gpu_lib):
-    """The split force pass (detection kernel + force kernel) and the
-    lane-walk pass over the tile table of an ensemble equal the default fused
-    pass, graphs and persistent grid included."""
+    """The split force pass (detection kernel + force kernel) and the wave
+    pass over the tile table of an ensemble equal the default fused pass
+    (each lane walking its own contacts), graphs and persistent grid
+    included."""
     wl = small_ensemble(nsys=5, seed=9)
     a = wl.system()
     b = wl.system()
@@ -134,17 +139,31 @@ def test_force_pass_shapes_ensemble_are_bitwise_equal(gpu_lib):
     b.set_launch(24)
     b.set_graph(True)
     c = wl.system()
-    c.set_force_pass(abi.FORCE_PASS_LANE)
+    c.set_force_pass(abi.FORCE_PASS_WAVE)
+    # other force tiles: their own tile tables
+    d = wl.system()
+    d.set_force_pass(abi.FORCE_PASS_WAVE)
+    d.set_shape(64, 0)
+    d.set_graph(True)
+    e = wl.system()
+    e.set_force_pass(abi.FORCE_PASS_SPLIT)
+    e.set_shape(256, 0)
+    e.set_launch(12)
     for nsteps in (17, 63):
-        for s_ in (a, b, c):
+        for s_ in (a, b, c, d, e):
             s_.step(nsteps)
-        sa, sb, sc = a.download(), b.download(), c.download()
-        for f in FIELDS:
-            assert np.array_equal(getattr(sa, f), getattr(sb, f)), (nsteps, f)
-            assert np.array_equal(getattr(sa, f), getattr(sc, f)), (nsteps, f)
+        sa = a.download()
+        for s_ in (b, c, d, e):
+            got = s_.download()
+            for f in FIELDS:
+                assert np.array_equal(getattr(sa, f), getattr(got, f)), (nsteps, f)
     assert sa.ss_cnt.sum() > 0
-    assert np.array_equal(a.system_diagnostics(), b.system_diagnostics())
-    for s_ in (a, b, c):
+    da = a.system_diagnostics()
+    assert np.array_equal(da, b.system_diagnostics())
+    for s_ in (d, e):  # per-tile partials of other sizes: rounding only
+        dd = s_.system_diagnostics()
+        assert np.all(np.abs(dd - da) <= 1e-13 * np.abs(da).max(axis=0)), dd - da
+    for s_ in (a, b, c, d, e):
         s_.close()
 
 
@@ -188,7 +207,11 @@ def test_autotune_restores_state_and_applies_choice(gpu_lib):
     b.set_launch(r.grid_blocks)
     b.set_graph(bool(r.use_graph))
     b.set_force_pass(r.force_pass)
+    b.set_shape(r.block, r.tile_rows)
+    assert a.shape() == (r.block, r.tile_rows)
     assert {t["force_pass"] for t in trials} == {0, 1, 2}
+    assert {t["block"] for t in trials} == {64, 128, 256}
+    assert {t["tile_rows"] for t in trials} == {4, 8, 16}
     a.step(100)
     b.step(100)
     sa, sb = a.download(), b.download()

@@ -140,9 +140,11 @@ def test_integrator_stages_bit_exact(gpu_lib):
     assert np.array_equal(got.omega, ref.omega)
 
 
-@pytest.mark.parametrize("mode,K", [(0, 20), (1, 20), (0, 1)])
-def test_run_ke_momentum_history(gpu_lib, mode, K):
-    # the small plain-DEM configuration's physics at 12x12x14 = 2016 spheres
+@pytest.mark.parametrize("mode,K,block,rows", [(0, 20, 0, 0), (1, 20, 0, 0), (0, 1, 0, 0),
+                                               (0, 20, 64, 4), (1, 20, 256, 16)])
+def test_run_ke_momentum_history(gpu_lib, mode, K, block, rows):
+    # the small plain-DEM configuration's physics at 12x12x14 = 2016 spheres,
+    # at the default tile shapes and at other force tiles and cell tiles
     st, w, lo, hi = granular_box(12, 12, 14, spacing_r=2.05, jitter_r=0.02, vsigma=0.05,
                                  seed=11)
     dt, rc = 1e-6, 2.2e-3
@@ -152,6 +154,8 @@ def test_run_ke_momentum_history(gpu_lib, mode, K):
     diag_ref, _ = O.run(ref, w, oracle_params(dt, gravity=g, mode=mode), nsteps=nsteps, rc=rc,
                         lo=lo, hi=hi, rebuild_every=K, max_n=64, diag_every=every)
     sysm = make_system(st, w, dt=dt, rc=rc, lo=lo, hi=hi, gravity=g, rebuild_every=K, mode=mode)
+    sysm.set_shape(block, rows)
+    assert sysm.shape() == (block or 128, rows or 8)
     sysm.set_diag_every(every)
     sysm.step(nsteps)
     diag = sysm.diag_history()
@@ -180,19 +184,24 @@ def test_run_ke_momentum_history(gpu_lib, mode, K):
 
 @pytest.mark.parametrize("mode", [0, 1])
 def test_force_pass_shapes_are_bitwise_equal(gpu_lib, mode):
-    """The three force-pass shapes: fused with the contact arithmetic spread
-    over each wave (the default), split (a detection kernel, then the force
-    kernel over its contact lists) and fused with each lane walking its own
-    contacts.  A dense run with rebuilds every 5 steps and sampled
-    diagnostics, then a stage-wise force pass: every field bitwise equal."""
+    """The three force-pass shapes -- fused with each lane walking its own
+    contacts (the default), split (a detection kernel, then the force kernel
+    over its contact lists) and one kernel with the contact arithmetic
+    spread over each wave -- at force tiles of 64, 128 and 256 slots.  A
+    dense run with rebuilds every 5 steps and sampled diagnostics, then a
+    stage-wise force pass: every particle field bitwise equal; the sampled
+    sums (per-tile partials) bitwise equal at the same tile size and within
+    1e-13 across tile sizes."""
     st, w, lo, hi, dt, rc = _dense_state(seed=8, mode=mode)
     g = (0., 0., -9.81)
     kw = dict(dt=dt, rc=rc, lo=lo, hi=hi, gravity=g, mode=mode, rebuild_every=5)
-    shapes = (abi.FORCE_PASS_FUSED, abi.FORCE_PASS_SPLIT, abi.FORCE_PASS_LANE)
+    shapes = [(block, p) for block in (128, 64, 256)
+              for p in (abi.FORCE_PASS_FUSED, abi.FORCE_PASS_SPLIT, abi.FORCE_PASS_WAVE)]
     systems = []
-    for shape in shapes:
+    for block, shape in shapes:
         s_ = make_system(st, w, **kw)
         s_.set_force_pass(shape)
+        s_.set_shape(block, 0)
         s_.set_diag_every(7)
         systems.append(s_)
     for nsteps in (1, 23, 36):
@@ -201,22 +210,46 @@ def test_force_pass_shapes_are_bitwise_equal(gpu_lib, mode):
     fields = ("x", "u", "omega", "force", "torque", "ss_idx", "ss_cnt", "ss_x", "ss_y", "ss_z",
               "sw_x", "sw_y", "sw_z")
     ref = systems[0].download()
+    d0 = systems[0].diag_history()
     assert ref.ss_cnt.sum() > 100 and ref.ss_cnt.max() >= 4
-    for s_ in systems[1:]:
-        assert np.array_equal(systems[0].diag_history(), s_.diag_history())
+    for (block, _), s_ in zip(shapes[1:], systems[1:]):
+        d = s_.diag_history()
+        if block == 128:
+            assert np.array_equal(d0, d)
+        else:
+            assert np.array_equal(d0[:, 0], d[:, 0])
+            for k in range(1, d0.shape[1]):
+                scale = max(np.abs(d0[:, k]).max(), 1e-300)
+                assert np.abs(d[:, k] - d0[:, k]).max() <= 1e-13 * scale, (block, k)
         got = s_.download()
         for f in fields:
-            assert np.array_equal(getattr(ref, f), getattr(got, f)), f
+            assert np.array_equal(getattr(ref, f), getattr(got, f)), (block, f)
     for s_ in systems:
         s_.build_neighbors()
         s_.compute_forces()
     ref = systems[0].download()
-    for s_ in systems[1:]:
+    for (block, _), s_ in zip(shapes[1:], systems[1:]):
         got = s_.download()
         for f in fields:
-            assert np.array_equal(getattr(ref, f), getattr(got, f)), f
+            assert np.array_equal(getattr(ref, f), getattr(got, f)), (block, f)
     for s_ in systems:
         s_.close()
+
+
+def test_tile_shape_arguments(gpu_lib):
+    st, w, lo, hi, dt, rc = _dense_state(seed=2)
+    sysm = make_system(st, w, dt=dt, rc=rc, lo=lo, hi=hi)
+    assert sysm.shape() == (128, 8)
+    for bad in ((32, 0), (512, 0), (0, -1), (0, 33)):
+        with pytest.raises(abi.DemError):
+            sysm.set_shape(*bad)
+    assert sysm.shape() == (128, 8)
+    sysm.set_shape(256, 1)
+    assert sysm.shape() == (256, 1)
+    sysm.step(3)
+    sysm.set_shape(0, 0)
+    assert sysm.shape() == (128, 8)
+    sysm.close()
 
 
 def test_automatic_rebuild_cadence_matches_every_step_rebuilds(gpu_lib):
@@ -239,6 +272,8 @@ def test_automatic_rebuild_cadence_matches_every_step_rebuilds(gpu_lib):
     c = sysm.counters()
     # 0.2 mm at up to ~1.5 m/s: a rebuild every ~100 steps or more, not every step
     assert 3 <= c.rebuilds <= nsteps // 20, c.rebuilds
+    # about three checks per build once under way (a doubling series from rest first)
+    assert c.displacement_checks <= 5 * c.rebuilds + 12, (c.displacement_checks, c.rebuilds)
     diag = sysm.diag_history()
     assert np.array_equal(diag[:, 0], diag_ref[:, 0])
     for k in range(1, 6):
