@@ -245,10 +245,12 @@ def run_ensemble(args):
     sysm.set_diag_every(0)
     sysm.set_graph(use_graph)
     sysm.sync()
+    c0 = sysm.counters()
     t0 = time.perf_counter()
     sysm.step(args.steps)
     sysm.sync()
     elapsed = time.perf_counter() - t0
+    c1 = sysm.counters()
     value = n * args.steps / elapsed
     k_ms = ms[0] / max(launches[0], 1)
     b_alg = W.bytes_reference_passes(nbar, cbar, wbar)
@@ -276,6 +278,9 @@ def run_ensemble(args):
                      "launches": launches[0]},
         "contacts": {"mean_neighbors": nbar, "mean_contacts": cbar, "mean_wall_contacts": wbar,
                      "contact_pairs_per_s": cbar * value},
+        "step_model": {"rebuilds": int(c1.rebuilds - c0.rebuilds),
+                       "displacement_checks": int(c1.displacement_checks - c0.displacement_checks),
+                       "kernel_share": k_ms / (elapsed / args.steps * 1e3)},
     }
     sysm.close()
     del sysm

@@ -96,6 +96,9 @@ struct dem_ctx {
   bool walls_move = false;
 
   // automatic rebuild cadence (rebuild_every = DEM_REBUILD_AUTO)
+  int* d_work = nullptr;       // [n] particles with contacts (split force passes)
+  int* d_work_count = nullptr; // [1]
+  int cus = 256;               // compute units of the device
   unsigned long long* d_disp = nullptr; // [2] max |x - xb|^2, max |u|^2 (bits)
   unsigned long long* h_disp = nullptr; // pinned host copy of d_disp (a pageable
                                         // destination costs a staged copy per check)
@@ -185,6 +188,7 @@ struct dem_ctx {
     hipFree(nn); hipFree(nbr); hipFree(xb); hipFree(key); hipFree(rank); hipFree(key_sorted);
     hipFree(cell_count); hipFree(cell_start); hipFree(perm); hipFree(inv); hipFree(scan_tmp);
     hipFree(flags); hipFree(diag_part); hipFree(diag_rows); hipFree(d_disp);
+    hipFree(d_work); hipFree(d_work_count);
     if (h_disp) hipHostFree(h_disp);
     for (int b = 0; b < 3; ++b) { hipFree(d_tiles_b[b]); hipFree(d_tile0_b[b]); }
     hipFree(d_sys); hipFree(d_sysid); hipFree(d_grids);
@@ -243,6 +247,10 @@ struct dem_ctx {
     }
     a.fext = fext[cS];
     a.split = (force_pass == DEM_FORCE_PASS_SPLIT && !bonds_on) ? 1 : 0;
+    a.work = d_work;
+    a.work_count = d_work_count;
+    a.sysid = nsys > 1 ? d_sysid : nullptr;
+    a.list_grid = 8 * cus;
     a.wave = (force_pass != DEM_FORCE_PASS_FUSED && !bonds_on) ? 1 : 0;
     a.tw = (double)(step + 1 - wall_step0) * cfg.dt;
     return a;
@@ -782,6 +790,12 @@ int create_ctx(const dem_config* common, int nsys, const dem_system_desc* sd, de
   }
   hipError_t e = hipStreamCreate(&c->stream);
   if (e != hipSuccess) { int r = c->hip_fail(e, "hipStreamCreate"); delete c; return r; }
+  {
+    int dev = 0, cu = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cu > 0)
+      c->cus = cu;
+  }
   const size_t n = (size_t)c->n, ld = (size_t)c->ld;
   e = hipSuccess;
   for (int b = 0; b < 2; ++b) {
@@ -808,6 +822,8 @@ int create_ctx(const dem_config* common, int nsys, const dem_system_desc* sd, de
   c->scan_tmp = dalloc<int>((size_t)(c->ncell / 4096 + 2), &e);
   c->flags = dalloc<DevFlags>(1, &e);
   c->d_disp = dalloc<unsigned long long>(2, &e);
+  c->d_work = dalloc<int>(n, &e);
+  c->d_work_count = dalloc<int>(1, &e);
   c->diag_part = dalloc<double>((size_t)c->ntiles_b[0] * 8, &e); // the most tiles (64 slots)
   c->d_sys = dalloc<SysParams>((size_t)nsys, &e);
   for (int b = 0; b < 3; ++b) c->d_tile0_b[b] = dalloc<int>((size_t)nsys + 1, &e);

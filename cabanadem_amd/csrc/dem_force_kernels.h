@@ -43,6 +43,12 @@ constexpr int kPhase1Batch = DEM_PHASE1_BATCH; // neighbour gathers issued toget
 #ifndef DEM_DETECT_BATCH
 #define DEM_DETECT_BATCH 4
 #endif
+#ifndef DEM_DEFER_MIN_WAVES
+#define DEM_DEFER_MIN_WAVES 1
+#endif
+#ifndef DEM_DEFER_BATCH
+#define DEM_DEFER_BATCH 4 // neighbour gathers in flight in k_defer (70 VGPRs, 7 waves/SIMD)
+#endif
 #ifndef DEM_PRE_MIN_WAVES
 #define DEM_PRE_MIN_WAVES 1
 #endif
@@ -498,18 +504,42 @@ __device__ __forceinline__ void wave_contacts(const StepArgs& a, const SysParams
 // whose parameters travel in the kernel arguments.  Each lane walks its own
 // contact list, or (a.wave, systems without bonds) the wave runs the contact
 // arithmetic of all its lanes spread over the lanes (wave_contacts).
-template <bool REF, bool PP, bool PW, int KICK, bool ACCUM, bool DIAG, bool BOND, bool ENS, bool PRE, int B>
+// PASS (systems without bonds, whole force passes, no sampling):
+//   0  everything in one pass;
+//   1  deferral: phase 1 only; a particle without particle or wall contacts
+//      finishes here (its force is the start value), the others write their
+//      contact ids to cj_out and their count to cmeta_out and join the work
+//      list a.work (one atomic per wave); the FP64 contact arithmetic is not
+//      compiled in, so this pass runs at the occupancy of a gather kernel;
+//   2  the work list: tile t is entries [t B, (t+1) B) of a.work, run as
+//      with PRE (ids in cj_out).
+// Every particle's arithmetic is that of PASS 0, so the results are bitwise
+// the same.
+template <bool REF, bool PP, bool PW, int KICK, bool ACCUM, bool DIAG, bool BOND, bool ENS, bool PRE, int B,
+          int PASS = 0>
 __device__ __forceinline__ void force_tile(const StepArgs& a, const int tile, int* s_cj) {
-  constexpr bool WAVE = PP && !BOND;
+  constexpr bool DEFER = PASS == 1, LIST = PASS == 2;
+  static_assert(PASS == 0 || (PP && PW && !ACCUM && !DIAG && !BOND), "deferral: whole passes only");
+  static_assert(!LIST || PRE, "the work-list pass reads its contact ids from cj_out");
+  constexpr bool WAVE = PP && !BOND && PASS == 0;
   const int tid = threadIdx.x;
   int sys = 0, first = tile * B, end = a.n;
-  if (ENS) {
-    const int4 t = a.tiles[tile];
-    sys = t.x; first = t.y; end = t.z;
+  int i;
+  bool act;
+  if (LIST) {
+    const int k = tile * B + tid;
+    act = k < *a.work_count;
+    i = act ? a.work[k] : 0;
+    if (ENS && act) sys = a.sysid[i];
+  } else {
+    if (ENS) {
+      const int4 t = a.tiles[tile];
+      sys = t.x; first = t.y; end = t.z;
+    }
+    i = first + tid;
+    act = i < end;
   }
   const SysParams& P = ENS ? a.sys[sys] : a.sys0;
-  const int i = first + tid;
-  const bool act = i < end;
   const long ld = a.ld;
   unsigned char* s_op = reinterpret_cast<unsigned char*>(s_cj + (long)(a.max_c > 0 ? a.max_c : 1) * B);
   double* s_con = reinterpret_cast<double*>(
@@ -528,9 +558,14 @@ __device__ __forceinline__ void force_tile(const StepArgs& a, const int tile, in
   const bool pp_on = !a.nbond || a.unbonded_contacts;
   if (act) {
     xi = a.xr_in[i];
-    load6(a.uw_in + 6l * i, ui);
-    mI = a.mI[i];
-    if (ACCUM) {
+    // deferral: u, omega, m, I and the start force only after phase 1, for
+    // the particles that finish here (fewer registers live over the gathers)
+    if (!DEFER) {
+      load6(a.uw_in + 6l * i, ui);
+      mI = a.mI[i];
+    }
+    if (DEFER) {
+    } else if (ACCUM) {
       double ft[6];
       load6(a.ft + 6l * i, ft);
       F[0] = ft[0]; F[1] = ft[1]; F[2] = ft[2];
@@ -574,18 +609,19 @@ __device__ __forceinline__ void force_tile(const StepArgs& a, const int tile, in
       // Phase 1: contact detection over the neighbour row.
       nnb = a.nn[i];
       nc = 0;
-      // Batches of kPhase1Batch independent row loads, then gathers, then
+      constexpr int NB = DEFER ? DEM_DEFER_BATCH : kPhase1Batch;
+      // Batches of NB independent row loads, then gathers, then
       // tests: several gathers in flight per lane instead of one chain.
-      for (int k0 = 0; k0 < nnb; k0 += kPhase1Batch) {
-        int jb[kPhase1Batch];
-        double4 xb[kPhase1Batch];
+      for (int k0 = 0; k0 < nnb; k0 += NB) {
+        int jb[NB];
+        double4 xb[NB];
 #pragma unroll
-        for (int q = 0; q < kPhase1Batch; ++q)
+        for (int q = 0; q < NB; ++q)
           jb[q] = (k0 + q < nnb) ? a.nbr[(k0 + q) * ld + i] : i;
 #pragma unroll
-        for (int q = 0; q < kPhase1Batch; ++q) xb[q] = a.xr_in[jb[q]];
+        for (int q = 0; q < NB; ++q) xb[q] = a.xr_in[jb[q]];
 #pragma unroll
-        for (int q = 0; q < kPhase1Batch; ++q) {
+        for (int q = 0; q < NB; ++q) {
           if (k0 + q >= nnb) break;
           const double4 xj = xb[q];
           const double d0 = xi.x - xj.x, d1 = xi.y - xj.y, d2 = xi.z - xj.z;
@@ -595,7 +631,10 @@ __device__ __forceinline__ void force_tile(const StepArgs& a, const int tile, in
           // the rounding of s*s, so no contact is ever filtered out here.
           if (r2 < s * s * (1. + 1e-12)) {
             if (s - sqrt(r2) > 0. && !(nbk > 0 && bonded_to(a, i, nbk, jb[q]))) {
-              if (nc < a.max_c) s_cj[nc * B + tid] = jb[q];
+              if (nc < a.max_c) {
+                if (DEFER) a.cj_out[nc * ld + i] = jb[q];
+                else s_cj[nc * B + tid] = jb[q];
+              }
               ++nc;
             }
           }
@@ -628,7 +667,7 @@ __device__ __forceinline__ void force_tile(const StepArgs& a, const int tile, in
     // every lane of the wave, active or not: the contacts of the wave
     wave_lds_sync();
     wave_contacts<REF, PRE, B>(a, P, first, act ? nc : 0, s_cj, s_op, s_con, F, T);
-  } else if (PP && pp_on && act) {
+  } else if (!DEFER && PP && pp_on && act) {
     // Phase 2 walking each lane's own list:
     // contact arithmetic, merged with last step's tracked list.
     const int oc = meta_count(meta);
@@ -658,9 +697,39 @@ __device__ __forceinline__ void force_tile(const StepArgs& a, const int tile, in
       a.ch_out[(3 * m + 2) * ld + i] = h[2];
     }
   }
-  if (act) {
+  bool defer = false;
+  if (DEFER) {
+    if (act) {
+      defer = nc > 0;
+      for (int w = 0; w < P.n_walls && !defer; ++w) {
+        const WallConst& W = P.walls[w];
+        double wx[3] = {W.x[0], W.x[1], W.x[2]};
+        if (!REF)
+          for (int d = 0; d < 3; ++d) wx[d] += W.u[d] * a.tw;
+        const double q0 = xi.x - wx[0], q1 = xi.y - wx[1], q2 = xi.z - wx[2];
+        const double tmp = q0 * W.n[0] + q1 * W.n[1] + q2 * W.n[2];
+        defer = xi.w - tmp > 0.;
+      }
+      if (defer) a.cmeta_out[i] = make_meta(nc, 0);
+    }
+    // the wave's deferred particles into the work list, in lane order
+    const unsigned long long mask = __ballot(defer);
+    if (mask) {
+      const int lane = tid & 63, lead = __builtin_ffsll((long long)mask) - 1;
+      int base = 0;
+      if (lane == lead) base = atomicAdd(a.work_count, __popcll(mask));
+      base = __shfl(base, lead);
+      if (defer) a.work[base + __popcll(mask & ((1ull << lane) - 1ull))] = i;
+    }
+  }
+  if (DEFER && act && !defer) {
+    load6(a.uw_in + 6l * i, ui);
+    mI = a.mI[i];
+    initial_force(a, i, mI.x, F);
+  }
+  if (act && !defer) {
     int wn = meta_walls(meta);
-    if (PW) {
+    if (PW && !DEFER) {
       const int wm = wn;
       wn = 0;
       for (int w = 0; w < P.n_walls; ++w) {
@@ -690,6 +759,7 @@ __device__ __forceinline__ void force_tile(const StepArgs& a, const int tile, in
         // no overlap: the history is zero (:666-670); the cleared bit says so
       }
     }
+    if (DEFER) wn = 0; // no wall overlaps: any old history is dropped
     if (PP) a.cmeta_out[i] = make_meta(nc, wn);
     else if (PW) a.cmeta_in[i] = make_meta(nc, wn); // walls only: in place
     if (!ACCUM) apply_force_bc(a, i, F);
@@ -760,6 +830,27 @@ __global__ __launch_bounds__(B, PRE ? DEM_PRE_MIN_WAVES : DEM_FORCE_MIN_WAVES) v
     force_tile<REF, PP, PW, KICK, ACCUM, DIAG, BOND, ENS, PRE, B>(a, xcd_block(v, a.ntiles), s_cj);
     if (DIAG) __syncthreads(); // the reduction's LDS is reused by the next tile
   }
+}
+
+// The split force pass without sampling: k_defer (PASS 1) over all tiles,
+// then k_force_list (PASS 2) over the particles that have contacts, on a
+// persistent grid that reads the list's length on the device.
+template <bool REF, int KICK, bool ENS, int B>
+__global__ __launch_bounds__(B, DEM_DEFER_MIN_WAVES) void k_defer(StepArgs a) {
+  if ((int)gridDim.x >= a.ntiles) {
+    force_tile<REF, true, true, KICK, false, false, false, ENS, false, B, 1>(
+        a, xcd_block(blockIdx.x, a.ntiles), nullptr);
+    return;
+  }
+  for (int v = blockIdx.x; v < a.ntiles; v += gridDim.x)
+    force_tile<REF, true, true, KICK, false, false, false, ENS, false, B, 1>(a, xcd_block(v, a.ntiles), nullptr);
+}
+
+template <bool REF, int KICK, bool ENS, int B>
+__global__ __launch_bounds__(B, DEM_PRE_MIN_WAVES) void k_force_list(StepArgs a) {
+  const int cnt = *a.work_count;
+  for (int t = blockIdx.x; t * B < cnt; t += gridDim.x)
+    force_tile<REF, true, true, KICK, false, false, false, ENS, true, B, 2>(a, t, nullptr);
 }
 
 // Phase 1 of the force pass on its own (the split force pass): for each
@@ -852,6 +943,33 @@ hipError_t launch_split(ForceKind kind, const StepArgs& a, bool sample, const di
   return hipGetLastError();
 }
 
+// The split force pass without sampling: zero the list's length (a kernel,
+// so that captured graphs hold no memset), k_defer, k_force_list.
+template <bool REF, bool ENS, int B>
+hipError_t launch_deferred(ForceKind kind, const StepArgs& a, const dim3& g, hipStream_t s) {
+  hipError_t e = launch_fill_int(a.work_count, 1, 0, s);
+  if (e != hipSuccess) return e;
+  const int nl = a.ntiles < a.list_grid ? a.ntiles : a.list_grid;
+  const dim3 gl(nl > 0 ? nl : 1), b(B);
+  switch (kind) {
+    case FK_PP_PW_FORCES:
+      hipLaunchKernelGGL((k_defer<REF, 0, ENS, B>), g, b, 0, s, a);
+      hipLaunchKernelGGL((k_force_list<REF, 0, ENS, B>), gl, b, 0, s, a);
+      break;
+    case FK_STEP_FUSED:
+      hipLaunchKernelGGL((k_defer<REF, 2, ENS, B>), g, b, 0, s, a);
+      hipLaunchKernelGGL((k_force_list<REF, 2, ENS, B>), gl, b, 0, s, a);
+      break;
+    case FK_STEP_LAST:
+      hipLaunchKernelGGL((k_defer<REF, 1, ENS, B>), g, b, 0, s, a);
+      hipLaunchKernelGGL((k_force_list<REF, 1, ENS, B>), gl, b, 0, s, a);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 // B: the workgroup size and tile of the launch (kBlock for bonded systems).
 template <bool REF, bool BOND, bool ENS, int B>
 hipError_t launch_force_mode(ForceKind kind, const StepArgs& a, bool sample, int grid_blocks,
@@ -861,8 +979,10 @@ hipError_t launch_force_mode(ForceKind kind, const StepArgs& a, bool sample, int
   const int nb = (grid_blocks > 0 && grid_blocks < a.ntiles) ? grid_blocks : a.ntiles;
   const dim3 g(nb), b(B);
   if constexpr (!BOND) {
-    if (a.split && (kind == FK_PP_PW_FORCES || kind == FK_STEP_FUSED || kind == FK_STEP_LAST))
-      return launch_split<REF, ENS, B>(kind, a, sample, g, s);
+    if (a.split && (kind == FK_PP_PW_FORCES || kind == FK_STEP_FUSED || kind == FK_STEP_LAST)) {
+      if (sample) return launch_split<REF, ENS, B>(kind, a, sample, g, s);
+      return launch_deferred<REF, ENS, B>(kind, a, g, s);
+    }
   }
   switch (kind) {
     case FK_PP_PW_FORCES:
