@@ -326,6 +326,11 @@ def run_surrogate(args):
     rng = np.random.default_rng(5)
     eval_frames = [(0, int(t)) for t in rng.integers(S.HIST - 1, frames - 1, size=16)]
     before, baseline = held.evaluate(eval_frames)
+    # free roll-outs against the held-out DEM trajectory (no judge, no
+    # fallback): the error accumulates over the horizon
+    horizons = [1, 5, 10, 25, 50, 100]
+    hz_starts = [s_ for s_ in (10, 60, 120, 180) if s_ + max(horizons) < frames]
+    hz_untrained = S.horizon_errors(net, trajs[-1], sc, hz_starts, horizons) if hz_starts else {}
     t0 = time.perf_counter()
     losses = []
     for it in range(args.surrogate_steps):
@@ -338,6 +343,12 @@ def run_surrogate(args):
     after, _ = held.evaluate(eval_frames)
     log(f"[surrogate] held-out one-step MSE (units of a_scale^2): untrained {before:.4f}, trained "
         f"{after:.4f}, no network {baseline:.4f}; training {t_train:.1f} s")
+    hz_trained = S.horizon_errors(net, trajs[-1], sc, hz_starts, horizons) if hz_starts else {}
+    hz_none = (S.horizon_errors(net, trajs[-1], sc, hz_starts, horizons, no_network=True)
+               if hz_starts else {})
+    for h in horizons if hz_starts else []:
+        log(f"[surrogate] horizon {h} macro-steps: mean |dx|/r trained {hz_trained[h]:.3e}, "
+            f"untrained {hz_untrained[h]:.3e}, no network {hz_none[h]:.3e}")
 
     # forward-pass MFMA rate on the 64k column's graph
     wl = W.column(dims=(40, 40, 40), seed=7)
@@ -429,6 +440,18 @@ def run_surrogate(args):
             "mean_kinetic_energy_error_rel", "max_kinetic_energy_error_rel",
             "mean_mechanical_energy_error_rel")},
         "one_step_mse_held_out": {"untrained": before, "trained": after, "no_network": baseline},
+        "horizon_held_out": {
+            "what": "free roll-out (no judge, no fallback) from recorded frames of the held-out "
+                    "DEM trajectory; mean |x - x_DEM| / r_mean over all spheres at each horizon "
+                    "(macro-steps of 100 DEM steps), averaged over the start frames",
+            "starts": hz_starts,
+            "trained": {str(h): v for h, v in hz_trained.items()},
+            "untrained": {str(h): v for h, v in hz_untrained.items()},
+            "no_network": {str(h): v for h, v in hz_none.items()},
+            "tolerance_over_r": 0.02,
+            "trained_within_tolerance_at_check_interval": (hz_trained.get(10, float("inf")) <= 0.02
+                                                           if hz_starts else None),
+        },
         "mfma": {"forward_ms": fwd_ms, "nodes": g.N, "edges": g.E, "flops": flops,
                  "flops_minimal": forward_flops(10, g.N, g.E, False),
                  "achieved_tflops": mfma_tf, "peak_tflops": 2500.0, "frac": mfma_tf / 2500.0},

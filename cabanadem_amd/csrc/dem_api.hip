@@ -96,8 +96,8 @@ struct dem_ctx {
   bool walls_move = false;
 
   // automatic rebuild cadence (rebuild_every = DEM_REBUILD_AUTO)
-  int* d_work = nullptr;       // [n] particles with contacts (split force passes)
-  int* d_work_count = nullptr; // [1]
+  int* d_work = nullptr;       // [tiles][block] particles with contacts (split force passes)
+  int* d_work_tile = nullptr;  // [tiles] their number per tile
   int cus = 256;               // compute units of the device
   unsigned long long* d_disp = nullptr; // [2] max |x - xb|^2, max |u|^2 (bits)
   unsigned long long* h_disp = nullptr; // pinned host copy of d_disp (a pageable
@@ -188,7 +188,7 @@ struct dem_ctx {
     hipFree(nn); hipFree(nbr); hipFree(xb); hipFree(key); hipFree(rank); hipFree(key_sorted);
     hipFree(cell_count); hipFree(cell_start); hipFree(perm); hipFree(inv); hipFree(scan_tmp);
     hipFree(flags); hipFree(diag_part); hipFree(diag_rows); hipFree(d_disp);
-    hipFree(d_work); hipFree(d_work_count);
+    hipFree(d_work); hipFree(d_work_tile);
     if (h_disp) hipHostFree(h_disp);
     for (int b = 0; b < 3; ++b) { hipFree(d_tiles_b[b]); hipFree(d_tile0_b[b]); }
     hipFree(d_sys); hipFree(d_sysid); hipFree(d_grids);
@@ -248,9 +248,7 @@ struct dem_ctx {
     a.fext = fext[cS];
     a.split = (force_pass == DEM_FORCE_PASS_SPLIT && !bonds_on) ? 1 : 0;
     a.work = d_work;
-    a.work_count = d_work_count;
-    a.sysid = nsys > 1 ? d_sysid : nullptr;
-    a.list_grid = 8 * cus;
+    a.work_tile = d_work_tile;
     a.wave = (force_pass != DEM_FORCE_PASS_FUSED && !bonds_on) ? 1 : 0;
     a.tw = (double)(step + 1 - wall_step0) * cfg.dt;
     return a;
@@ -822,8 +820,13 @@ int create_ctx(const dem_config* common, int nsys, const dem_system_desc* sd, de
   c->scan_tmp = dalloc<int>((size_t)(c->ncell / 4096 + 2), &e);
   c->flags = dalloc<DevFlags>(1, &e);
   c->d_disp = dalloc<unsigned long long>(2, &e);
-  c->d_work = dalloc<int>(n, &e);
-  c->d_work_count = dalloc<int>(1, &e);
+  {
+    // work lists: every tile's slots at every tile size
+    size_t slots = 0;
+    for (int b = 0; b < 3; ++b) slots = std::max(slots, (size_t)c->ntiles_b[b] * kForceBlocks[b]);
+    c->d_work = dalloc<int>(slots, &e);
+    c->d_work_tile = dalloc<int>((size_t)c->ntiles_b[0], &e); // the most tiles (64 slots)
+  }
   c->diag_part = dalloc<double>((size_t)c->ntiles_b[0] * 8, &e); // the most tiles (64 slots)
   c->d_sys = dalloc<SysParams>((size_t)nsys, &e);
   for (int b = 0; b < 3; ++b) c->d_tile0_b[b] = dalloc<int>((size_t)nsys + 1, &e);

@@ -508,11 +508,12 @@ __device__ __forceinline__ void wave_contacts(const StepArgs& a, const SysParams
 //   0  everything in one pass;
 //   1  deferral: phase 1 only; a particle without particle or wall contacts
 //      finishes here (its force is the start value), the others write their
-//      contact ids to cj_out and their count to cmeta_out and join the work
-//      list a.work (one atomic per wave); the FP64 contact arithmetic is not
-//      compiled in, so this pass runs at the occupancy of a gather kernel;
-//   2  the work list: tile t is entries [t B, (t+1) B) of a.work, run as
-//      with PRE (ids in cj_out).
+//      contact ids to cj_out and their count to cmeta_out and join their
+//      tile's work list (slots [t B, t B + work_tile[t]) of a.work, in lane
+//      order: no atomics); the FP64 contact arithmetic is not compiled in, so
+//      this pass runs at the occupancy of a gather kernel;
+//   2  the work lists: tile t runs the particles of its list, as with PRE
+//      (ids in cj_out); a tile with an empty list returns at once.
 // Every particle's arithmetic is that of PASS 0, so the results are bitwise
 // the same.
 template <bool REF, bool PP, bool PW, int KICK, bool ACCUM, bool DIAG, bool BOND, bool ENS, bool PRE, int B,
@@ -527,10 +528,11 @@ __device__ __forceinline__ void force_tile(const StepArgs& a, const int tile, in
   int i;
   bool act;
   if (LIST) {
-    const int k = tile * B + tid;
-    act = k < *a.work_count;
-    i = act ? a.work[k] : 0;
-    if (ENS && act) sys = a.sysid[i];
+    const int cnt = a.work_tile[tile];
+    if (cnt == 0) return; // uniform over the workgroup
+    act = tid < cnt;
+    i = act ? a.work[(long)tile * B + tid] : 0;
+    if (ENS) sys = a.tiles[tile].x;
   } else {
     if (ENS) {
       const int4 t = a.tiles[tile];
@@ -712,15 +714,20 @@ __device__ __forceinline__ void force_tile(const StepArgs& a, const int tile, in
       }
       if (defer) a.cmeta_out[i] = make_meta(nc, 0);
     }
-    // the wave's deferred particles into the work list, in lane order
+    // the tile's deferred particles into its work list, in lane order
+    __shared__ int s_wc[B / 64];
     const unsigned long long mask = __ballot(defer);
-    if (mask) {
-      const int lane = tid & 63, lead = __builtin_ffsll((long long)mask) - 1;
-      int base = 0;
-      if (lane == lead) base = atomicAdd(a.work_count, __popcll(mask));
-      base = __shfl(base, lead);
-      if (defer) a.work[base + __popcll(mask & ((1ull << lane) - 1ull))] = i;
+    const int lane = tid & 63, wv = tid >> 6;
+    if (lane == 0) s_wc[wv] = __popcll(mask);
+    __syncthreads();
+    int base = 0, total = 0;
+#pragma unroll
+    for (int k = 0; k < B / 64; ++k) {
+      base += k < wv ? s_wc[k] : 0;
+      total += s_wc[k];
     }
+    if (defer) a.work[(long)tile * B + base + __popcll(mask & ((1ull << lane) - 1ull))] = i;
+    if (tid == 0) a.work_tile[tile] = total;
   }
   if (DEFER && act && !defer) {
     load6(a.uw_in + 6l * i, ui);
@@ -833,8 +840,8 @@ __global__ __launch_bounds__(B, PRE ? DEM_PRE_MIN_WAVES : DEM_FORCE_MIN_WAVES) v
 }
 
 // The split force pass without sampling: k_defer (PASS 1) over all tiles,
-// then k_force_list (PASS 2) over the particles that have contacts, on a
-// persistent grid that reads the list's length on the device.
+// then k_force_list (PASS 2) over the same tiles, each running only its
+// particles with contacts (on the same launch grid).
 template <bool REF, int KICK, bool ENS, int B>
 __global__ __launch_bounds__(B, DEM_DEFER_MIN_WAVES) void k_defer(StepArgs a) {
   if ((int)gridDim.x >= a.ntiles) {
@@ -842,15 +849,21 @@ __global__ __launch_bounds__(B, DEM_DEFER_MIN_WAVES) void k_defer(StepArgs a) {
         a, xcd_block(blockIdx.x, a.ntiles), nullptr);
     return;
   }
-  for (int v = blockIdx.x; v < a.ntiles; v += gridDim.x)
+  for (int v = blockIdx.x; v < a.ntiles; v += gridDim.x) {
     force_tile<REF, true, true, KICK, false, false, false, ENS, false, B, 1>(a, xcd_block(v, a.ntiles), nullptr);
+    __syncthreads(); // the list compaction's LDS is reused by the next tile
+  }
 }
 
 template <bool REF, int KICK, bool ENS, int B>
 __global__ __launch_bounds__(B, DEM_PRE_MIN_WAVES) void k_force_list(StepArgs a) {
-  const int cnt = *a.work_count;
-  for (int t = blockIdx.x; t * B < cnt; t += gridDim.x)
-    force_tile<REF, true, true, KICK, false, false, false, ENS, true, B, 2>(a, t, nullptr);
+  if ((int)gridDim.x >= a.ntiles) {
+    force_tile<REF, true, true, KICK, false, false, false, ENS, true, B, 2>(
+        a, xcd_block(blockIdx.x, a.ntiles), nullptr);
+    return;
+  }
+  for (int v = blockIdx.x; v < a.ntiles; v += gridDim.x)
+    force_tile<REF, true, true, KICK, false, false, false, ENS, true, B, 2>(a, xcd_block(v, a.ntiles), nullptr);
 }
 
 // Phase 1 of the force pass on its own (the split force pass): for each
@@ -943,14 +956,10 @@ hipError_t launch_split(ForceKind kind, const StepArgs& a, bool sample, const di
   return hipGetLastError();
 }
 
-// The split force pass without sampling: zero the list's length (a kernel,
-// so that captured graphs hold no memset), k_defer, k_force_list.
+// The split force pass without sampling: k_defer, then k_force_list.
 template <bool REF, bool ENS, int B>
 hipError_t launch_deferred(ForceKind kind, const StepArgs& a, const dim3& g, hipStream_t s) {
-  hipError_t e = launch_fill_int(a.work_count, 1, 0, s);
-  if (e != hipSuccess) return e;
-  const int nl = a.ntiles < a.list_grid ? a.ntiles : a.list_grid;
-  const dim3 gl(nl > 0 ? nl : 1), b(B);
+  const dim3 gl = g, b(B);
   switch (kind) {
     case FK_PP_PW_FORCES:
       hipLaunchKernelGGL((k_defer<REF, 0, ENS, B>), g, b, 0, s, a);

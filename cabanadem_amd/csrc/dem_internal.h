@@ -173,14 +173,12 @@ struct StepArgs {
   double vgv[DEM_MAX_GROUPS][3];
   const double* fext;    // [n][3] constant external force, or null
   // 1: whole force passes run split (systems without bonds): k_defer
-  // finishes the particles without contacts and lists the others in work,
-  // then k_force_list runs those (sampling passes: k_detect, then the force
-  // kernel over its contact lists); 0: one fused kernel
+  // finishes the particles without contacts and lists the others per tile
+  // in work, then k_force_list runs those (sampling passes: k_detect, then
+  // the force kernel over its contact lists); 0: one fused kernel
   int split;
-  int* work;             // [n] particles with contacts (split passes)
-  int* work_count;       // [1] their number
-  const int* sysid;      // system of each slot (ensembles), else null
-  int list_grid;         // workgroups of the work-list kernel
+  int* work;             // [ntiles][block] each tile's particles with contacts (split passes)
+  int* work_tile;        // [ntiles] their number per tile
   // 1: the contact arithmetic runs as a wave pass over the contacts of all
   // lanes of a wave (systems without bonds); 0: each lane walks its own list
   int wave;

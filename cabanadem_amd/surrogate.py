@@ -681,6 +681,35 @@ class Trainer:
         return ln / len(frames), lb / len(frames)
 
 
+def horizon_errors(net, traj, sc: Scales, starts, horizons, no_network=False):
+    """Free roll-out against the DEM's own trajectory: from each recorded
+    frame f in `starts` (its positions and the HIST velocities up to it),
+    the surrogate alone -- no judge, no fallback -- advances max(horizons)
+    macro-steps, and at each horizon h the mean |x_surrogate - x_DEM| / r_mean
+    over all spheres is taken against the DEM's frame f + h.  no_network:
+    the method without the network (a = g over every macro-step).  Returns
+    {h: mean over the starts}.  Unlike the judge's one-macro-step check this
+    error accumulates, so a tolerance on it can be missed."""
+    torch = _torch()
+    H = max(horizons)
+    T = traj.x.shape[0]
+    for f in starts:
+        if f - HIST + 1 < 0 or f + H >= T:
+            raise ValueError(f"start frame {f} with horizon {H} lies outside the {T}-frame trajectory")
+    ro = Rollout(net, traj.wl, sc, no_network=no_network)
+    ro.dem.close()
+    out = {h: [] for h in horizons}
+    for f in starts:
+        ro.x = traj.x_dev[f].clone()
+        ro.vhist = traj.v_dev[f - HIST + 1:f + 1].contiguous().clone()
+        for h in range(1, H + 1):
+            ro.surrogate_step()
+            if h in out:
+                err = torch.linalg.vector_norm(ro.x - traj.x_dev[f + h], dim=1).mean()
+                out[h].append(float(err) / ro.r_mean)
+    return {h: float(np.mean(v)) for h, v in out.items()}
+
+
 class Rollout:
     """The surrogate advancing a DEM workload by macro-steps, with the FP64
     DEM as judge and fallback: every `check_every` macro-steps the DEM

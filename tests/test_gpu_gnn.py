@@ -462,6 +462,26 @@ def test_training_reduces_loss_and_beats_no_network_baseline(gpu_lib):
     assert after < baseline, (after, baseline)
 
 
+def test_horizon_errors_accumulate(gpu_lib):
+    """Free roll-outs from recorded frames against the DEM's own later
+    frames: the a = g error grows with the horizon (the column's supported
+    spheres fall through their supports), and the network's roll-out gives
+    finite errors of its own."""
+    trajs = small_column_data(n_traj=1, frames=40)
+    sc = S.fit_scales(trajs, 2.5e-3)
+    net = S.Network(n_layers=2, seed=3)
+    hz = [1, 5, 20]
+    none = S.horizon_errors(net, trajs[0], sc, [5, 12], hz, no_network=True)
+    withnet = S.horizon_errors(net, trajs[0], sc, [5, 12], hz)
+    assert set(none) == set(withnet) == set(hz)
+    assert all(np.isfinite(v) and v > 0. for v in list(none.values()) + list(withnet.values()))
+    assert none[1] < none[5] < none[20]
+    # after one macro-step a = g is off by g Dt^2 / 2 at most for a resting sphere
+    assert none[1] < 1e-3
+    with pytest.raises(ValueError):
+        S.horizon_errors(net, trajs[0], sc, [30], [20])
+
+
 def test_judged_rollout_records_error_and_falls_back(gpu_lib):
     trajs = small_column_data(n_traj=1, frames=20)
     sc = S.fit_scales(trajs, 2.5e-3)
