@@ -591,31 +591,41 @@ def features(g: Graph, x_dev, vhist_dev, rad_dev, planes_dev, n_planes, sc: Scal
 
 @dataclass
 class Trajectory:
+    """Frames every MACRO DEM steps.  Velocities are the macro-step mean
+    velocities (x_t - x_{t-1}) / Dt, not the DEM's instantaneous ones: the
+    spheres of a packing rattle many times within a macro-step, and the mean
+    velocity is what carries a position from frame to frame.  With them the
+    surrogate's update v_{t+1} = v_t + (a + g) Dt, x_{t+1} = x_t + v_{t+1} Dt
+    reproduces the DEM's positions exactly when a is the target
+    (x_{t+1} - 2 x_t + x_{t-1}) / Dt^2 - g."""
     x: np.ndarray        # [T][N][3] float64 (host, for the graph build)
     x_dev: object        # [T][N][3] float64 (device)
-    v_dev: object        # [T][N][3] float32 (device)
+    v_dev: object        # [T][N][3] float32 (device): (x_t - x_{t-1}) / Dt
     a_host: np.ndarray   # [T-1][N][3] float32: (v_{t+1} - v_t)/Dt - g
     rad: np.ndarray
     wl: object
 
 
 def generate(wl, n_frames, log=None):
-    """Runs the DEM from the workload's state and keeps every MACRO-th step."""
+    """Runs the DEM from the workload's state and keeps every MACRO-th step:
+    positions, and mean velocities over each macro-step."""
     torch = _torch()
     sysm = wl.system()
     st = abi.ParticleState(wl.state.n, max_cnts=wl.max_contacts, max_walls=6)
     xs = np.empty((n_frames, wl.state.n, 3))
     vs = np.empty((n_frames, wl.state.n, 3), dtype=np.float32)
+    dt = wl.dt * MACRO
+    x_prev = wl.state.x.copy()
     t0 = time.perf_counter()
     for f in range(n_frames):
         sysm.step(MACRO)
-        sysm.download(st, fields=("x", "u"))
+        sysm.download(st, fields=("x",))
         xs[f] = st.x
-        vs[f] = st.u
+        vs[f] = (st.x - x_prev) / dt
+        x_prev = xs[f]
     sysm.close()
     if log:
         log(f"[surrogate] {n_frames} frames of {wl.state.n} spheres in {time.perf_counter() - t0:.1f} s")
-    dt = wl.dt * MACRO
     g = np.asarray(wl.gravity, dtype=np.float32)
     a = (vs[1:] - vs[:-1]) / np.float32(dt) - g
     return Trajectory(xs, torch.from_numpy(xs).cuda(), torch.from_numpy(vs).cuda(), a.astype(np.float32),
@@ -715,7 +725,8 @@ class Rollout:
     DEM as judge and fallback: every `check_every` macro-steps the DEM
     re-simulates the same macro-step from the same state; the mean position
     error |x_surrogate - x_DEM| / r_mean and the kinetic and mechanical
-    (kinetic + gravitational) energy errors are recorded and, above `tol`
+    (kinetic + gravitational) energy errors are recorded (kinetic energy of
+    the macro-step mean velocities on both sides) and, above `tol`
     in position, the run continues from the DEM state.  tune_dem: the judge's
     DEM context runs the autotuner's configuration (rebuild cadence, launch
     shape), picked from the warm-up state.  no_network: the method without
@@ -758,16 +769,24 @@ class Rollout:
         self.dem.get_kinematics_device(self.xd.data_ptr(), self.ud.data_ptr())
         self.dem.sync()
 
+    def _dem_macro(self, x_before):
+        """One macro-step of the DEM context; its positions in self.xd and its
+        mean velocity over the macro-step, (x - x_before) / Dt (device)."""
+        self.dem.step(MACRO)
+        self._dem_pull()
+        return (self.xd - x_before) / self.Dt
+
     def warm_up(self):
         """HIST macro-steps of DEM to fill the velocity history."""
         torch = _torch()
         if self.tune_dem and self.dem_tune is None:
             self.dem_tune = self.dem.autotune(trial_steps=80)  # restores the state
         vh = []
+        self._dem_pull()
+        x_prev = self.xd.clone()
         for _ in range(HIST):
-            self.dem.step(MACRO)
-            self._dem_pull()
-            vh.append(self.ud.float())
+            vh.append(self._dem_macro(x_prev).float())
+            x_prev = self.xd.clone()
         self.x = self.xd.clone()
         self.vhist = torch.stack(vh).contiguous()
         # one untimed check, so that first-call costs (kernel loads) stay out
@@ -803,13 +822,12 @@ class Rollout:
     def dem_step_from(self, x, v):
         """One macro-step of DEM from positions x and velocities v (device,
         double [n][3]) with spins, forces and contact histories cleared;
-        returns the DEM's positions and velocities (device, valid until the
-        next DEM call)."""
+        returns the DEM's positions (device, valid until the next DEM call)
+        and its mean velocity over the macro-step."""
         _torch().cuda.current_stream().synchronize()
         self.dem.set_kinematics_device(x.data_ptr(), v.data_ptr(), True)
-        self.dem.step(MACRO)
-        self._dem_pull()
-        return self.xd, self.ud
+        v_mean = self._dem_macro(x)
+        return self.xd, v_mean
 
     def dem_advance(self, x, vhist, n_macro):
         """n_macro macro-steps of DEM from positions x and the last velocity
@@ -821,10 +839,11 @@ class Rollout:
         torch.cuda.current_stream().synchronize()
         self.dem.set_kinematics_device(x.data_ptr(), v0.data_ptr(), True)
         vh = list(vhist)
+        x_prev = x.clone()
         for _ in range(n_macro):
-            self.dem.step(MACRO)
-            self._dem_pull()
-            vh = vh[1:] + [self.ud.float()]
+            v_mean = self._dem_macro(x_prev)
+            vh = vh[1:] + [v_mean.float()]
+            x_prev = self.xd.clone()
         return self.xd.clone(), torch.stack(vh).contiguous()
 
     def run(self, n_macro):
