@@ -59,15 +59,25 @@ def diag_interval(steps):
     return max(1, steps // 10)
 
 
-def measured_traffic(n):
-    """HBM bytes per launch of the fused kernel at this size, from the
-    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the
-    copy-kernel calibration recorded with it), or None."""
-    p = ROOT / "profiles" / "hbm_traffic_fused_step_4M.json"
+TRAFFIC_RECORDS = {  # (workload, physics mode) -> committed rocprofv3 counter summary
+    ("plain_large", 0): "hbm_traffic_fused_step_4M.json",
+    ("plain_packed", 1): "hbm_traffic_fused_step_packed_4M.json",
+    ("bonded_block", 1): "hbm_traffic_bonded_step_1M.json",
+}
+
+
+def measured_traffic(workload, mode, n):
+    """HBM bytes per launch of the fused kernel for this workload, mode and
+    size, from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled
+    by the copy-kernel calibration recorded with it), or None."""
+    name = TRAFFIC_RECORDS.get((workload, mode))
+    if name is None:
+        return None
+    p = ROOT / "profiles" / name
     if not p.exists():
         return None
     d = json.loads(p.read_text())
-    if d.get("spheres") != n:
+    if d.get("spheres") != n or d.get("workload") != workload or d.get("physics_mode") != mode:
         return None
     return {"bytes_per_launch": d["bytes_per_launch"], "source": str(p.relative_to(ROOT))}
 
@@ -163,6 +173,7 @@ def run_bonded(args):
     k_ms = ms[0] / max(launches[0], 1)
     b_alg = W.bytes_bonded_passes(bbar, nbar, cbar)
     achieved = b_alg * n / (k_ms * 1e-3) / 1e9
+    traffic = measured_traffic("bonded_block", args.mode, n)
     out = {
         "metric": "FP64 bonded-DEM particle-steps/s (1M-sphere tensile block)",
         "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
@@ -175,7 +186,11 @@ def run_bonded(args):
                    "physics_mode": "reference" if args.mode == 0 else "corrected",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic["bytes_per_launch"] if traffic else None,
+                     "traffic_source": traffic["source"] if traffic else None,
+                     "frac_measured_traffic": (traffic["bytes_per_launch"] / (k_ms * 1e-3) / 1e9
+                                               / HBM_PEAK_GBS if traffic else None),
                      "kernel": "fused bonded step kernel (bonds + contacts + stage3 + next stage1/2)",
                      "algorithmic_bytes_per_particle": b_alg, "kernel_ms": k_ms,
                      "launches": launches[0]},
@@ -232,18 +247,26 @@ def run_ensemble(args):
         f"{tune.use_graph}, force pass {tune.force_pass}, tiles {tune.block} slots / "
         f"{tune.tile_rows} cell rows, {tune.ms_per_step:.4f} ms/step ({tune.n_trials} trials, "
         f"{time.perf_counter() - t0:.1f} s)")
-    # kernel duration and contact statistics: a short timed window, eager
+    # kernel duration and contact statistics: short eager windows right before
+    # and right after the timed run (the systems' spheres start moving apart
+    # and colliding, so the kernel slows over the run; the two windows bracket
+    # it)
     use_graph = bool(tune.use_graph)
-    sysm.set_graph(False)
-    sysm.set_diag_every(50)
-    sysm.set_timing(True)
-    sysm.step(200)
-    ms, launches = sysm.timing()
-    sysm.set_timing(False)
-    diag = sysm.diag_history()
-    nbar, cbar, wbar = (float(diag[:, c].mean()) for c in (6, 7, 8))
-    sysm.set_diag_every(0)
-    sysm.set_graph(use_graph)
+
+    def window():
+        sysm.set_graph(False)
+        sysm.set_diag_every(50)
+        sysm.set_timing(True)
+        sysm.step(200)
+        ms, launches = sysm.timing()
+        sysm.set_timing(False)
+        diag = sysm.diag_history()[-4:]
+        sysm.set_diag_every(0)
+        sysm.set_graph(use_graph)
+        return (ms[0] / max(launches[0], 1), launches[0],
+                tuple(float(diag[:, c].mean()) for c in (6, 7, 8)))
+
+    k_ms0, launches0, stats0 = window()
     sysm.sync()
     c0 = sysm.counters()
     t0 = time.perf_counter()
@@ -252,7 +275,10 @@ def run_ensemble(args):
     elapsed = time.perf_counter() - t0
     c1 = sysm.counters()
     value = n * args.steps / elapsed
-    k_ms = ms[0] / max(launches[0], 1)
+    k_ms1, launches1, stats1 = window()
+    k_ms = 0.5 * (k_ms0 + k_ms1)
+    launches = (launches0 + launches1,)
+    nbar, cbar, wbar = (0.5 * (a + b) for a, b in zip(stats0, stats1))
     b_alg = W.bytes_reference_passes(nbar, cbar, wbar)
     achieved = b_alg * n / (k_ms * 1e-3) / 1e9
     out = {
@@ -275,7 +301,9 @@ def run_ensemble(args):
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "fused step kernel over all systems",
                      "algorithmic_bytes_per_particle": b_alg, "kernel_ms": k_ms,
-                     "launches": launches[0]},
+                     "launches": launches[0],
+                     "windows": {"before": {"kernel_ms": k_ms0, "n_c_w": stats0},
+                                 "after": {"kernel_ms": k_ms1, "n_c_w": stats1}}},
         "contacts": {"mean_neighbors": nbar, "mean_contacts": cbar, "mean_wall_contacts": wbar,
                      "contact_pairs_per_s": cbar * value},
         "step_model": {"rebuilds": int(c1.rebuilds - c0.rebuilds),
@@ -321,7 +349,7 @@ def run_surrogate(args):
     radius = 2.5e-3
     sc = S.fit_scales(trajs[:-1], radius)
     net = S.Network(n_layers=10, seed=0)
-    tr = S.Trainer(net, trajs[:-1], sc, seed=0)
+    tr = S.Trainer(net, trajs[:-1], sc, seed=0, noise=args.surrogate_noise)
     held = S.Trainer(net, trajs[-1:], sc, seed=1)
     rng = np.random.default_rng(5)
     eval_frames = [(0, int(t)) for t in rng.integers(S.HIST - 1, frames - 1, size=16)]
@@ -422,7 +450,8 @@ def run_surrogate(args):
                                "DEM check every 10 macro-steps",
                    "network": "10 message-passing layers, latent 128, radius 2.5 mm graph",
                    "training": f"{n_traj - 1} trajectories x {frames} frames (8-10k spheres), "
-                               f"{args.surrogate_steps} Adam steps; 1 held-out trajectory",
+                               f"{args.surrogate_steps} Adam steps, input noise {args.surrogate_noise} "
+                               f"a_scale Dt; 1 held-out trajectory",
                    "parallelism": "single GPU"},
         "speedup_without_checks": speed_no_checks,
         "rollout": dict(res, horizon_mean_position_error_over_r=horizon_err,
@@ -485,6 +514,8 @@ def main():
                          "dem_set_force_pass")
     ap.add_argument("--surrogate-traj", type=int, default=5)
     ap.add_argument("--surrogate-frames", type=int, default=300)
+    ap.add_argument("--surrogate-noise", type=float, default=0.0,
+                    help="training-input velocity noise, in units of a_scale * Dt (Trainer)")
     ap.add_argument("--surrogate-steps", type=int, default=3000)
     ap.add_argument("--surrogate-macro", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -574,7 +605,7 @@ def main():
     # the rebuild term at the cadence the timed window actually ran
     k_eff = args.steps / max(launches[1], 1)
     b_ref = W.bytes_reference_model(nbar, cbar, wbar, k_eff)
-    traffic = measured_traffic(n)
+    traffic = measured_traffic(args.workload, args.mode, n)
 
     out = {
         "metric": "FP64 DEM particle-steps/s at 4M spheres",

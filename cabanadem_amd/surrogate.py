@@ -641,9 +641,20 @@ def fit_scales(trajs, radius):
 
 
 class Trainer:
-    def __init__(self, net: Network, trajs, scales: Scales, seed=0):
+    """noise: training inputs see a velocity history perturbed by a random walk
+    whose last step has standard deviation noise * a_scale * Dt (in velocity),
+    and the target is corrected so that the predicted step still lands on the
+    DEM's next velocity: a' = a + (v_t - v'_t) / Dt.  A roll-out feeds the
+    network its own, slightly wrong, velocities; training on such inputs
+    teaches it to steer back instead of drifting (noise injection, as in
+    learned particle simulators).  Evaluation never adds noise."""
+
+    def __init__(self, net: Network, trajs, scales: Scales, seed=0, noise=0.):
         torch = _torch()
         self.net, self.trajs, self.sc = net, trajs, scales
+        self.noise = float(noise)
+        self.gen = torch.Generator(device="cuda")
+        self.gen.manual_seed(int(seed) + 12345)
         self.rng = np.random.default_rng(seed)
         self.builders = []
         self.planes = []
@@ -655,15 +666,23 @@ class Trainer:
         self.rad = [torch.from_numpy(t.rad).cuda() for t in trajs]
         self.targets = [torch.from_numpy(t.a_host / np.float32(scales.a)).cuda() for t in trajs]
 
-    def sample(self, k, t):
+    def sample(self, k, t, noise=0.):
         torch = _torch()
         tr = self.trajs[k]
         g = self.builders[k].build(tr.x_dev[t])
         vh = tr.v_dev[t - HIST + 1:t + 1].contiguous()
+        target = self.targets[k][t]
+        if noise > 0.:
+            # random walk over the history, newest step last
+            Dt = tr.wl.dt * MACRO
+            sd = noise * self.sc.a * Dt / math.sqrt(HIST)
+            walk = torch.cumsum(torch.randn(vh.shape, device="cuda", generator=self.gen) * sd, dim=0)
+            vh = (vh + walk).contiguous()
+            target = (target - walk[-1] / np.float32(Dt * self.sc.a)).contiguous()
         xn = torch.empty((g.N, FEAT), dtype=torch.bfloat16, device="cuda")
         xe = torch.empty((max(g.E, 1), FEAT), dtype=torch.bfloat16, device="cuda")
         features(g, tr.x_dev[t], vh, self.rad[k], self.planes[k], len(tr.wl.extra["planes"]), self.sc, xn, xe)
-        return g, xn, xe, self.targets[k][t]
+        return g, xn, xe, target
 
     def step(self, lr, k=None, t=None):
         if k is None:
@@ -671,7 +690,7 @@ class Trainer:
         T = self.trajs[k].x.shape[0]
         if t is None:
             t = int(self.rng.integers(HIST - 1, T - 1))
-        g, xn, xe, target = self.sample(k, t)
+        g, xn, xe, target = self.sample(k, t, self.noise)
         out = self.net.forward(g, xn, xe[:g.E], train=True)
         loss, d = mse(out, target)
         self.net.backward(g, xn, xe[:g.E], d)
