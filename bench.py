@@ -514,7 +514,7 @@ def main():
                          "dem_set_force_pass")
     ap.add_argument("--surrogate-traj", type=int, default=5)
     ap.add_argument("--surrogate-frames", type=int, default=300)
-    ap.add_argument("--surrogate-noise", type=float, default=0.0,
+    ap.add_argument("--surrogate-noise", type=float, default=0.5,
                     help="training-input velocity noise, in units of a_scale * Dt (Trainer)")
     ap.add_argument("--surrogate-steps", type=int, default=3000)
     ap.add_argument("--surrogate-macro", type=int, default=50)

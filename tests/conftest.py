@@ -12,6 +12,13 @@ for p in (ROOT, ROOT / "oracle", ROOT / "tests"):
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an AMD GPU (MI355X) and the built HIP library")
+    # One HIP runtime per process: PyTorch's must be loaded before
+    # libcabanadem_hip.so initialises the device, or torch sees no GPU later
+    # in the session (the surrogate tests move data through torch tensors).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
 
 
 @pytest.fixture(scope="session")
