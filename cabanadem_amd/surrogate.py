@@ -732,7 +732,20 @@ def horizon_errors(net, traj, sc: Scales, starts, horizons, no_network=False):
         ro.x = traj.x_dev[f].clone()
         ro.vhist = traj.v_dev[f - HIST + 1:f + 1].contiguous().clone()
         for h in range(1, H + 1):
-            ro.surrogate_step()
+            # a roll-out that has left every finite position (an untrained
+            # network can) is off by infinity from here on
+            ok = bool(torch.isfinite(ro.x).all())
+            if ok:
+                try:
+                    ro.surrogate_step()
+                    ok = bool(torch.isfinite(ro.x).all())
+                except RuntimeError:
+                    ok = False
+            if not ok:
+                for hh in horizons:
+                    if hh >= h:
+                        out[hh].append(float("inf"))
+                break
             if h in out:
                 err = torch.linalg.vector_norm(ro.x - traj.x_dev[f + h], dim=1).mean()
                 out[h].append(float(err) / ro.r_mean)
