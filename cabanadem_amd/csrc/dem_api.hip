@@ -1816,7 +1816,7 @@ int dem_enable_bonds(dem_ctx* c, const dem_bond_config* b) {
   c->max_b = b->max_bonds;
   c->bcfg = *b;
   c->bonds_on = true;
-  c->select_block(c->bidx); // bonded systems run tiles of kBlock
+  c->select_block(1); // bonded systems run tiles of kBlock
   ++c->gen;
   return c->upload_bEG(id);
 }

@@ -39,6 +39,7 @@ constexpr int kBlock = DEM_BLOCK;  // threads per block of the particle kernels
 // Tile (workgroup) sizes of the force kernels of systems without bonds,
 // chosen at run time (dem_set_shape); bonded systems run kBlock.
 constexpr int kForceBlocks[3] = {64, 128, 256};
+static_assert(kForceBlocks[1] == kBlock, "index 1 is the default tile size, the bonded one");
 constexpr int kMaxContacts = 32;  // upper bound of max_contacts
 constexpr int kMaxNeighbors = 255;
 constexpr int kXcds = 8;          // MI355X: 8 XCDs, workgroups dealt round-robin

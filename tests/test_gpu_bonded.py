@@ -207,4 +207,17 @@ def test_bond_api_errors(gpu_lib):
     with pytest.raises(abi.DemError) as e:
         sysm.set_groups(np.full(st.n, 3, dtype=np.int32), [0, 0], [(0, 0, 0), (0, 0, 0)])
     assert e.value.code == -1
+    # bonded systems run force tiles of 128; cell tiles may change
+    with pytest.raises(abi.DemError) as e:
+        sysm.set_shape(64, 0)
+    assert e.value.code == -1
+    sysm.set_shape(0, 16)
+    assert sysm.shape() == (128, 16)
+    sysm.close()
+    # a context with 64-slot tiles falls back to 128 once bonds are enabled
+    sysm = make_system(st, None, dt=1e-6, rc=1.2e-3, lo=lo, hi=hi)
+    sysm.set_shape(64, 0)
+    assert sysm.shape() == (64, 8)
+    sysm.enable_bonds(8)
+    assert sysm.shape()[0] == 128
     sysm.close()
