@@ -13,6 +13,9 @@ before the timed region.  One JSON line on stdout.
 Also reported:
   roofline     the fused step kernel's compulsory HBM bytes per launch over
                its mean duration (HIP events on the library's stream).
+  step_model   rebuilds and displacement checks in the timed window, and the
+               same number of steps right after it rebuilding every 20 steps
+               (fixed_cadence; --no-fixed-compare skips it).
   cpu_baseline the CPU oracle (OpenMP, this host's cores) on a bounded sample
                of the same workload.
 """
@@ -503,6 +506,9 @@ def main():
     ap.add_argument("--rebuild-every", type=int, default=-1,
                     help="plain workloads: neighbour rebuild cadence in steps, or -1 for the "
                          "automatic cadence (default)")
+    ap.add_argument("--no-fixed-compare", dest="fixed_compare", action="store_false",
+                    help="plain workloads on the automatic cadence: skip the second window "
+                         "rebuilding every 20 steps")
     ap.add_argument("--block", type=int, default=0, choices=(0, 64, 128, 256),
                     help="plain workloads: force-kernel tile size (dem_set_shape; 0: 128)")
     ap.add_argument("--tile-rows", type=int, default=0,
@@ -594,6 +600,21 @@ def main():
     wbar = float(diag[:, 8].mean()) if len(diag) else float("nan")
     cnt = sysm.counters()
 
+    fixed = None
+    if wl.rebuild_every == abi.REBUILD_AUTO and args.fixed_compare:
+        # the same number of steps right after, rebuilding every 20 steps
+        sysm.set_diag_every(0)
+        sysm.set_params(rebuild_every=20)
+        sysm.sync()
+        t0 = time.perf_counter()
+        sysm.step(args.steps)
+        sysm.sync()
+        el20 = time.perf_counter() - t0
+        fixed = {"rebuild_every": 20, "value": n * args.steps / el20,
+                 "ms_per_step": el20 / args.steps * 1e3,
+                 "window": f"the {args.steps} steps after the timed window"}
+        log(f"[bench] the next {args.steps} steps rebuilding every 20: {el20:.3f} s")
+
     value = n * args.steps / elapsed
     step_ms = elapsed / args.steps * 1e3
     k_ms = ms[0] / max(launches[0], 1)
@@ -642,7 +663,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic["bytes_per_launch"] if traffic else None,
-            "kernel": ("detection kernel + step kernel (forces + stage3 + next stage1/stage2)"
+            "kernel": ("gather pass + work-list step kernel (forces + stage3 + next stage1/stage2)"
                        if args.force_pass == "split" else
                        "fused step kernel (forces + stage3 + next stage1/stage2)"),
             "algorithmic_bytes_per_particle": b_alg,
@@ -662,6 +683,7 @@ def main():
             "rebuilds": launches[1],
             "steps_per_rebuild": k_eff,
             "displacement_checks": int(cnt.displacement_checks - checks0),
+            "fixed_cadence": fixed,
         },
         "contacts": {"mean_neighbors": nbar, "mean_contacts": cbar, "mean_wall_contacts": wbar,
                      "max_neighbors": cnt.max_neighbors_seen,
