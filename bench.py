@@ -66,6 +66,7 @@ TRAFFIC_RECORDS = {  # (workload, physics mode) -> committed rocprofv3 counter s
     ("plain_large", 0): "hbm_traffic_fused_step_4M.json",
     ("plain_packed", 1): "hbm_traffic_fused_step_packed_4M.json",
     ("bonded_block", 1): "hbm_traffic_bonded_step_1M.json",
+    ("ensemble", 0): "hbm_traffic_fused_step_ensemble_256.json",
 }
 
 
@@ -237,7 +238,11 @@ def run_ensemble(args):
     from cabanadem_amd import workloads as W
 
     t0 = time.perf_counter()
-    wl = W.ensemble()
+    # --ensemble-dense: lattice spacing 1.9 r_max (polydisperse neighbours
+    # overlap from the start) and u ~ N(0, 0.5 m/s), walls 0.95 r_max outside
+    # the lattice: contacts with spheres and walls from the first step
+    wl = (W.ensemble(spacing_r=1.9, vsigma=0.5, wall_offset=0.95 * 1.2e-3) if args.ensemble_dense
+          else W.ensemble())
     n = wl.n
     sysm = wl.system(mode=args.mode)
     log(f"[bench] ensemble of {len(wl.systems)} systems, {n} spheres, built in "
@@ -284,6 +289,7 @@ def run_ensemble(args):
     nbar, cbar, wbar = (0.5 * (a + b) for a, b in zip(stats0, stats1))
     b_alg = W.bytes_reference_passes(nbar, cbar, wbar)
     achieved = b_alg * n / (k_ms * 1e-3) / 1e9
+    traffic = None if args.ensemble_dense else measured_traffic("ensemble", args.mode, n)
     out = {
         "metric": "FP64 DEM particle-steps/s, batched ensemble of 256 systems",
         "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
@@ -291,8 +297,10 @@ def run_ensemble(args):
         "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (per-system jittered-lattice packings, seeded)",
         "config": {"workload": "ensemble: %d wall-bounded systems, %d spheres in total (1k-64k "
-                               "log-uniform), per-system restitution and friction, gravity"
-                               % (len(wl.systems), n),
+                               "log-uniform), per-system restitution and friction, gravity%s"
+                               % (len(wl.systems), n,
+                                  "; dense: spacing 1.9 r_max, u ~ N(0, 0.5 m/s), walls 0.95 "
+                                  "r_max out" if args.ensemble_dense else ""),
                    "spheres": n, "systems": len(wl.systems), "dt": wl.dt,
                    "autotuned": {"rebuild_every": tune.rebuild_every,
                                  "grid_blocks": tune.grid_blocks, "graph": tune.use_graph,
@@ -301,7 +309,11 @@ def run_ensemble(args):
                    "physics_mode": "reference" if args.mode == 0 else "corrected",
                    "parallelism": "single GPU, one launch per step for all systems"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic["bytes_per_launch"] if traffic else None,
+                     "traffic_source": traffic["source"] if traffic else None,
+                     "frac_measured_traffic": (traffic["bytes_per_launch"] / (k_ms * 1e-3) / 1e9
+                                               / HBM_PEAK_GBS if traffic else None),
                      "kernel": "fused step kernel over all systems",
                      "algorithmic_bytes_per_particle": b_alg, "kernel_ms": k_ms,
                      "launches": launches[0],
@@ -509,6 +521,9 @@ def main():
     ap.add_argument("--no-fixed-compare", dest="fixed_compare", action="store_false",
                     help="plain workloads on the automatic cadence: skip the second window "
                          "rebuilding every 20 steps")
+    ap.add_argument("--ensemble-dense", action="store_true",
+                    help="ensemble: the contact-dense variant (spacing 1.9 r_max, u ~ N(0, 0.5), "
+                         "walls 0.95 r_max outside the lattices)")
     ap.add_argument("--block", type=int, default=0, choices=(0, 64, 128, 256),
                     help="plain workloads: force-kernel tile size (dem_set_shape; 0: 128)")
     ap.add_argument("--tile-rows", type=int, default=0,
