@@ -98,7 +98,7 @@ struct dem_ctx {
   // automatic rebuild cadence (rebuild_every = DEM_REBUILD_AUTO)
   int* d_work = nullptr;       // [tiles][block] particles with contacts (split force passes)
   int* d_work_tile = nullptr;  // [tiles] their number per tile
-  int cus = 256;               // compute units of the device
+  int cus = 256;               // compute units of the device (autotuner launch grids)
   unsigned long long* d_disp = nullptr; // [2] max |x - xb|^2, max |u|^2 (bits)
   unsigned long long* h_disp = nullptr; // pinned host copy of d_disp (a pageable
                                         // destination costs a staged copy per check)
@@ -1613,13 +1613,7 @@ int dem_autotune(dem_ctx* c, int trial_steps, dem_tune_result* out) {
   *out = dem_tune_result{};
   Snapshot snap;
   if (int r = take_snapshot(c, &snap)) return r;
-  int cus = 256;
-  {
-    int dev = 0;
-    hipDeviceProp_t prop;
-    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
-      cus = prop.multiProcessorCount;
-  }
+  const int cus = c->cus;
   std::vector<int> Ks;
   const int K0 = snap.cfg.rebuild_every;
   if (snap.cfg.skin > 0.) Ks = {5, 10, 20, 40, 80, DEM_REBUILD_AUTO};

@@ -176,7 +176,6 @@ hipError_t launch_force_b128(bool ref, bool bond, bool ens, ForceKind kind, cons
 }
 
 
-int force_blocks(int n) { return (int)nblk(n); }
 
 hipError_t launch_force(ForceKind kind, const StepArgs& a, bool sample, int grid_blocks,
                         hipStream_t s) {

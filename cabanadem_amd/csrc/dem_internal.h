@@ -295,6 +295,5 @@ hipError_t launch_diag_final(const double* part, int nblocks, double* out_row,
 // Per-system sums of the tile partials: out[s][8] over tiles [tile0[s], tile0[s+1]).
 hipError_t launch_diag_systems(const double* part, const int* tile0, int nsys, double* out,
                                hipStream_t s);
-int force_blocks(int n);
 
 } // namespace demhip

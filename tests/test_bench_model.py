@@ -74,3 +74,18 @@ def test_timed_window_always_holds_a_diagnostic_row(steps):
     every = bench.diag_interval(steps)
     for start in (0, 1, 2000, 2005, 2019):
         assert any((start + s) % every == 0 for s in range(steps))
+
+
+def test_measured_traffic_records_match_workload_mode_and_size():
+    """A committed counter record is used only for the workload, arithmetic
+    mode and sphere count it was measured on."""
+    rec = bench.measured_traffic("plain_large", 0, 4_000_000)
+    assert rec is not None and 0.5e9 < rec["bytes_per_launch"] < 2e9
+    assert rec["source"].startswith("profiles/")
+    assert bench.measured_traffic("plain_large", 1, 4_000_000) is None
+    assert bench.measured_traffic("plain_large", 0, 1_000) is None
+    assert bench.measured_traffic("plain_dense", 0, 4_000_000) is None
+    packed = bench.measured_traffic("plain_packed", 1, 4_000_000)
+    assert packed is not None and packed["bytes_per_launch"] > 2 * rec["bytes_per_launch"]
+    assert bench.measured_traffic("plain_packed", 0, 4_000_000) is None
+    assert bench.measured_traffic("bonded_block", 1, 1_000_000) is not None
