@@ -46,8 +46,11 @@ constexpr int kPhase1Batch = DEM_PHASE1_BATCH; // neighbour gathers issued toget
 #ifndef DEM_DEFER_MIN_WAVES
 #define DEM_DEFER_MIN_WAVES 1
 #endif
+#ifndef DEM_DEFER_EARLY_LOADS
+#define DEM_DEFER_EARLY_LOADS 1 // k_defer loads u, omega, m, I before phase 1
+#endif
 #ifndef DEM_DEFER_BATCH
-#define DEM_DEFER_BATCH 4 // neighbour gathers in flight in k_defer (70 VGPRs, 7 waves/SIMD)
+#define DEM_DEFER_BATCH 6 // neighbour gathers in flight in k_defer (109 VGPRs, 4 waves/SIMD)
 #endif
 #ifndef DEM_PRE_MIN_WAVES
 #define DEM_PRE_MIN_WAVES 1
@@ -560,9 +563,9 @@ __device__ __forceinline__ void force_tile(const StepArgs& a, const int tile, in
   const bool pp_on = !a.nbond || a.unbonded_contacts;
   if (act) {
     xi = a.xr_in[i];
-    // deferral: u, omega, m, I and the start force only after phase 1, for
-    // the particles that finish here (fewer registers live over the gathers)
-    if (!DEFER) {
+    // loaded here, in flight over phase 1 (the deferral pass too, unless
+    // DEM_DEFER_EARLY_LOADS is 0: fewer registers, one more round trip)
+    if (!DEFER || DEM_DEFER_EARLY_LOADS) {
       load6(a.uw_in + 6l * i, ui);
       mI = a.mI[i];
     }
@@ -730,8 +733,10 @@ __device__ __forceinline__ void force_tile(const StepArgs& a, const int tile, in
     if (tid == 0) a.work_tile[tile] = total;
   }
   if (DEFER && act && !defer) {
-    load6(a.uw_in + 6l * i, ui);
-    mI = a.mI[i];
+    if (!DEM_DEFER_EARLY_LOADS) {
+      load6(a.uw_in + 6l * i, ui);
+      mI = a.mI[i];
+    }
     initial_force(a, i, mI.x, F);
   }
   if (act && !defer) {
