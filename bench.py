@@ -57,9 +57,18 @@ def emit(out):
 
 
 def diag_interval(steps):
-    """Sampling interval of the timed window's diagnostics: about ten rows,
-    at least one whatever the window's length."""
-    return max(1, steps // 10)
+    """Sampling interval of the timed window's diagnostics: at least one row
+    whatever the window's length, about ten in a long one, and no closer
+    than every 10 steps (a sampled step runs the force kernel's diagnostic
+    variant and a reduction)."""
+    return max(1, steps // 10, min(steps, 10))
+
+
+def timing_interval(steps):
+    """Every how many force passes the timed window times one with HIP
+    events: every 10th in a window of 20 steps or more (two events cost the
+    GPU a few microseconds between kernels), at least two passes timed."""
+    return max(1, min(10, steps // 2))
 
 
 TRAFFIC_RECORDS = {  # (workload, physics mode) -> committed rocprofv3 counter summary
@@ -161,7 +170,7 @@ def run_bonded(args):
         sysm.step(args.warmup)
     nrows0 = sysm.diag_history().shape[0]
     checks0 = sysm.counters().displacement_checks
-    sysm.set_timing(True)
+    sysm.set_timing(timing_interval(args.steps))
     sysm.sync()
     t0 = time.perf_counter()
     sysm.step(args.steps)
@@ -599,7 +608,7 @@ def main():
     nrows0 = sysm.diag_history().shape[0]
 
     checks0 = sysm.counters().displacement_checks
-    sysm.set_timing(True)
+    sysm.set_timing(timing_interval(args.steps))
     sysm.sync()
     t0 = time.perf_counter()
     sysm.step(args.steps)
@@ -684,6 +693,9 @@ def main():
             "algorithmic_bytes_per_particle": b_alg,
             "kernel_ms": k_ms,
             "launches": launches[0],
+            "kernel_timing": "HIP events on the context's stream around every %d-th force pass of "
+                             "the timed window (passes sampling diagnostics excluded)"
+                             % timing_interval(args.steps),
             "compulsory_bytes_per_particle_fused": b_own,
             "achieved_compulsory_fused": achieved_own,
             "frac_compulsory_fused": achieved_own / HBM_PEAK_GBS,

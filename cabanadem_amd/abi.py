@@ -561,8 +561,10 @@ class System:
         self._check(self._lib.dem_set_groups(self._h, _ptr(g, C.c_int), len(k), _ptr(k, C.c_int),
                                              _ptr(v)))
 
-    def set_timing(self, on: bool):
-        self._check(self._lib.dem_set_timing(self._h, 1 if on else 0))
+    def set_timing(self, on):
+        """True / 1: time every force pass; k > 1: every k-th; False / 0: off."""
+        every = int(on) if not isinstance(on, bool) else (1 if on else 0)
+        self._check(self._lib.dem_set_timing(self._h, every))
 
     def timing(self):
         ms = np.zeros(2)

@@ -154,7 +154,9 @@ struct dem_ctx {
   DevFlags* flags = nullptr;
   double* diag_part = nullptr;
   double* diag_rows = nullptr;
+  double* h_diag_rows = nullptr; // pinned host copy of diag_rows
   int diag_rows_cap = 0;
+  DevFlags* h_flags = nullptr;   // pinned host copy of flags
 
   bool list_valid = false;
   bool built_once = false;
@@ -171,6 +173,9 @@ struct dem_ctx {
 
   // timing
   bool timing = false;
+  int timing_every = 1;    // time every timing_every-th force pass
+  int64_t timing_seen = 0; // force passes eligible for timing so far
+  bool mark_open = false;  // mark_begin recorded a start event
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
   std::vector<std::pair<int, size_t>> ev_marks; // (slot, index of start event)
@@ -190,6 +195,8 @@ struct dem_ctx {
     hipFree(flags); hipFree(diag_part); hipFree(diag_rows); hipFree(d_disp);
     hipFree(d_work); hipFree(d_work_tile);
     if (h_disp) hipHostFree(h_disp);
+    if (h_diag_rows) hipHostFree(h_diag_rows);
+    if (h_flags) hipHostFree(h_flags);
     for (int b = 0; b < 3; ++b) { hipFree(d_tiles_b[b]); hipFree(d_tile0_b[b]); }
     hipFree(d_sys); hipFree(d_sysid); hipFree(d_grids);
     hipFree(d_kbase); hipFree(d_sysdiag);
@@ -295,23 +302,31 @@ struct dem_ctx {
   hipEvent_t next_event() {
     if (ev_used == ev_pool.size()) {
       hipEvent_t e;
-      if (hipEventCreate(&e) != hipSuccess) return nullptr;
+      if (hipEventCreateWithFlags(&e, DEM_TIMING_EVENT_FLAGS) != hipSuccess) return nullptr;
       ev_pool.push_back(e);
     }
     return ev_pool[ev_used++];
   }
-  void mark_begin(int slot) {
+  // Events around a force pass (slot 0) or a rebuild (slot 1).  Force passes
+  // are timed every timing_every-th one, sampled passes (the diagnostic
+  // variant, another kernel) never and without advancing the count: each
+  // pair of events costs the GPU a few microseconds between the kernels.
+  void mark_begin(int slot, bool sampled = false) {
+    mark_open = false;
     if (!timing || capturing) return;
-    hipEvent_t e = next_event();
-    if (!e) return;
-    hipEventRecord(e, stream);
-    ev_marks.push_back({slot, ev_used - 1});
+    if (slot == 0 && (sampled || timing_seen++ % timing_every != 0)) return;
+    const size_t i = ev_used;
+    hipEvent_t e0 = next_event();
+    hipEvent_t e1 = e0 ? next_event() : nullptr;
+    if (!e1) return;
+    ev_marks.push_back({slot, i});
+    hipEventRecord(e0, stream);
+    mark_open = true;
   }
   void mark_end() {
-    if (!timing || capturing || ev_marks.empty()) return;
-    hipEvent_t e = next_event();
-    if (!e) return;
-    hipEventRecord(e, stream);
+    if (!mark_open) return;
+    mark_open = false;
+    hipEventRecord(ev_pool[ev_marks.back().second + 1], stream);
   }
   void collect_timing() {
     if (!timing) { ev_used = 0; ev_marks.clear(); return; }
@@ -327,11 +342,19 @@ struct dem_ctx {
     ev_marks.clear();
   }
 
-  int check_flags(DevFlags* out = nullptr) {
-    DevFlags f;
-    hipError_t e = hipMemcpyAsync(&f, flags, sizeof(f), hipMemcpyDeviceToHost, stream);
+  // The flags' read-back into pinned memory, enqueued (allocated on first use).
+  hipError_t enqueue_flags() {
+    hipError_t e = hipSuccess;
+    if (!h_flags) e = hipHostMalloc(reinterpret_cast<void**>(&h_flags), sizeof(DevFlags), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMemcpyAsync(h_flags, flags, sizeof(DevFlags), hipMemcpyDeviceToHost, stream);
+    return e;
+  }
+  // enqueued: the caller has enqueued the read-back already
+  int check_flags(DevFlags* out = nullptr, bool enqueued = false) {
+    hipError_t e = enqueued ? hipSuccess : enqueue_flags();
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
     if (e != hipSuccess) return hip_fail(e, "flag read-back");
+    const DevFlags f = *h_flags;
     max_nn_seen = std::max(max_nn_seen, f.max_nn);
     if (out) *out = f;
     if (f.nonfinite) return fail(DEM_ERR_NONFINITE, "non-finite particle position met by the cell binning");
@@ -352,9 +375,9 @@ struct dem_ctx {
     return DEM_OK;
   }
   // Reports and clears the sticky device error flags.
-  int check_and_clear_flags() {
+  int check_and_clear_flags(bool enqueued = false) {
     DevFlags f{};
-    int r = check_flags(&f);
+    int r = check_flags(&f, enqueued);
     if (r != DEM_OK && r != DEM_ERR_HIP) {
       DevFlags z{};
       z.bonds_broken = f.bonds_broken; // a running count, not an error
@@ -362,6 +385,25 @@ struct dem_ctx {
       hipStreamSynchronize(stream);
     }
     return r;
+  }
+
+  // Device rows for the sampled diagnostics of one call and their pinned
+  // host copy.
+  int ensure_diag_rows(int rows) {
+    if (rows <= diag_rows_cap) return DEM_OK;
+    hipFree(diag_rows);
+    diag_rows = nullptr;
+    if (h_diag_rows) hipHostFree(h_diag_rows);
+    h_diag_rows = nullptr;
+    diag_rows_cap = 0;
+    hipError_t e = hipSuccess;
+    diag_rows = dalloc<double>((size_t)rows * 9, &e);
+    if (e == hipSuccess)
+      e = hipHostMalloc(reinterpret_cast<void**>(&h_diag_rows), (size_t)rows * 9 * sizeof(double),
+                        hipHostMallocDefault);
+    if (e != hipSuccess) return hip_fail(e, "diagnostic rows");
+    diag_rows_cap = rows;
+    return DEM_OK;
   }
 
   // Particle-particle contacts are found through the neighbour rows (every
@@ -827,7 +869,8 @@ int create_ctx(const dem_config* common, int nsys, const dem_system_desc* sd, de
     c->d_work = dalloc<int>(slots, &e);
     c->d_work_tile = dalloc<int>((size_t)c->ntiles_b[0], &e); // the most tiles (64 slots)
   }
-  c->diag_part = dalloc<double>((size_t)c->ntiles_b[0] * 8, &e); // the most tiles (64 slots)
+  // the most tiles (64 slots), then the reduction's group partials
+  c->diag_part = dalloc<double>(((size_t)c->ntiles_b[0] + kDiagThreads) * 8, &e);
   c->d_sys = dalloc<SysParams>((size_t)nsys, &e);
   for (int b = 0; b < 3; ++b) c->d_tile0_b[b] = dalloc<int>((size_t)nsys + 1, &e);
   c->d_sysdiag = dalloc<double>((size_t)nsys * 8, &e);
@@ -1327,7 +1370,8 @@ int dem_stage3(dem_ctx* c) { return kick_call(c, 1, 0); }
 int dem_set_diag_every(dem_ctx* c, int every) {
   if (int r = check_ctx(c)) return r;
   c->diag_every = every > 0 ? every : 0;
-  return DEM_OK;
+  // rows for the calls of up to 64 samples, allocated here rather than in a step
+  return c->diag_every > 0 ? c->ensure_diag_rows(64) : DEM_OK;
 }
 
 // Enqueues nsteps full steps (the first kick, then fused steps, the last one
@@ -1350,7 +1394,7 @@ static int enqueue_steps(dem_ctx* c, int nsteps, int* row) {
     const bool sample = c->diag_every > 0 && gstep % c->diag_every == 0;
     const bool last = (s == nsteps - 1);
     StepArgs a = c->step_args();
-    c->mark_begin(0);
+    c->mark_begin(0, sample);
     e = launch_force(last ? FK_STEP_LAST : FK_STEP_FUSED, a, sample, c->grid_blocks, c->stream);
     c->mark_end();
     if (e != hipSuccess) return c->hip_fail(e, "step launch");
@@ -1419,14 +1463,7 @@ int dem_step(dem_ctx* c, int nsteps) {
   if (c->diag_every > 0)
     for (int s = 0; s < nsteps; ++s)
       if ((c->step + s) % c->diag_every == 0) ++rows;
-  if (rows > c->diag_rows_cap) {
-    hipFree(c->diag_rows);
-    c->diag_rows = nullptr;
-    hipError_t e = hipSuccess;
-    c->diag_rows = dalloc<double>((size_t)rows * 9, &e);
-    if (e != hipSuccess) return c->hip_fail(e, "hipMalloc diag rows");
-    c->diag_rows_cap = rows;
-  }
+  if (int r = c->ensure_diag_rows(rows)) return r;
   int row = 0;
   hipError_t e;
   // captured launches carry fixed arguments, so steps whose walls translate
@@ -1464,23 +1501,26 @@ int dem_step(dem_ctx* c, int nsteps) {
     if (left > 0)
       if (int r = enqueue_steps(c, left, &row)) return r;
   }
+  // the rows and the flags into pinned memory, then one synchronisation
   if (row > 0) {
-    std::vector<double> h((size_t)row * 9);
-    e = hipMemcpyAsync(h.data(), c->diag_rows, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    e = hipMemcpyAsync(c->h_diag_rows, c->diag_rows, (size_t)row * 9 * sizeof(double), hipMemcpyDeviceToHost,
+                       c->stream);
     if (e != hipSuccess) return c->hip_fail(e, "diag read-back");
-    const double nD = (double)c->n;
-    for (int r = 0; r < row; ++r) {
-      double* q = &h[(size_t)r * 9];
-      q[6] /= nD; q[7] /= nD; q[8] /= nD; // sums -> per-particle means
-      c->diag_hist.insert(c->diag_hist.end(), q, q + 9);
-      c->last_sample[0] = q[6]; c->last_sample[1] = q[7]; c->last_sample[2] = q[8];
-    }
   }
-  e = hipStreamSynchronize(c->stream);
-  if (e != hipSuccess) return c->hip_fail(e, "step sync");
+  e = c->enqueue_flags();
+  if (e != hipSuccess) return c->hip_fail(e, "flag read-back");
+  const int rf = c->check_and_clear_flags(true);
+  if (rf == DEM_ERR_HIP) return rf;
+  const double nD = (double)c->n;
+  for (int r = 0; r < row; ++r) {
+    double q[9];
+    std::copy(c->h_diag_rows + (size_t)r * 9, c->h_diag_rows + (size_t)r * 9 + 9, q);
+    q[6] /= nD; q[7] /= nD; q[8] /= nD; // sums -> per-particle means
+    c->diag_hist.insert(c->diag_hist.end(), q, q + 9);
+    c->last_sample[0] = q[6]; c->last_sample[1] = q[7]; c->last_sample[2] = q[8];
+  }
   c->collect_timing();
-  return c->check_and_clear_flags();
+  return rf;
 }
 
 int dem_set_graph(dem_ctx* c, int enable) {
@@ -1710,12 +1750,7 @@ int dem_diagnostics(dem_ctx* c, double out[5]) {
   StepArgs a = c->step_args();
   hipError_t e = launch_diag_current(a, c->uw[c->cP], c->stream);
   if (e != hipSuccess) return c->hip_fail(e, "diag launch");
-  if (c->diag_rows_cap < 1) {
-    hipError_t ea = hipSuccess;
-    c->diag_rows = dalloc<double>(9, &ea);
-    if (ea != hipSuccess) return c->hip_fail(ea, "hipMalloc diag rows");
-    c->diag_rows_cap = 1;
-  }
+  if (int r = c->ensure_diag_rows(1)) return r;
   e = launch_diag_final(c->diag_part, c->ntiles, c->diag_rows, (double)c->step, c->stream);
   double h[9];
   if (e == hipSuccess) e = hipMemcpyAsync(h, c->diag_rows, sizeof(h), hipMemcpyDeviceToHost, c->stream);
@@ -1755,6 +1790,8 @@ int dem_set_timing(dem_ctx* c, int enable) {
   hipStreamSynchronize(c->stream);
   c->collect_timing();
   c->timing = enable != 0;
+  c->timing_every = enable > 1 ? enable : 1;
+  c->timing_seen = 0;
   c->t_ms[0] = c->t_ms[1] = 0.;
   c->t_launches[0] = c->t_launches[1] = 0;
   return DEM_OK;

@@ -147,19 +147,32 @@ __global__ __launch_bounds__(kBlock) void k_diag_systems(const double* part, con
 }
 
 // Sum block partials in block order; row = (step, 8 sums).
-__global__ __launch_bounds__(kBlock) void k_diag_final(const double* part, int nblocks,
-                                                       double* row, double step) {
+// The per-tile partials summed in two fixed-order stages (the result does
+// not depend on timing): workgroup g sums tiles [g chunk, (g + 1) chunk)
+// into group partial g, then one workgroup sums the groups into the row.
+__global__ __launch_bounds__(kDiagThreads) void k_diag_groups(const double* part, int nblocks, int chunk,
+                                                              double* gpart) {
   double v[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
-  for (int b = threadIdx.x; b < nblocks; b += kBlock)
-    for (int k = 0; k < 8; ++k) v[k] += part[(long)b * 8 + k];
+  const long b0 = (long)blockIdx.x * chunk;
+  const long b1 = b0 + chunk < nblocks ? b0 + chunk : nblocks;
+  for (long b = b0 + threadIdx.x; b < b1; b += kDiagThreads)
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] += part[b * 8 + k];
+  block_reduce8<kDiagThreads>(v, gpart, blockIdx.x);
+}
+
+__global__ __launch_bounds__(kDiagThreads) void k_diag_final(const double* gpart, int ngroups, double* row,
+                                                             double step) {
+  double v[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
+  if ((int)threadIdx.x < ngroups)
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = gpart[threadIdx.x * 8 + k];
   __shared__ double tmp[8];
-  double* scratch = tmp;
-  // block_reduce8 writes part-style output; reuse it into a local slot.
-  block_reduce8<kBlock>(v, scratch, 0);
+  block_reduce8<kDiagThreads>(v, tmp, 0);
   __syncthreads();
   if (threadIdx.x == 0) {
     row[0] = step;
-    for (int k = 0; k < 8; ++k) row[1 + k] = scratch[k];
+    for (int k = 0; k < 8; ++k) row[1 + k] = tmp[k];
   }
 }
 
@@ -237,9 +250,14 @@ hipError_t launch_diag_systems(const double* part, const int* tile0, int nsys, d
   return hipGetLastError();
 }
 
-hipError_t launch_diag_final(const double* part, int nblocks, double* row, double step,
-                             hipStream_t s) {
-  hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(kBlock), 0, s, part, nblocks, row, step);
+hipError_t launch_diag_final(double* part, int nblocks, double* row, double step, hipStream_t s) {
+  // the group partials follow the tile partials ([kDiagThreads][8] more)
+  const int chunk = nblocks > kDiagThreads ? (nblocks + kDiagThreads - 1) / kDiagThreads : 1;
+  const int ngroups = nblocks > 0 ? (nblocks + chunk - 1) / chunk : 1;
+  double* gpart = part + (long)(nblocks > 0 ? nblocks : 0) * 8;
+  hipLaunchKernelGGL(k_diag_groups, dim3(ngroups), dim3(kDiagThreads), 0, s, (const double*)part, nblocks,
+                     chunk, gpart);
+  hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(kDiagThreads), 0, s, (const double*)gpart, ngroups, row, step);
   return hipGetLastError();
 }
 

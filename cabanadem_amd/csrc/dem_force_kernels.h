@@ -413,12 +413,13 @@ __device__ inline void wave_lds_sync() {
 }
 
 // Bytes of dynamic LDS per workgroup of B threads: the contact ids found in
-// phase 1 (int [max_c][B]), for the wave pass also the old slot of each
-// (byte [max_c][B], padded to 8) and the contact shares (double [B][6]).
+// phase 1 (int [max_c][B]), for the wave/tile pass also the old slot of each
+// (byte [max_c][B], padded to 8), the contact shares (double [B][6]) and the
+// lanes' first contact numbers (int [B]).
 __host__ __device__ inline size_t force_lds_bytes(int max_c, bool wave, int B) {
   const size_t mc = (size_t)(max_c > 0 ? max_c : 1);
   size_t b = mc * B * sizeof(int);
-  if (wave) b += (mc * B + 7) / 8 * 8 + (size_t)B * 6 * sizeof(double);
+  if (wave) b += (mc * B + 7) / 8 * 8 + (size_t)B * 6 * sizeof(double) + (size_t)B * sizeof(int);
   return b;
 }
 
@@ -500,13 +501,98 @@ __device__ __forceinline__ void wave_contacts(const StepArgs& a, const SysParams
   }
 }
 
+// The same over a whole tile of B > 64 lanes: the tile numbers its contacts
+// owner-major (a wave scan, then the waves' totals through LDS) and runs them
+// in rounds of B, every lane taking one; owners add their shares in contact
+// order, as above.  With few contacts per tile (a sparse bed) one wave of
+// the tile runs them all and the others skip the FP64 block, where a lane
+// walk or a wave pass has every wave holding a contact run it.  Every lane
+// of the workgroup must call this (barriers).
+template <bool REF, bool PRE, int B>
+__device__ __forceinline__ void tile_contacts(const StepArgs& a, const SysParams& P, const int first,
+                                              const int nc, const int* s_cj, const unsigned char* s_op,
+                                              double* s_con, int* s_ex, double* F, double* T) {
+  if constexpr (B == 64) {
+    wave_contacts<REF, PRE, B>(a, P, first, nc, s_cj, s_op, s_con, F, T);
+  } else {
+    constexpr int NW = B / 64;
+    __shared__ int s_wt[NW];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const long ld = a.ld;
+    int incl = nc;
+    for (int off = 1; off < 64; off <<= 1) {
+      const int t = __shfl_up(incl, off);
+      if (lane >= off) incl += t;
+    }
+    if (lane == 63) s_wt[wv] = incl;
+    __syncthreads(); // also orders every lane's s_cj / s_op columns before the rounds
+    int wb = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      const int t = s_wt[w];
+      wb += w < wv ? t : 0;
+      total += t;
+    }
+    incl += wb;
+    const int excl = incl - nc;
+    s_ex[tid] = excl;
+    __syncthreads();
+    for (int base = 0; base < total; base += B) {
+      const int k = base + tid;
+      if (k < total) {
+        // owner: the last lane whose first contact number is <= k
+        int L = 0;
+#pragma unroll
+        for (int st = B / 2; st > 0; st >>= 1)
+          if (s_ex[L + st] <= k) L += st;
+        const int m = k - s_ex[L];
+        const int io = first + L;
+        const int j = PRE ? a.cj_out[m * ld + io] : s_cj[m * B + L];
+        const int p1 = s_op[m * B + L];
+        double h[3] = {0., 0., 0.};
+        if (p1 > 0) { // continued contact: src/CabanaDEM_Force.hpp:166-178
+          h[0] = a.ch_in[(3 * (p1 - 1) + 0) * ld + io];
+          h[1] = a.ch_in[(3 * (p1 - 1) + 1) * ld + io];
+          h[2] = a.ch_in[(3 * (p1 - 1) + 2) * ld + io];
+        }
+        const double4 xo = a.xr_in[io];
+        double uo[6];
+        load6(a.uw_in + 6l * io, uo);
+        const double4 mo = a.mat[io];
+        const double4 xj = a.xr_in[j];
+        double uj[6];
+        load6(a.uw_in + 6l * j, uj);
+        const double4 mj = a.mat[j];
+        double Fc[3] = {-0., -0., -0.}, Tc[3] = {-0., -0., -0.};
+        pp_contact<REF>(xo, uo, mo, xj, uj, mj, h, Fc, Tc, a, P);
+        if (!PRE) a.cj_out[m * ld + io] = j;
+        a.ch_out[(3 * m + 0) * ld + io] = h[0];
+        a.ch_out[(3 * m + 1) * ld + io] = h[1];
+        a.ch_out[(3 * m + 2) * ld + io] = h[2];
+        double* c = s_con + (long)tid * 6;
+        c[0] = Fc[0]; c[1] = Fc[1]; c[2] = Fc[2];
+        c[3] = Tc[0]; c[4] = Tc[1]; c[5] = Tc[2];
+      }
+      __syncthreads();
+      const int q0 = excl > base ? excl : base;
+      const int q1 = incl < base + B ? incl : base + B;
+      for (int q = q0; q < q1; ++q) {
+        const double* c = s_con + (long)(q - base) * 6;
+        F[0] += c[0]; F[1] += c[1]; F[2] += c[2];
+        if (!REF) { T[0] += c[3]; T[1] += c[4]; T[2] += c[5]; }
+      }
+      __syncthreads(); // the next round rewrites the shares
+    }
+  }
+}
+
 // KICK: 0 = forces only (write F,T); 1 = + stage3 (write state, F,T);
 //       2 = + stage3 + stage1 + stage2 of the next step (write state).
 // One tile: B consecutive slots of one system.  ENS: an ensemble (tile
 // table and per-system parameters from the device); otherwise one system
 // whose parameters travel in the kernel arguments.  Each lane walks its own
-// contact list, or (a.wave, systems without bonds) the wave runs the contact
-// arithmetic of all its lanes spread over the lanes (wave_contacts).
+// contact list, or (a.wave, systems without bonds) the tile runs the contact
+// arithmetic of all its lanes spread over the lanes (tile_contacts).
 // PASS (systems without bonds, whole force passes, no sampling):
 //   0  everything in one pass;
 //   1  deferral: phase 1 only; a particle without particle or wall contacts
@@ -669,9 +755,10 @@ __device__ __forceinline__ void force_tile(const StepArgs& a, const int tile, in
   }
   const bool wave = WAVE && pp_on && a.wave; // uniform over the launch
   if (wave) {
-    // every lane of the wave, active or not: the contacts of the wave
+    // every lane of the tile, active or not: the contacts of the tile
     wave_lds_sync();
-    wave_contacts<REF, PRE, B>(a, P, first, act ? nc : 0, s_cj, s_op, s_con, F, T);
+    tile_contacts<REF, PRE, B>(a, P, first, act ? nc : 0, s_cj, s_op, s_con,
+                               reinterpret_cast<int*>(s_con + (long)B * 6), F, T);
   } else if (!DEFER && PP && pp_on && act) {
     // Phase 2 walking each lane's own list:
     // contact arithmetic, merged with last step's tracked list.

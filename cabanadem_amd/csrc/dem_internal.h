@@ -48,6 +48,15 @@ constexpr int kXcds = 8;          // MI355X: 8 XCDs, workgroups dealt round-robi
 #endif
 constexpr int kTileRows = DEM_TILE_ROWS;      // default y-rows per cell tile
 constexpr int kMaxTileRows = 32;              // dem_set_shape accepts 1..32
+// Workgroup size and number of groups of the diagnostics' second stage; the
+// tile partials are followed by kDiagThreads * 8 doubles of group partials.
+constexpr int kDiagThreads = 256;
+
+// Timing events skip the system-scope fence (an L2 write-back and
+// invalidation per event that the kernels after it would pay for).
+#ifndef DEM_TIMING_EVENT_FLAGS
+#define DEM_TIMING_EVENT_FLAGS hipEventDisableSystemFence
+#endif
 
 // Logical block index that gives each XCD one contiguous run of blocks, so
 // that the neighbour gathers of one XCD stay inside its own L2.  Workgroup b
@@ -180,8 +189,8 @@ struct StepArgs {
   int split;
   int* work;             // [ntiles][block] each tile's particles with contacts (split passes)
   int* work_tile;        // [ntiles] their number per tile
-  // 1: the contact arithmetic runs as a wave pass over the contacts of all
-  // lanes of a wave (systems without bonds); 0: each lane walks its own list
+  // 1: the contact arithmetic of all lanes of a tile runs spread over the
+  // tile's lanes (systems without bonds); 0: each lane walks its own list
   int wave;
   // time of this force pass since the walls' positions were set (moving
   // walls, corrected mode)
@@ -290,8 +299,9 @@ hipError_t launch_force_b128(bool ref, bool bond, bool ens, ForceKind kind, cons
                              bool sample, int grid_blocks, hipStream_t s);
 hipError_t launch_force_b256(bool ref, bool ens, ForceKind kind, const StepArgs& a, bool sample,
                              int grid_blocks, hipStream_t s);
-hipError_t launch_diag_final(const double* part, int nblocks, double* out_row,
-                             double step, hipStream_t s);
+// Sum of the tile partials part[nblocks][8] into out_row (step, then the 8
+// sums); part must hold kDiagThreads * 8 more doubles after the tiles'.
+hipError_t launch_diag_final(double* part, int nblocks, double* out_row, double step, hipStream_t s);
 // Per-system sums of the tile partials: out[s][8] over tiles [tile0[s], tile0[s+1]).
 hipError_t launch_diag_systems(const double* part, const int* tile0, int nsys, double* out,
                                hipStream_t s);

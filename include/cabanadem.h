@@ -338,8 +338,9 @@ int dem_set_launch(dem_ctx* ctx, int grid_blocks);
  *     each particle's overlapping neighbours, then the force kernel running
  *     the contact arithmetic as a wave pass (below);
  *   DEM_FORCE_PASS_WAVE: one kernel; each lane finds its contacts, then the
- *     wave runs the contact arithmetic of all its lanes' contacts spread
- *     over its 64 lanes, in the same summation order.
+ *     tile (workgroup) runs the contact arithmetic of all its lanes'
+ *     contacts spread over its lanes, one contact per lane per round, in
+ *     the same summation order.
  * Bonded systems always run DEM_FORCE_PASS_FUSED. */
 enum dem_force_pass { DEM_FORCE_PASS_FUSED = 0, DEM_FORCE_PASS_SPLIT = 1, DEM_FORCE_PASS_WAVE = 2 };
 int dem_set_force_pass(dem_ctx* ctx, int mode);
@@ -398,7 +399,11 @@ typedef struct dem_tune_result {
 int dem_autotune(dem_ctx* ctx, int trial_steps, dem_tune_result* out);
 
 /* ---- kernel timing (HIP events on the context's stream) ------------------ */
-/* Enables per-launch HIP-event timing of the kernels dem_step launches. */
+/* Enables per-launch HIP-event timing of the kernels dem_step launches
+ * (0: off).  enable = k > 1 times every k-th force pass only (each timed
+ * pass costs the GPU a few microseconds of idle time around its events);
+ * passes that sample diagnostics (another kernel variant) are not timed.
+ * Every rebuild is timed. */
 int dem_set_timing(dem_ctx* ctx, int enable);
 /* Accumulated milliseconds and launch counts for the fused step kernel
  * (slot 0) and the rebuild sequence (slot 1). */

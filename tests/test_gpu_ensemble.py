@@ -140,6 +140,7 @@ def test_force_pass_shapes_ensemble_are_bitwise_equal(gpu_lib):
     b.set_graph(True)
     c = wl.system()
     c.set_force_pass(abi.FORCE_PASS_WAVE)
+    c.set_launch(20)
     # other force tiles: their own tile tables
     d = wl.system()
     d.set_force_pass(abi.FORCE_PASS_WAVE)

@@ -252,6 +252,38 @@ def test_tile_shape_arguments(gpu_lib):
     sysm.close()
 
 
+def test_kernel_timing_samples_force_passes(gpu_lib):
+    """dem_set_timing(k): every k-th force pass is timed, passes that sample
+    diagnostics never (and they do not advance the count); every rebuild is
+    timed; timing does not change the run."""
+    st, w, lo, hi, dt, rc = _dense_state(seed=6)
+    kw = dict(dt=dt, rc=rc, lo=lo, hi=hi, rebuild_every=5)
+    a = make_system(st, w, **kw)
+    b = make_system(st, w, **kw)
+    b.set_timing(1)
+    b.step(12)
+    ms, n = b.timing()
+    assert n == [12, 3] and ms[0] > 0 and ms[1] > 0
+    b.set_timing(4)
+    b.step(12)  # passes 0, 4, 8 of the call; rebuilds at steps 15, 20
+    assert b.timing()[1] == [3, 2]
+    b.set_diag_every(4)  # global steps 24, 28, 32 sample diagnostics
+    b.set_timing(4)
+    b.step(12)  # eligible: 25, 26, 27, 29, 30, 31, 33, 34, 35 -> 25, 30, 35 timed
+    assert b.timing()[1][0] == 3
+    b.set_timing(False)
+    a.step(12)
+    a.step(12)
+    a.set_diag_every(4)
+    a.step(12)
+    assert np.array_equal(a.diag_history(), b.diag_history())
+    sa, sb = a.download(), b.download()
+    for f in ("x", "u", "omega", "force", "torque"):
+        assert np.array_equal(getattr(sa, f), getattr(sb, f)), f
+    a.close()
+    b.close()
+
+
 def test_automatic_rebuild_cadence_matches_every_step_rebuilds(gpu_lib):
     """rebuild_every = DEM_REBUILD_AUTO: the rows are rebuilt only when the
     displacement budget (skin/2) may run out, and the run equals the oracle
