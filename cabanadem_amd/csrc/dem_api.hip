@@ -308,13 +308,14 @@ struct dem_ctx {
     return ev_pool[ev_used++];
   }
   // Events around a force pass (slot 0) or a rebuild (slot 1).  Force passes
-  // are timed every timing_every-th one, sampled passes (the diagnostic
-  // variant, another kernel) never and without advancing the count: each
-  // pair of events costs the GPU a few microseconds between the kernels.
-  void mark_begin(int slot, bool sampled = false) {
+  // are timed every timing_every-th one; passes that also sample diagnostics
+  // (another kernel variant) or check the displacements (extra reads) never,
+  // and without advancing the count.  Each pair of events costs the GPU a few
+  // microseconds between the kernels.
+  void mark_begin(int slot, bool skip = false) {
     mark_open = false;
     if (!timing || capturing) return;
-    if (slot == 0 && (sampled || timing_seen++ % timing_every != 0)) return;
+    if (slot == 0 && (skip || timing_seen++ % timing_every != 0)) return;
     const size_t i = ev_used;
     hipEvent_t e0 = next_event();
     hipEvent_t e1 = e0 ? next_event() : nullptr;
@@ -516,12 +517,13 @@ struct dem_ctx {
     const double m = std::floor(budget / per_step) - 1.;
     return m < 1. ? 0 : (m > cap ? cap : (int)m);
   }
+  // The maxima that the step's force pass has just computed (StepArgs::disp)
+  // decide the next interval.
   int auto_check() {
     ++auto_checks;
     hipError_t e = hipSuccess;
     if (!h_disp) e = hipHostMalloc(reinterpret_cast<void**>(&h_disp), 2 * sizeof(unsigned long long),
                                    hipHostMallocDefault);
-    if (e == hipSuccess) e = launch_disp_check(n, xr[cP], xb, uw[cP], d_disp, stream);
     if (e == hipSuccess)
       e = hipMemcpyAsync(h_disp, d_disp, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream);
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
@@ -1394,7 +1396,15 @@ static int enqueue_steps(dem_ctx* c, int nsteps, int* row) {
     const bool sample = c->diag_every > 0 && gstep % c->diag_every == 0;
     const bool last = (s == nsteps - 1);
     StepArgs a = c->step_args();
-    c->mark_begin(0, sample);
+    // the displacement check after this step, computed by its own force pass
+    const bool check = autoK && c->auto_left <= 1;
+    if (check) {
+      e = launch_zero_u64(c->d_disp, 2, c->stream);
+      if (e != hipSuccess) return c->hip_fail(e, "displacement check clear");
+      a.xb = c->xb;
+      a.disp = c->d_disp;
+    }
+    c->mark_begin(0, sample || check);
     e = launch_force(last ? FK_STEP_LAST : FK_STEP_FUSED, a, sample, c->grid_blocks, c->stream);
     c->mark_end();
     if (e != hipSuccess) return c->hip_fail(e, "step launch");

@@ -420,50 +420,6 @@ __global__ __launch_bounds__(kBlock) void k_setup_bonds(BondSetupArgs a) {
   a.nbond[i] = nb < a.max_b ? nb : a.max_b;
 }
 
-// Largest squared displacement since the last build (positions x against
-// the build's xb, both in the current slot order) and largest squared speed,
-// over all particles.  One atomic per wave and quantity: non-negative
-// doubles order like their bit patterns, so atomicMax on the bits is a max
-// on the values (a NaN lands above every finite value).
-// Largest |x - xb|^2 and |u|^2 over all particles (NaN wins): a grid-stride
-// loop, a wave reduction, a block reduction through LDS, then one atomicMax
-// per block and value on the bit patterns (order-preserving for doubles
-// >= 0 and sending NaN to the top).
-constexpr int kDispBlock = 256;
-constexpr int kDispBlocksPerCU = 4;
-__device__ __forceinline__ double max_nan(double a, double b) { return (b > a || b != b) ? b : a; }
-
-__global__ __launch_bounds__(kDispBlock) void k_disp_check(int n, const double4* xr, const double* xb,
-                                                          const double* uw, unsigned long long* out) {
-  double d2 = 0., v2 = 0.;
-  for (int i = blockIdx.x * kDispBlock + threadIdx.x; i < n; i += gridDim.x * kDispBlock) {
-    const double4 p = xr[i];
-    const double d0 = p.x - xb[3 * i + 0], d1 = p.y - xb[3 * i + 1], dz = p.z - xb[3 * i + 2];
-    d2 = max_nan(d2, d0 * d0 + d1 * d1 + dz * dz);
-    const double* u = uw + 6l * i;
-    v2 = max_nan(v2, u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
-  }
-  for (int off = 32; off > 0; off >>= 1) {
-    d2 = max_nan(d2, __shfl_xor(d2, off));
-    v2 = max_nan(v2, __shfl_xor(v2, off));
-  }
-  __shared__ double s_d[kDispBlock / 64], s_v[kDispBlock / 64];
-  const int w = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) {
-    s_d[w] = d2;
-    s_v[w] = v2;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int k = 1; k < kDispBlock / 64; ++k) {
-      d2 = max_nan(d2, s_d[k]);
-      v2 = max_nan(v2, s_v[k]);
-    }
-    atomicMax(&out[0], (unsigned long long)__double_as_longlong(d2));
-    atomicMax(&out[1], (unsigned long long)__double_as_longlong(v2));
-  }
-}
-
 __global__ void k_zero_u64(unsigned long long* p, int count) {
   if ((int)threadIdx.x < count) p[threadIdx.x] = 0ull;
 }
@@ -472,18 +428,8 @@ inline unsigned nblk(long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 } // namespace
 
-hipError_t launch_disp_check(int n, const double4* xr, const double* xb, const double* uw,
-                             unsigned long long* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, out, 2);
-  static const int cus = [] {
-    int dev = 0, c = 0;
-    return (hipGetDevice(&dev) == hipSuccess &&
-            hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && c > 0)
-               ? c : 256;
-  }();
-  const long need = ((long)n + kDispBlock - 1) / kDispBlock;
-  const unsigned grid = (unsigned)std::max(1L, std::min(need, (long)cus * kDispBlocksPerCU));
-  hipLaunchKernelGGL(k_disp_check, dim3(grid), dim3(kDispBlock), 0, s, n, xr, xb, uw, out);
+hipError_t launch_zero_u64(unsigned long long* p, int count, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, p, count);
   return hipGetLastError();
 }
 

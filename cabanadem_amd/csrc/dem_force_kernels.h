@@ -405,6 +405,44 @@ __device__ inline void block_reduce8(const double* v, double* part, int blk) {
   }
 }
 
+// The displacement check's terms of one written particle (automatic
+// cadence): |x - xb|^2 and |u|^2 of the state this pass writes.
+__device__ __forceinline__ void disp_terms(const StepArgs& a, int i, const double4& p, const double* u,
+                                           double& d2, double& v2) {
+  const double d0 = p.x - a.xb[3 * i + 0], d1 = p.y - a.xb[3 * i + 1], dz = p.z - a.xb[3 * i + 2];
+  d2 = max_nan(0., d0 * d0 + d1 * d1 + dz * dz);
+  v2 = max_nan(0., u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+}
+
+// The tile's maxima into out[0], out[1] (bits); the atomic is skipped when
+// the value already there is at least as large (the maxima only grow).
+// Every lane of the workgroup must call this (barriers).
+template <int B>
+__device__ inline void disp_block_max(double d2, double v2, unsigned long long* out) {
+  __shared__ double s_d[B / 64], s_v[B / 64];
+  for (int off = 32; off > 0; off >>= 1) {
+    d2 = max_nan(d2, __shfl_xor(d2, off));
+    v2 = max_nan(v2, __shfl_xor(v2, off));
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    s_d[w] = d2;
+    s_v[w] = v2;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < B / 64; ++k) {
+      d2 = max_nan(d2, s_d[k]);
+      v2 = max_nan(v2, s_v[k]);
+    }
+    const unsigned long long bd = (unsigned long long)__double_as_longlong(d2);
+    const unsigned long long bv = (unsigned long long)__double_as_longlong(v2);
+    if (bd > __hip_atomic_load(&out[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(&out[0], bd);
+    if (bv > __hip_atomic_load(&out[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(&out[1], bv);
+  }
+  __syncthreads(); // the next tile of a persistent grid rewrites s_d, s_v
+}
+
 // A wave's own LDS columns: its writes before its later reads, and back.
 __device__ inline void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -637,6 +675,7 @@ __device__ __forceinline__ void force_tile(const StepArgs& a, const int tile, in
       reinterpret_cast<char*>(s_cj) + force_lds_bytes(a.max_c, false, B) +
       ((size_t)(a.max_c > 0 ? a.max_c : 1) * B + 7) / 8 * 8);
   double dg[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
+  double cd2 = 0., cv2 = 0.; // displacement check terms (a.disp)
   double4 xi = make_double4(0., 0., 0., 0.);
   double ui[6] = {0., 0., 0., 0., 0., 0.};
   double2 mI = make_double2(1., 1.); // m, I
@@ -899,11 +938,13 @@ __device__ __forceinline__ void force_tile(const StepArgs& a, const int tile, in
         xo.w = xi.w;
         a.xr_out[i] = xo;
         store6(a.uw_out + 6l * i, s);
+        if (a.disp) disp_terms(a, i, xo, s, cd2, cv2);
       } else {
         a.xr_out[i] = xi;
         store6(a.uw_out + 6l * i, s);
         double ft[6] = {F[0], F[1], F[2], T[0], T[1], T[2]};
         store6(a.ft + 6l * i, ft);
+        if (a.disp) disp_terms(a, i, xi, s, cd2, cv2);
       }
     }
     dg[5] = (double)nnb;
@@ -911,6 +952,7 @@ __device__ __forceinline__ void force_tile(const StepArgs& a, const int tile, in
     dg[7] = (double)nwc;
   }
   if (DIAG) block_reduce8<B>(dg, a.diag_part, tile);
+  if (KICK != 0 && a.disp) disp_block_max<B>(cd2, cv2, a.disp);
 }
 
 // The fused force / step kernel.  A grid of one workgroup per tile, or a

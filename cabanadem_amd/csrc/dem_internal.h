@@ -195,7 +195,16 @@ struct StepArgs {
   // time of this force pass since the walls' positions were set (moving
   // walls, corrected mode)
   double tw;
+  // the displacement check of the automatic cadence fused into this pass,
+  // or null: max |x_out - xb|^2 and max |u_out|^2 over the particles it
+  // writes, as bits into disp[0], disp[1] (cleared before the pass)
+  const double* xb;      // [n][3] positions at the last build
+  unsigned long long* disp;
 };
+
+// Maximum that sends NaN to the top (the displacement check's maxima are
+// >= 0, so their bits order as the values do).
+__device__ __forceinline__ double max_nan(double a, double b) { return (b > a || b != b) ? b : a; }
 
 struct BuildArgs {
   int n, ld, max_n, max_c;
@@ -263,10 +272,8 @@ hipError_t launch_setup_bonds(const BondSetupArgs& a, hipStream_t s);
 hipError_t launch_rebuild(const BuildArgs& a, hipStream_t s);
 hipError_t launch_neighbors(const BuildArgs& a, hipStream_t s);
 hipError_t launch_fill_int(int* p, long count, int value, hipStream_t s);
-// out[0] = bits of max |x - xb|^2, out[1] = bits of max |u|^2 over n slots
-// (cleared first; read back as doubles).
-hipError_t launch_disp_check(int n, const double4* xr, const double* xb, const double* uw,
-                             unsigned long long* out, hipStream_t s);
+// p[0 .. count) = 0 (count <= 64)
+hipError_t launch_zero_u64(unsigned long long* p, int count, hipStream_t s);
 
 enum ForceKind {
   FK_PP_PW_FORCES = 0, // zero + update + pp + pw, write F,T

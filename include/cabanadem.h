@@ -402,7 +402,8 @@ int dem_autotune(dem_ctx* ctx, int trial_steps, dem_tune_result* out);
 /* Enables per-launch HIP-event timing of the kernels dem_step launches
  * (0: off).  enable = k > 1 times every k-th force pass only (each timed
  * pass costs the GPU a few microseconds of idle time around its events);
- * passes that sample diagnostics (another kernel variant) are not timed.
+ * passes that sample diagnostics (another kernel variant) or check the
+ * displacements of the automatic cadence (extra reads) are not timed.
  * Every rebuild is timed. */
 int dem_set_timing(dem_ctx* ctx, int enable);
 /* Accumulated milliseconds and launch counts for the fused step kernel

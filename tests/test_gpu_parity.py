@@ -282,6 +282,16 @@ def test_kernel_timing_samples_force_passes(gpu_lib):
         assert np.array_equal(getattr(sa, f), getattr(sb, f)), f
     a.close()
     b.close()
+    # automatic cadence: the passes that check the displacements are not timed
+    c = make_system(st, w, dt=dt, rc=rc, lo=lo, hi=hi, rebuild_every=abi.REBUILD_AUTO,
+                    skin=rc - 2e-3)
+    c.step(5)
+    checks0 = c.counters().displacement_checks
+    c.set_timing(1)
+    c.step(40)
+    checks = c.counters().displacement_checks - checks0
+    assert checks > 0 and c.timing()[1][0] == 40 - checks
+    c.close()
 
 
 def test_automatic_rebuild_cadence_matches_every_step_rebuilds(gpu_lib):
