@@ -737,17 +737,27 @@ __device__ __forceinline__ void force_tile(const StepArgs& a, const int tile, in
       nc = meta_count(a.cmeta_out[i]);
     } else if (PP && pp_on) {
       // Phase 1: contact detection over the neighbour row.
+      constexpr int NB = DEFER ? DEM_DEFER_BATCH : kPhase1Batch;
+      // The first batch's row entries are loaded together with the count
+      // instead of after it: entries below the row capacity max_n are in
+      // bounds whatever the count, and the count masks them afterwards.
+      int jr0[NB];
+#pragma unroll
+      for (int q = 0; q < NB; ++q) jr0[q] = (q < a.max_n) ? a.nbr[q * ld + i] : i;
       nnb = a.nn[i];
       nc = 0;
-      constexpr int NB = DEFER ? DEM_DEFER_BATCH : kPhase1Batch;
       // Batches of NB independent row loads, then gathers, then
       // tests: several gathers in flight per lane instead of one chain.
       for (int k0 = 0; k0 < nnb; k0 += NB) {
         int jb[NB];
         double4 xb[NB];
+        if (k0 == 0) {
 #pragma unroll
-        for (int q = 0; q < NB; ++q)
-          jb[q] = (k0 + q < nnb) ? a.nbr[(k0 + q) * ld + i] : i;
+          for (int q = 0; q < NB; ++q) jb[q] = (q < nnb) ? jr0[q] : i;
+        } else {
+#pragma unroll
+          for (int q = 0; q < NB; ++q) jb[q] = (k0 + q < nnb) ? a.nbr[(k0 + q) * ld + i] : i;
+        }
 #pragma unroll
         for (int q = 0; q < NB; ++q) xb[q] = a.xr_in[jb[q]];
 #pragma unroll
