@@ -39,12 +39,22 @@ namespace {
 #ifndef DEM_FORCE_MIN_WAVES
 #define DEM_FORCE_MIN_WAVES 1
 #endif
+// The corrected arithmetic's force kernels would take about 150 VGPRs (3
+// waves per SIMD); held to 128 (4 waves, a few spilled values) they are 4 %
+// faster on contact-dense beds.  The bonded and sampling variants spill far
+// more at 128 and stay unbounded.
+#ifndef DEM_FORCE_MIN_WAVES_CORRECTED
+#define DEM_FORCE_MIN_WAVES_CORRECTED 4
+#endif
 constexpr int kPhase1Batch = DEM_PHASE1_BATCH; // neighbour gathers issued together
 #ifndef DEM_DETECT_BATCH
 #define DEM_DETECT_BATCH 4
 #endif
 #ifndef DEM_DEFER_MIN_WAVES
 #define DEM_DEFER_MIN_WAVES 1
+#endif
+#ifndef DEM_ROW_PRELOAD
+#define DEM_ROW_PRELOAD 1 // batches of row entries loaded with the count (1 or 2)
 #endif
 #ifndef DEM_DEFER_EARLY_LOADS
 #define DEM_DEFER_EARLY_LOADS 1 // k_defer loads u, omega, m, I before phase 1
@@ -741,9 +751,9 @@ __device__ __forceinline__ void force_tile(const StepArgs& a, const int tile, in
       // The first batch's row entries are loaded together with the count
       // instead of after it: entries below the row capacity max_n are in
       // bounds whatever the count, and the count masks them afterwards.
-      int jr0[NB];
+      int jr0[DEM_ROW_PRELOAD * NB];
 #pragma unroll
-      for (int q = 0; q < NB; ++q) jr0[q] = (q < a.max_n) ? a.nbr[q * ld + i] : i;
+      for (int q = 0; q < DEM_ROW_PRELOAD * NB; ++q) jr0[q] = (q < a.max_n) ? a.nbr[q * ld + i] : i;
       nnb = a.nn[i];
       nc = 0;
       // Batches of NB independent row loads, then gathers, then
@@ -754,6 +764,9 @@ __device__ __forceinline__ void force_tile(const StepArgs& a, const int tile, in
         if (k0 == 0) {
 #pragma unroll
           for (int q = 0; q < NB; ++q) jb[q] = (q < nnb) ? jr0[q] : i;
+        } else if (DEM_ROW_PRELOAD > 1 && k0 == NB) {
+#pragma unroll
+          for (int q = 0; q < NB; ++q) jb[q] = (NB + q < nnb) ? jr0[(DEM_ROW_PRELOAD > 1 ? NB : 0) + q] : i;
         } else {
 #pragma unroll
           for (int q = 0; q < NB; ++q) jb[q] = (k0 + q < nnb) ? a.nbr[(k0 + q) * ld + i] : i;
@@ -969,7 +982,9 @@ __device__ __forceinline__ void force_tile(const StepArgs& a, const int tile, in
 // persistent grid that strides over the tiles; either way tile v is mapped
 // XCD-aware (contiguous tiles per XCD when the grid is a multiple of 8).
 template <bool REF, bool PP, bool PW, int KICK, bool ACCUM, bool DIAG, bool BOND, bool ENS, bool PRE, int B>
-__global__ __launch_bounds__(B, PRE ? DEM_PRE_MIN_WAVES : DEM_FORCE_MIN_WAVES) void k_force(StepArgs a) {
+__global__ __launch_bounds__(B, PRE ? DEM_PRE_MIN_WAVES
+                                 : (!REF && !BOND && !DIAG) ? DEM_FORCE_MIN_WAVES_CORRECTED : DEM_FORCE_MIN_WAVES)
+void k_force(StepArgs a) {
   extern __shared__ int s_cj[]; // [max_c][B] contacts found in phase 1
   if ((int)gridDim.x >= a.ntiles) {
     // one workgroup per tile: no loop, so the compiler keeps the kernel
