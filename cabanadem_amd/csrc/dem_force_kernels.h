@@ -53,9 +53,6 @@ constexpr int kPhase1Batch = DEM_PHASE1_BATCH; // neighbour gathers issued toget
 #ifndef DEM_DEFER_MIN_WAVES
 #define DEM_DEFER_MIN_WAVES 1
 #endif
-#ifndef DEM_ROW_PRELOAD
-#define DEM_ROW_PRELOAD 1 // batches of row entries loaded with the count (1 or 2)
-#endif
 #ifndef DEM_DEFER_EARLY_LOADS
 #define DEM_DEFER_EARLY_LOADS 1 // k_defer loads u, omega, m, I before phase 1
 #endif
@@ -751,9 +748,9 @@ __device__ __forceinline__ void force_tile(const StepArgs& a, const int tile, in
       // The first batch's row entries are loaded together with the count
       // instead of after it: entries below the row capacity max_n are in
       // bounds whatever the count, and the count masks them afterwards.
-      int jr0[DEM_ROW_PRELOAD * NB];
+      int jr0[NB];
 #pragma unroll
-      for (int q = 0; q < DEM_ROW_PRELOAD * NB; ++q) jr0[q] = (q < a.max_n) ? a.nbr[q * ld + i] : i;
+      for (int q = 0; q < NB; ++q) jr0[q] = (q < a.max_n) ? a.nbr[q * ld + i] : i;
       nnb = a.nn[i];
       nc = 0;
       // Batches of NB independent row loads, then gathers, then
@@ -764,9 +761,6 @@ __device__ __forceinline__ void force_tile(const StepArgs& a, const int tile, in
         if (k0 == 0) {
 #pragma unroll
           for (int q = 0; q < NB; ++q) jb[q] = (q < nnb) ? jr0[q] : i;
-        } else if (DEM_ROW_PRELOAD > 1 && k0 == NB) {
-#pragma unroll
-          for (int q = 0; q < NB; ++q) jb[q] = (NB + q < nnb) ? jr0[(DEM_ROW_PRELOAD > 1 ? NB : 0) + q] : i;
         } else {
 #pragma unroll
           for (int q = 0; q < NB; ++q) jb[q] = (k0 + q < nnb) ? a.nbr[(k0 + q) * ld + i] : i;
