@@ -79,10 +79,12 @@ TRAFFIC_RECORDS = {  # (workload, physics mode) -> committed rocprofv3 counter s
 }
 
 
-def measured_traffic(workload, mode, n):
+def measured_traffic(workload, mode, n, shape=None):
     """HBM bytes per launch of the fused kernel for this workload, mode and
     size, from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled
-    by the copy-kernel calibration recorded with it), or None."""
+    by the copy-kernel calibration recorded with it), or None.  shape: the
+    run's (force pass, tile slots, cell-tile rows); a record measured with
+    another shape is not used."""
     name = TRAFFIC_RECORDS.get((workload, mode))
     if name is None:
         return None
@@ -91,6 +93,8 @@ def measured_traffic(workload, mode, n):
         return None
     d = json.loads(p.read_text())
     if d.get("spheres") != n or d.get("workload") != workload or d.get("physics_mode") != mode:
+        return None
+    if shape is not None and "shape" in d and tuple(d["shape"]) != tuple(shape):
         return None
     return {"bytes_per_launch": d["bytes_per_launch"], "source": str(p.relative_to(ROOT))}
 
@@ -186,7 +190,7 @@ def run_bonded(args):
     k_ms = ms[0] / max(launches[0], 1)
     b_alg = W.bytes_bonded_passes(bbar, nbar, cbar)
     achieved = b_alg * n / (k_ms * 1e-3) / 1e9
-    traffic = measured_traffic("bonded_block", args.mode, n)
+    traffic = measured_traffic("bonded_block", args.mode, n, (0,) + tuple(sysm.shape()))
     out = {
         "metric": "FP64 bonded-DEM particle-steps/s (1M-sphere tensile block)",
         "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
@@ -244,6 +248,7 @@ def cpu_baseline_ensemble(wl, nsys_sample, steps):
 
 def run_ensemble(args):
     """Batched ensemble: 256 independent systems, one fused launch per step."""
+    from cabanadem_amd import abi
     from cabanadem_amd import workloads as W
 
     t0 = time.perf_counter()
@@ -258,12 +263,25 @@ def run_ensemble(args):
         f"{time.perf_counter() - t0:.1f} s")
     if args.warmup > 0:
         sysm.step(args.warmup)
+    # The autotuner times its candidates from the state it is given, and the
+    # systems start apart and collide later in the run: a second context of
+    # the same ensemble, advanced to the middle of the timed run, is tuned,
+    # and the timed context takes its choice.
     t0 = time.perf_counter()
-    tune = sysm.autotune(trial_steps=80)
-    log(f"[bench] autotune: rebuild every {tune.rebuild_every}, grid {tune.grid_blocks}, graph "
-        f"{tune.use_graph}, force pass {tune.force_pass}, tiles {tune.block} slots / "
-        f"{tune.tile_rows} cell rows, {tune.ms_per_step:.4f} ms/step ({tune.n_trials} trials, "
-        f"{time.perf_counter() - t0:.1f} s)")
+    tune_at = args.warmup + args.steps // 2
+    tuner = wl.system(mode=args.mode)
+    tuner.set_params(rebuild_every=abi.REBUILD_AUTO)  # advanced safely whatever the speeds
+    tuner.step(tune_at)
+    tune = tuner.autotune(trial_steps=80)
+    sysm.set_params(rebuild_every=tune.rebuild_every)
+    sysm.set_launch(tune.grid_blocks)
+    sysm.set_graph(bool(tune.use_graph))
+    sysm.set_force_pass(tune.force_pass)
+    sysm.set_shape(tune.block, tune.tile_rows)
+    log(f"[bench] autotune at step {tune_at}: rebuild every {tune.rebuild_every}, grid "
+        f"{tune.grid_blocks}, graph {tune.use_graph}, force pass {tune.force_pass}, tiles "
+        f"{tune.block} slots / {tune.tile_rows} cell rows, {tune.ms_per_step:.4f} ms/step "
+        f"({tune.n_trials} trials, {time.perf_counter() - t0:.1f} s)")
     # kernel duration and contact statistics: short eager windows right before
     # and right after the timed run (the systems' spheres start moving apart
     # and colliding, so the kernel slows over the run; the two windows bracket
@@ -298,7 +316,8 @@ def run_ensemble(args):
     nbar, cbar, wbar = (0.5 * (a + b) for a, b in zip(stats0, stats1))
     b_alg = W.bytes_reference_passes(nbar, cbar, wbar)
     achieved = b_alg * n / (k_ms * 1e-3) / 1e9
-    traffic = None if args.ensemble_dense else measured_traffic("ensemble", args.mode, n)
+    traffic = None if args.ensemble_dense else measured_traffic(
+        "ensemble", args.mode, n, (tune.force_pass, tune.block, tune.tile_rows))
     out = {
         "metric": "FP64 DEM particle-steps/s, batched ensemble of 256 systems",
         "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
@@ -311,7 +330,7 @@ def run_ensemble(args):
                                   "; dense: spacing 1.9 r_max, u ~ N(0, 0.5 m/s), walls 0.95 "
                                   "r_max out" if args.ensemble_dense else ""),
                    "spheres": n, "systems": len(wl.systems), "dt": wl.dt,
-                   "autotuned": {"rebuild_every": tune.rebuild_every,
+                   "autotuned": {"tuned_at_step": tune_at, "rebuild_every": tune.rebuild_every,
                                  "grid_blocks": tune.grid_blocks, "graph": tune.use_graph,
                                  "force_pass": tune.force_pass, "block": tune.block,
                                  "tile_rows": tune.tile_rows, "trials": tune.trials()},
@@ -335,7 +354,8 @@ def run_ensemble(args):
                        "kernel_share": k_ms / (elapsed / args.steps * 1e3)},
     }
     sysm.close()
-    del sysm
+    tuner.close()
+    del sysm, tuner
     out["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline_ensemble(wl, 64, 500)
     emit(out)
 
@@ -659,7 +679,8 @@ def main():
     # the rebuild term at the cadence the timed window actually ran
     k_eff = args.steps / max(launches[1], 1)
     b_ref = W.bytes_reference_model(nbar, cbar, wbar, k_eff)
-    traffic = measured_traffic(args.workload, args.mode, n)
+    traffic = measured_traffic(args.workload, args.mode, n,
+                               ({"fused": 0, "split": 1, "wave": 2}[args.force_pass],) + tuple(sysm.shape()))
 
     out = {
         "metric": "FP64 DEM particle-steps/s at 4M spheres",

@@ -91,3 +91,7 @@ def test_measured_traffic_records_match_workload_mode_and_size():
     assert packed is not None and packed["bytes_per_launch"] > 2 * rec["bytes_per_launch"]
     assert bench.measured_traffic("plain_packed", 0, 4_000_000) is None
     assert bench.measured_traffic("bonded_block", 1, 1_000_000) is not None
+    # and only for the force-pass shape and tile shape it was measured with
+    assert bench.measured_traffic("plain_large", 0, 4_000_000, (0, 128, 8)) is not None
+    assert bench.measured_traffic("plain_large", 0, 4_000_000, (2, 128, 8)) is None
+    assert bench.measured_traffic("plain_large", 0, 4_000_000, (0, 256, 8)) is None
