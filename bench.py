@@ -59,9 +59,9 @@ def emit(out):
 def diag_interval(steps):
     """Sampling interval of the timed window's diagnostics: at least one row
     whatever the window's length, about ten in a long one, and no closer
-    than every 10 steps (a sampled step runs the force kernel's diagnostic
+    than every 20 steps (a sampled step runs the force kernel's diagnostic
     variant and a reduction)."""
-    return max(1, steps // 10, min(steps, 10))
+    return max(1, steps // 10, min(steps, 20))
 
 
 def timing_interval(steps):
