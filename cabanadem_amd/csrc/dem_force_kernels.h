@@ -1,9 +1,8 @@
 // dem_force_kernels.h -- the force kernels and their launch templates,
 // shared by the translation units that instantiate them for one force-tile
 // size each (dem_force.hip: 128 and the bonded kernels; dem_force_b64.hip,
-// dem_force_b256.hip), so that they compile in parallel.
-//
-// dem_force.hip -- contact forces and velocity-Verlet stages on the GPU.
+// dem_force_b256.hip), so that they compile in parallel: contact forces and
+// velocity-Verlet stages on the GPU.
 //
 // Reference paths are relative to the ExaPhysics/CabanaDEM root.
 //   pp contact    src/CabanaDEM_Force.hpp:73-327  (Hertz normal + tangential spring)
