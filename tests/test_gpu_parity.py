@@ -333,6 +333,37 @@ def test_automatic_rebuild_cadence_matches_every_step_rebuilds(gpu_lib):
     sysm.close()
 
 
+def test_automatic_cadence_is_the_same_in_every_force_pass_shape(gpu_lib):
+    """The displacement check of the automatic cadence runs inside whichever
+    kernels write the step (the fused kernel, the gather pass and its work
+    lists, the tile-spread pass): every shape and tile size takes the same
+    rebuild decisions and ends bitwise equal."""
+    st, w, lo, hi = granular_box(10, 10, 10, spacing_r=2.05, jitter_r=0.02, vsigma=0.5, seed=23)
+    dt, rc = 1e-6, 2.4e-3
+    systems = []
+    for block in (128, 64, 256):
+        for shape in (abi.FORCE_PASS_FUSED, abi.FORCE_PASS_SPLIT, abi.FORCE_PASS_WAVE):
+            s_ = make_system(st, w, dt=dt, rc=rc, lo=lo, hi=hi, gravity=(0., 0., -9.81),
+                             rebuild_every=abi.REBUILD_AUTO, skin=rc - 2e-3)
+            s_.set_force_pass(shape)
+            s_.set_shape(block, 0)
+            systems.append(s_)
+    for chunk in (1, 299, 400):
+        for s_ in systems:
+            s_.step(chunk)
+    c0 = systems[0].counters()
+    assert c0.rebuilds >= 2 and c0.displacement_checks > c0.rebuilds
+    ref = systems[0].download()
+    for s_ in systems[1:]:
+        c = s_.counters()
+        assert (c.rebuilds, c.displacement_checks) == (c0.rebuilds, c0.displacement_checks)
+        got = s_.download()
+        for f in ("x", "u", "omega", "force", "torque", "ss_idx", "ss_cnt"):
+            assert np.array_equal(getattr(ref, f), getattr(got, f)), f
+    for s_ in systems:
+        s_.close()
+
+
 def test_automatic_rebuild_cadence_needs_a_skin(gpu_lib):
     st, w, lo, hi = granular_box(4, 4, 4, seed=2)
     with pytest.raises(abi.DemError):
