@@ -102,6 +102,7 @@ struct dem_ctx {
   unsigned long long* d_disp = nullptr; // [2] max |x - xb|^2, max |u|^2 (bits)
   unsigned long long* h_disp = nullptr; // pinned host copy of d_disp (a pageable
                                         // destination costs a staged copy per check)
+  hipEvent_t ev_disp = nullptr;         // recorded behind the copy into h_disp
   int auto_left = 0;         // force passes until the next displacement check
   int auto_first = 0;        // interval planned right after the last build
   int64_t auto_since = 0;    // step at which the positions were last set
@@ -195,6 +196,7 @@ struct dem_ctx {
     hipFree(flags); hipFree(diag_part); hipFree(diag_rows); hipFree(d_disp);
     hipFree(d_work); hipFree(d_work_tile);
     if (h_disp) hipHostFree(h_disp);
+    if (ev_disp) hipEventDestroy(ev_disp);
     if (h_diag_rows) hipHostFree(h_diag_rows);
     if (h_flags) hipHostFree(h_flags);
     for (int b = 0; b < 3; ++b) { hipFree(d_tiles_b[b]); hipFree(d_tile0_b[b]); }
@@ -473,9 +475,10 @@ struct dem_ctx {
 
   // Automatic rebuild cadence.  Rows built with cutoff >= 2 r_max + skin
   // hold every pair that can touch as long as no particle has moved more
-  // than skin/2 since the build.  auto_check measures the largest
-  // displacement since the build, d, and the largest speed, v (one kernel,
-  // one synchronisation), and plans the next check M force passes ahead so
+  // than skin/2 since the build.  A check measures the largest
+  // displacement since the build, d, and the largest speed, v (inside a
+  // step's force pass, read on the host once the next pass is queued), and
+  // plans the next check M force passes ahead so
   // that d + 2 v dt (M + 1) stays within 0.9 skin/2: speeds may double in
   // the meantime, and a call boundary adds a drift.  M is at most the
   // number of force passes since the positions were set, so that under a
@@ -517,24 +520,35 @@ struct dem_ctx {
     const double m = std::floor(budget / per_step) - 1.;
     return m < 1. ? 0 : (m > cap ? cap : (int)m);
   }
-  // The maxima that the step's force pass has just computed (StepArgs::disp)
-  // decide the next interval.
-  int auto_check() {
+  // A check is read in two parts: the maxima that the step's force pass has
+  // computed (StepArgs::disp) are copied to pinned memory behind it, and read
+  // once the next force pass is queued too, so the GPU runs that pass while
+  // the host waits and plans (no idle gap).  The plan then starts one pass
+  // later: that pass's motion, at most 2 v dt, comes off the budget first.
+  int auto_check_enqueue() {
     ++auto_checks;
     hipError_t e = hipSuccess;
     if (!h_disp) e = hipHostMalloc(reinterpret_cast<void**>(&h_disp), 2 * sizeof(unsigned long long),
                                    hipHostMallocDefault);
+    if (e == hipSuccess && !ev_disp) e = hipEventCreateWithFlags(&ev_disp, hipEventDisableTiming);
     if (e == hipSuccess)
       e = hipMemcpyAsync(h_disp, d_disp, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e == hipSuccess) e = hipEventRecord(ev_disp, stream);
+    return e == hipSuccess ? DEM_OK : hip_fail(e, "displacement check");
+  }
+  // passes_after: force passes queued after the one that computed the maxima
+  int auto_check_read(int passes_after) {
+    hipError_t e = hipEventSynchronize(ev_disp);
     if (e != hipSuccess) return hip_fail(e, "displacement check");
     double d2, v2;
     std::memcpy(&d2, &h_disp[0], sizeof(double));
     std::memcpy(&v2, &h_disp[1], sizeof(double));
     auto_v = std::max(std::sqrt(v2), 2. * wall_speed()); // NaN stays NaN
     if (v2 != v2) auto_v = v2;
-    const int m = auto_interval(0.45 * cfg.skin - std::sqrt(d2)); // NaN: 0, rebuild
-    auto_due = m < 1 || 4 * m < auto_first;
+    const double gone = passes_after * 2. * auto_v * cfg.dt;
+    const int m = auto_interval(0.45 * cfg.skin - std::sqrt(d2) - gone); // NaN: 0, rebuild
+    // the quarter rule counts from the measurement, as the first interval does
+    auto_due = m < 1 || 4 * (m + passes_after) < auto_first;
     auto_left = m;
     return DEM_OK;
   }
@@ -1386,18 +1400,20 @@ static int enqueue_steps(dem_ctx* c, int nsteps, int* row) {
   StepArgs a0 = c->step_args();
   hipError_t e = launch_kick(a0, c->xr[c->cP], c->uw[c->cP], c->ft[c->cS], 1, 1, c->stream);
   if (e != hipSuccess) return c->hip_fail(e, "stage1/2 launch");
+  bool pending = false; // the previous pass computed a check, to be read after this pass
   for (int s = 0; s < nsteps; ++s) {
     const int64_t gstep = c->step;
     if (!c->list_valid || (autoK ? c->auto_due : gstep % K == 0)) {
       int r = c->rebuild(!c->built_once, c->list_valid, false);
       if (r) return r;
       if (autoK) c->auto_after_rebuild();
+      pending = false; // measured against the old build: dropped
     }
     const bool sample = c->diag_every > 0 && gstep % c->diag_every == 0;
     const bool last = (s == nsteps - 1);
     StepArgs a = c->step_args();
     // the displacement check after this step, computed by its own force pass
-    const bool check = autoK && c->auto_left <= 1;
+    const bool check = autoK && !pending && c->auto_left <= 1;
     if (check) {
       e = launch_zero_u64(c->d_disp, 2, c->stream);
       if (e != hipSuccess) return c->hip_fail(e, "displacement check clear");
@@ -1416,9 +1432,16 @@ static int enqueue_steps(dem_ctx* c, int nsteps, int* row) {
       ++*row;
     }
     c->step++;
-    if (autoK && --c->auto_left <= 0)
-      if (int r = c->auto_check()) return r;
+    if (pending) {
+      if (int r = c->auto_check_read(1)) return r;
+      pending = false;
+    } else if (autoK && --c->auto_left <= 0) {
+      if (int r = c->auto_check_enqueue()) return r;
+      pending = true;
+    }
   }
+  if (pending)
+    if (int r = c->auto_check_read(0)) return r;
   return DEM_OK;
 }
 
