@@ -492,6 +492,9 @@ struct dem_ctx {
   // three).  The rebuild's own skin check stays on behind this as the guard
   // (a violent impulse can still trip it).
   static constexpr int kAutoMaxInterval = 1000;
+#ifndef DEM_AUTO_SHRINK
+#define DEM_AUTO_SHRINK 4 // rebuild once the plan is below 1/DEM_AUTO_SHRINK of the first (0: never)
+#endif
   double wall_speed() const {
     if (cfg.physics_mode != DEM_MODE_CORRECTED) return 0.; // walls never move in reference mode
     double vmax = 0.;
@@ -548,7 +551,7 @@ struct dem_ctx {
     const double gone = passes_after * 2. * auto_v * cfg.dt;
     const int m = auto_interval(0.45 * cfg.skin - std::sqrt(d2) - gone); // NaN: 0, rebuild
     // the quarter rule counts from the measurement, as the first interval does
-    auto_due = m < 1 || 4 * (m + passes_after) < auto_first;
+    auto_due = m < 1 || (DEM_AUTO_SHRINK > 0 && DEM_AUTO_SHRINK * (m + passes_after) < auto_first);
     auto_left = m;
     return DEM_OK;
   }
